@@ -1,0 +1,187 @@
+/*
+ * oidt_fdtd.h -- C ABI of the MI355X FDTD time-stepping library (liboidt_fdtd.so).
+ *
+ * The library replaces the device half of the reference driver
+ * FDTD/GPU/FDTD_2D_TM/s-parameter_extraction/fdtd_main.py (and its TE / 3D
+ * twins): the host->device copies of :216-261, the per-step kernel launches of
+ * the time loop :275-304 (TE :278-307, 3D FDTD_3D/.../fdtd_main.py:357-406) and
+ * the copy-back of the running-DFT accumulators :349-357.  Everything else --
+ * parameter derivation, geometry, CPML grading, waveform, extraction, file
+ * output -- stays in the Python host code, which hands the library exactly the
+ * arrays the reference hands its Numba kernels (fdtd_funcs.py signatures).
+ *
+ * Conventions
+ *   - All functions return 0 (OIDT_OK) on success or a negative oidt_status;
+ *     oidt_last_error() then returns a thread-local message.
+ *   - Arrays are C-order ([x][y] or [x][y][z], z fastest), in the run's
+ *     precision (float or double) unless stated otherwise.
+ *   - The library owns all device memory; host buffers are read or written only
+ *     during the call that receives them.
+ *   - A handle is used from one host thread at a time; distinct handles are
+ *     independent.  No C++ exception crosses this boundary.
+ */
+#ifndef OIDT_FDTD_H
+#define OIDT_FDTD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OIDT_ABI_VERSION 1
+
+typedef enum {
+  OIDT_OK = 0,
+  OIDT_EINVAL = -1,   /* invalid argument or inconsistent description */
+  OIDT_EHIP = -2,     /* HIP runtime error */
+  OIDT_ENOMEM = -3,   /* device allocation failed */
+  OIDT_ESTATE = -4,   /* call not valid in the handle's current state */
+  OIDT_EFULL = -5     /* batch has no free slot */
+} oidt_status;
+
+/* Update schemes: FDTD_2D_TM, FDTD_2D_TE, FDTD_3D of the reference. */
+typedef enum { OIDT_TMZ = 0, OIDT_TEZ = 1, OIDT_3D = 2 } oidt_scheme;
+
+/* Precision of fields, psi and coefficients ("precision" of user_config.py:65).
+ * DFT accumulators are always double. */
+typedef enum { OIDT_F32 = 0, OIDT_F64 = 1 } oidt_precision;
+
+/* Field components (index into the library's field table). */
+typedef enum {
+  OIDT_EX = 0, OIDT_EY = 1, OIDT_EZ = 2, OIDT_HX = 3, OIDT_HY = 4, OIDT_HZ = 5
+} oidt_field;
+
+/* One simulation run: the scalars of one `python fdtd_main.py <line> <f|b> <i|r>`
+ * invocation, in grid units.  Update bounds follow fdtd_main.py:113-117:
+ * ib = jb = kb = offset, ie = nx - offset, ... */
+typedef struct oidt_run_desc {
+  int32_t abi_version;  /* must equal OIDT_ABI_VERSION */
+  int32_t scheme;       /* oidt_scheme */
+  int32_t precision;    /* oidt_precision */
+  int32_t nx, ny, nz;   /* nz = 1 for the 2D schemes */
+  int32_t num_cpml;     /* CPML thickness N (user_config num_cpml) */
+  int32_t offset;       /* PEC border, 2 in the reference */
+  int32_t nt;           /* number of time steps */
+  int32_t nf;           /* number of DFT frequencies (sparam_num_freqs) */
+  int32_t src_field;    /* oidt_field driven by J_SRC: TMz Ez, TEz Hz, 3D Ey (TE) or Hy (TM) */
+  int32_t src_box[6];   /* x0,x1,y0,y1,z0,z1: half-open box of update_source (z0=0,z1=1 in 2D) */
+  int32_t port[2];      /* port planes: p1x,p2x (2D) or p1z,p2z (3D) */
+  double dx;            /* delta_x, already rounded to the run precision */
+  double dt;            /* delta_t, already rounded to the run precision */
+  double db;            /* DB = delta_t / mu as passed to the H kernels */
+} oidt_run_desc;
+
+/* Arrays of one run, as the reference hands them to its kernels. */
+typedef struct oidt_run_inputs {
+  const void* cb[3];       /* E-component update maps dt/eps, [nx][ny][nz]:
+                              TMz {CBZ}, TEz {CBX, CBY}, 3D {CBX, CBY, CBZ} */
+  const void* kappa_e[3];  /* KXE, KYE, KZE: kappa*dx per axis (KZE NULL in 2D) */
+  const void* kappa_h[3];  /* KXH, KYH, KZH */
+  const void* be;          /* BE[num_cpml] */
+  const void* ce;          /* CE[num_cpml] */
+  const void* bh;          /* BH[num_cpml] */
+  const void* ch;          /* CH[num_cpml] */
+  const void* jsrc;        /* J_SRC[nt] */
+  const double* freqs;     /* sparam_freqs[nf], Hz */
+} oidt_run_inputs;
+
+typedef struct oidt_device_info {
+  char name[64];
+  char arch[32];
+  int32_t compute_units;
+  int32_t device_id;
+  uint64_t hbm_total_bytes;
+  uint64_t hbm_free_bytes;
+} oidt_device_info;
+
+typedef struct oidt_solver oidt_solver;
+typedef struct oidt_batch oidt_batch;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* ABI version compiled into the library. */
+int32_t oidt_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* oidt_last_error(void);
+
+/* Device 0 properties and free/total HBM (admission by footprint).
+ * Replaces cuda.current_context().get_memory_info(), fdtd_main.py:315. */
+int oidt_device_info_get(oidt_device_info* out);
+
+/* Device bytes a run will hold while resident (fields, psi, material index,
+ * coefficient tables, DFT ring and accumulators).  Replaces the estimate of
+ * fdtd_main.py:207-213. */
+int oidt_run_footprint(const oidt_run_desc* desc, uint64_t* bytes);
+
+/* Samples per DFT chunk: steps advance in chunks of this many. */
+int32_t oidt_chunk_steps(void);
+
+/* ---- single run ---------------------------------------------------------- */
+
+/* Allocate and upload one run (fields and psi zeroed).
+ * Replaces fdtd_main.py:216-261 (cuda.to_device of every array). */
+int oidt_solver_create(const oidt_run_desc* desc, const oidt_run_inputs* in, oidt_solver** out);
+
+/* Enqueue up to nsteps further steps (rounded up to whole chunks, never past
+ * nt).  Asynchronous.  Replaces the body of the time loop fdtd_main.py:275-304. */
+int oidt_solver_advance(oidt_solver* s, int32_t nsteps);
+
+/* Wait for enqueued work. */
+int oidt_solver_sync(oidt_solver* s);
+
+/* Steps completed once the enqueued work has finished. */
+int oidt_solver_steps(oidt_solver* s, int32_t* steps);
+
+/* Copy the running DFT of one port component to the host: re/im are
+ * [nf][cells] doubles over the reference window ([N, ny-N) in 2D,
+ * [N, nx-N) x [N, ny-N) in 3D).  comp: 0 = E sample, 1 = H sample (2D);
+ * 0 = Ex, 1 = Ey, 2 = Hx, 3 = Hy (3D).  port: 0 or 1.
+ * Replaces the copy_to_host calls of fdtd_main.py:349-357. */
+int oidt_solver_read_dft(oidt_solver* s, int32_t port, int32_t comp, double* re, double* im);
+
+/* Copy one field component ([nx][ny][nz], run precision) to the host. */
+int oidt_solver_read_field(oidt_solver* s, int32_t field, void* host, uint64_t nbytes);
+
+/* Free the run's device memory.  Replaces cuda.close(), fdtd_main.py:388. */
+int oidt_solver_destroy(oidt_solver* s);
+
+/* ---- batch: many runs, one launch per half-step -------------------------- */
+/* No counterpart in the reference (which runs one simulation per process,
+ * simulation_launcher.py:26-35).  All runs of a batch share scheme and
+ * precision; grid sizes, step counts and frequency counts may differ.  Runs can
+ * be added and removed between advance calls (continuous batching). */
+
+int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt_batch** out);
+
+/* Upload a run into a free slot; it starts at step 0 with the next advance. */
+int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inputs* in, int32_t* slot);
+
+/* Enqueue nchunks chunks for every resident unfinished run.  Asynchronous. */
+int oidt_batch_advance(oidt_batch* b, int32_t nchunks);
+
+int oidt_batch_sync(oidt_batch* b);
+
+/* Non-blocking progress: steps completed by the GPU for a slot (finished
+ * when equal to the run's nt). */
+int oidt_batch_poll(oidt_batch* b, int32_t slot, int32_t* steps_done);
+
+int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp, double* re, double* im);
+int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host, uint64_t nbytes);
+
+/* Release a slot (its device memory is returned in stream order). */
+int oidt_batch_remove(oidt_batch* b, int32_t slot);
+
+/* Launch geometry knobs for the autotuner: x-chunk depth of the stencil tiles
+ * (cells marched per thread) and the persistent grid size (0 = automatic). */
+int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks);
+
+int oidt_batch_destroy(oidt_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OIDT_FDTD_H */

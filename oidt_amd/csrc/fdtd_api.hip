@@ -1,0 +1,811 @@
+// C ABI of liboidt_fdtd.so (declared in include/oidt_fdtd.h).
+//
+// A batch owns a HIP stream (default priority), a device table of run
+// descriptors (Slot<T>), a tile list for the stencil kernels and a work list
+// for the DFT kernel.  One chunk = kChunk x (E launch, H launch) + DFT launch +
+// advance launch, captured once per batch composition as a hipGraph and
+// replayed.  A single run (oidt_solver) is a batch with one slot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/oidt_fdtd.h"
+#include "fdtd_kernels.h"
+
+using namespace oidt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(OIDT_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));       \
+  } while (0)
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Pinned staging buffers for asynchronous uploads; a buffer is reused once the
+// event recorded after its copy has completed.
+struct Staging {
+  struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  std::vector<Buf> bufs;
+  int get(size_t bytes, void** out) {
+    for (size_t i = 0; i < bufs.size(); ++i) {
+      Buf& b = bufs[i];
+      if (b.pending) {
+        if (hipEventQuery(b.ev) == hipSuccess) b.pending = false;
+        else continue;
+      }
+      if (b.cap >= bytes) { *out = b.p; return (int)i; }
+    }
+    Buf b;
+    b.cap = std::max(bytes, (size_t)1 << 20);
+    if (hipHostMalloc(&b.p, b.cap, hipHostMallocDefault) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&b.ev, hipEventDisableTiming) != hipSuccess) return -1;
+    bufs.push_back(b);
+    *out = b.p;
+    return (int)bufs.size() - 1;
+  }
+  int submit(int id, hipStream_t st) {
+    Buf& b = bufs[id];
+    if (hipEventRecord(b.ev, st) != hipSuccess) return -1;
+    b.pending = true;
+    return 0;
+  }
+  void release() {
+    for (auto& b : bufs) {
+      if (b.ev) hipEventSynchronize(b.ev), hipEventDestroy(b.ev);
+      if (b.p) hipHostFree(b.p);
+    }
+    bufs.clear();
+  }
+};
+
+struct SlotHost {
+  bool used = false;
+  oidt_run_desc desc{};
+  void* dmem = nullptr;
+  size_t bytes = 0;
+  std::vector<unsigned char> slot;  // host image of Slot<T>
+  int64_t admit_chunk = 0;          // batch chunk counter when admitted
+  int nplanes = 0, nq = 0, wcells = 0, pcells = 0, wyn = 0;
+};
+
+int field_count(int scheme) { return scheme == SCHEME_3D ? 6 : 3; }
+
+bool field_used(int scheme, int f) {
+  if (scheme == SCHEME_3D) return true;
+  if (scheme == SCHEME_TMZ) return f == F_EZ || f == F_HX || f == F_HY;
+  return f == F_EX || f == F_EY || f == F_HZ;
+}
+
+// psi (component, axis) pairs present per scheme, from the reference allocations.
+std::vector<std::pair<int, int>> psi_pairs(int scheme) {
+  if (scheme == SCHEME_TMZ) return {{F_EZ, 0}, {F_EZ, 1}, {F_HX, 1}, {F_HY, 0}};
+  if (scheme == SCHEME_TEZ) return {{F_EX, 1}, {F_EY, 0}, {F_HZ, 0}, {F_HZ, 1}};
+  return {{F_EX, 1}, {F_EX, 2}, {F_EY, 0}, {F_EY, 2}, {F_EZ, 0}, {F_EZ, 1},
+          {F_HX, 1}, {F_HX, 2}, {F_HY, 0}, {F_HY, 2}, {F_HZ, 0}, {F_HZ, 1}};
+}
+
+size_t psi_elems(const oidt_run_desc& d, int axis) {
+  const size_t nx = d.nx, ny = d.ny, nz = d.nz, N = d.num_cpml;
+  if (axis == 0) return N * ny * nz;
+  if (axis == 1) return nx * N * nz;
+  return nx * ny * N;
+}
+
+int validate(const oidt_run_desc* d) {
+  if (!d) return fail(OIDT_EINVAL, "null run description");
+  if (d->abi_version != OIDT_ABI_VERSION) return fail(OIDT_EINVAL, "abi_version mismatch");
+  if (d->scheme < 0 || d->scheme > 2) return fail(OIDT_EINVAL, "unknown scheme");
+  if (d->precision != OIDT_F32 && d->precision != OIDT_F64) return fail(OIDT_EINVAL, "unknown precision");
+  const bool is3d = d->scheme == SCHEME_3D;
+  if (!is3d && d->nz != 1) return fail(OIDT_EINVAL, "2D runs need nz == 1");
+  const int N = d->num_cpml, o = d->offset;
+  if (N < 2 || o < 1 || N < o) return fail(OIDT_EINVAL, "need num_cpml >= offset >= 1 and num_cpml >= 2");
+  const int dims[3] = {d->nx, d->ny, d->nz};
+  for (int a = 0; a < (is3d ? 3 : 2); ++a) {
+    // both CPML slabs must fit inside the update range without overlapping
+    if (dims[a] - 2 * o < 2 * N + 1) return fail(OIDT_EINVAL, "grid too small for the CPML thickness");
+  }
+  if ((double)d->nx * d->ny * d->nz > 4.0e9) return fail(OIDT_EINVAL, "grid larger than 2^32 cells");
+  if (d->nt < 0 || d->nf < 0) return fail(OIDT_EINVAL, "negative nt or nf");
+  const int plim = is3d ? d->nz : d->nx;
+  for (int p = 0; p < 2; ++p)
+    if (d->port[p] < 0 || d->port[p] + 1 >= plim) return fail(OIDT_EINVAL, "port plane outside the grid");
+  if (d->src_field < -1 || d->src_field > 5) return fail(OIDT_EINVAL, "bad src_field");
+  if (d->src_field >= 0) {
+    const int allowed = d->scheme == SCHEME_TMZ ? F_EZ : (d->scheme == SCHEME_TEZ ? F_HZ : -2);
+    if (is3d ? (d->src_field != F_EY && d->src_field != F_HY) : d->src_field != allowed)
+      return fail(OIDT_EINVAL, "src_field not driven by this scheme in the reference");
+    for (int a = 0; a < 3; ++a)
+      if (d->src_box[2 * a] > d->src_box[2 * a + 1]) return fail(OIDT_EINVAL, "inverted source box");
+    if (is3d && d->src_field == F_HY) {
+      // The fused H half-step applies the Hy source before the H curl.  The
+      // reference applies it before the E-CPML, which reads Hy at
+      // x in [ib, ib+N] u [ie-N, ie] and z in [kb, kb+N] u [ke-N, ke]; the two
+      // orders agree only if the box avoids those cells.
+      const int ib = o, ie = d->nx - o, kb = o, ke = d->nz - o;
+      auto hits = [](int a0, int a1, int b0, int b1) { return a0 <= b1 && b0 < a1; };
+      const int* s = d->src_box;
+      if (hits(s[0], s[1], ib, ib + N) || hits(s[0], s[1], ie - N, ie) ||
+          hits(s[4], s[5], kb, kb + N) || hits(s[4], s[5], ke - N, ke))
+        return fail(OIDT_EINVAL, "3D TM source box overlaps the E-CPML stencil");
+    }
+  }
+  return OIDT_OK;
+}
+
+template <typename T>
+struct Layout {
+  size_t off_f[6], off_psi[36], off_mat, off_cbt, off_ike[3], off_ikh[3], off_be, off_ceo, off_bh, off_cho,
+      off_jsrc, off_ring, off_acc, off_w, total;
+  int nplanes, nq, wcells, pcells, prow, wx0, wx1, wy0, wy1;
+};
+
+template <typename T>
+Layout<T> layout(const oidt_run_desc& d) {
+  Layout<T> L{};
+  const bool is3d = d.scheme == SCHEME_3D;
+  const size_t cells = (size_t)d.nx * d.ny * d.nz;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align_up(off + bytes, 256);
+    return o;
+  };
+  for (int f = 0; f < 6; ++f) L.off_f[f] = field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
+  for (int i = 0; i < 36; ++i) L.off_psi[i] = (size_t)-1;
+  for (auto pr : psi_pairs(d.scheme))
+    for (int side = 0; side < 2; ++side)
+      L.off_psi[psi_id(pr.first, pr.second, side)] = take(psi_elems(d, pr.second) * sizeof(T));
+  L.off_mat = take(cells);
+  L.off_cbt = take(kMatMax * 3 * sizeof(T));
+  for (int a = 0; a < 3; ++a) {
+    const int n = a == 0 ? d.nx : (a == 1 ? d.ny : d.nz);
+    L.off_ike[a] = take(n * sizeof(T));
+    L.off_ikh[a] = take(n * sizeof(T));
+  }
+  L.off_be = take(d.num_cpml * sizeof(T));
+  L.off_ceo = take(d.num_cpml * sizeof(T));
+  L.off_bh = take(d.num_cpml * sizeof(T));
+  L.off_cho = take(d.num_cpml * sizeof(T));
+  L.off_jsrc = take(std::max(d.nt, 1) * sizeof(T));
+  const int N = d.num_cpml;
+  L.wy0 = N;
+  L.wy1 = d.ny - N;
+  L.prow = L.wy1 - L.wy0 + 1;
+  if (is3d) {
+    L.wx0 = N;
+    L.wx1 = d.nx - N;
+    L.pcells = (L.wx1 - L.wx0 + 1) * L.prow;
+    L.wcells = (L.wx1 - L.wx0) * (L.wy1 - L.wy0);
+    L.nplanes = 12;
+    L.nq = 8;
+  } else {
+    L.wx0 = L.wx1 = 0;
+    L.pcells = L.prow;
+    L.wcells = L.wy1 - L.wy0;
+    L.nplanes = 6;
+    L.nq = 4;
+  }
+  L.off_ring = take((size_t)kChunk * L.nplanes * L.pcells * sizeof(T));
+  L.off_acc = take((size_t)L.nq * std::max(d.nf, 1) * L.wcells * 2 * sizeof(double));
+  L.off_w = take(std::max(d.nf, 1) * sizeof(double));
+  L.total = off;
+  return L;
+}
+
+uint64_t footprint(const oidt_run_desc& d) {
+  return d.precision == OIDT_F64 ? layout<double>(d).total : layout<float>(d).total;
+}
+
+}  // namespace
+
+struct oidt_batch {
+  int scheme = 0, precision = 0, max_slots = 0;
+  hipStream_t stream = nullptr;
+  void* d_slots = nullptr;
+  int* d_counts = nullptr;  // [0] stencil tiles, [1] DFT work items
+  int4* d_tiles = nullptr;
+  size_t tiles_cap = 0;
+  int4* d_work = nullptr;
+  size_t work_cap = 0;
+  std::vector<SlotHost> slots;
+  Staging staging;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool dirty = true;
+  int ntiles = 0, nwork = 0;
+  int xchunk = 8;
+  int grid_override = 0;
+  int64_t chunks_enqueued = 0;
+  int64_t chunks_done = 0;
+  std::deque<std::pair<hipEvent_t, int64_t>> marks;
+  std::vector<hipEvent_t> free_events;
+  size_t slot_bytes() const { return precision == OIDT_F64 ? sizeof(Slot<double>) : sizeof(Slot<float>); }
+};
+
+struct oidt_solver {
+  oidt_batch* b = nullptr;
+  int slot = -1;
+};
+
+namespace {
+
+// Device-memory image of one run, built on the host.
+template <typename T>
+int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_run_inputs* in) {
+  const Layout<T> L = layout<T>(d);
+  const size_t cells = (size_t)d.nx * d.ny * d.nz;
+  const int ncb = d.scheme == SCHEME_TMZ ? 1 : (d.scheme == SCHEME_TEZ ? 2 : 3);
+  const int cb_comp[3][3] = {{2, -1, -1}, {0, 1, -1}, {0, 1, 2}};
+  for (int c = 0; c < ncb; ++c)
+    if (!in->cb[c]) return fail(OIDT_EINVAL, "missing cb map");
+  const int naxes = d.scheme == SCHEME_3D ? 3 : 2;
+  for (int a = 0; a < naxes; ++a)
+    if (!in->kappa_e[a] || !in->kappa_h[a]) return fail(OIDT_EINVAL, "missing kappa array");
+  if (!in->be || !in->ce || !in->bh || !in->ch) return fail(OIDT_EINVAL, "missing CPML coefficients");
+  if (d.nt > 0 && !in->jsrc) return fail(OIDT_EINVAL, "missing J_SRC");
+  if (d.nf > 0 && !in->freqs) return fail(OIDT_EINVAL, "missing freqs");
+
+  // Material compression: distinct (cbx, cby, cbz) tuples -> uint8 index.
+  std::vector<uint8_t> mat(cells);
+  std::vector<T> cbt(kMatMax * 3, T(0));
+  {
+    const T* cbp[3] = {nullptr, nullptr, nullptr};
+    for (int c = 0; c < ncb; ++c) cbp[cb_comp[d.scheme][c]] = static_cast<const T*>(in->cb[c]);
+    std::map<std::vector<T>, int> index;
+    std::vector<T> key(3);
+    int last = -1;
+    std::vector<T> last_key;
+    for (size_t e = 0; e < cells; ++e) {
+      for (int c = 0; c < 3; ++c) key[c] = cbp[c] ? cbp[c][e] : T(0);
+      if (last >= 0 && key == last_key) {
+        mat[e] = (uint8_t)last;
+        continue;
+      }
+      auto it = index.find(key);
+      int id;
+      if (it == index.end()) {
+        id = (int)index.size();
+        if (id >= kMatMax) return fail(OIDT_EINVAL, "more than 256 distinct cb tuples");
+        index.emplace(key, id);
+        for (int c = 0; c < 3; ++c) cbt[id * 3 + c] = key[c];
+      } else {
+        id = it->second;
+      }
+      mat[e] = (uint8_t)id;
+      last = id;
+      last_key = key;
+    }
+  }
+  // Reciprocal kappa arrays and folded CPML c / dx.
+  auto recip = [](const void* src, int n, std::vector<T>& out) {
+    out.assign(n, T(0));
+    if (!src) return;
+    const T* s = static_cast<const T*>(src);
+    for (int i = 0; i < n; ++i) out[i] = (T)(1.0 / (double)s[i]);
+  };
+  const int dims[3] = {d.nx, d.ny, d.nz};
+  std::vector<T> ike[3], ikh[3];
+  for (int a = 0; a < 3; ++a) {
+    recip(a < naxes ? in->kappa_e[a] : nullptr, dims[a], ike[a]);
+    recip(a < naxes ? in->kappa_h[a] : nullptr, dims[a], ikh[a]);
+  }
+  const int N = d.num_cpml;
+  std::vector<T> be(N), ceo(N), bh(N), cho(N);
+  for (int i = 0; i < N; ++i) {
+    be[i] = static_cast<const T*>(in->be)[i];
+    bh[i] = static_cast<const T*>(in->bh)[i];
+    ceo[i] = (T)((double)static_cast<const T*>(in->ce)[i] / d.dx);
+    cho[i] = (T)((double)static_cast<const T*>(in->ch)[i] / d.dx);
+  }
+  std::vector<double> w(std::max(d.nf, 1), 0.0);
+  for (int f = 0; f < d.nf; ++f) w[f] = 2.0 * M_PI * in->freqs[f];
+
+  // Device allocation (stream ordered) and zeroing of state.
+  void* dm = nullptr;
+  if (hipMallocAsync(&dm, L.total, b->stream) != hipSuccess)
+    return fail(OIDT_ENOMEM, "device allocation of " + std::to_string(L.total) + " bytes failed");
+  sh.dmem = dm;
+  sh.bytes = L.total;
+  char* base = static_cast<char*>(dm);
+  HIP_TRY(hipMemsetAsync(dm, 0, L.total, b->stream));
+
+  // Stage the constant arrays in pinned memory and upload.
+  struct Piece { size_t off; const void* src; size_t bytes; };
+  std::vector<Piece> pieces = {
+      {L.off_mat, mat.data(), cells},
+      {L.off_cbt, cbt.data(), cbt.size() * sizeof(T)},
+      {L.off_be, be.data(), N * sizeof(T)},
+      {L.off_ceo, ceo.data(), N * sizeof(T)},
+      {L.off_bh, bh.data(), N * sizeof(T)},
+      {L.off_cho, cho.data(), N * sizeof(T)},
+      {L.off_w, w.data(), w.size() * sizeof(double)},
+  };
+  if (d.nt > 0) pieces.push_back({L.off_jsrc, in->jsrc, (size_t)d.nt * sizeof(T)});
+  for (int a = 0; a < 3; ++a) {
+    pieces.push_back({L.off_ike[a], ike[a].data(), ike[a].size() * sizeof(T)});
+    pieces.push_back({L.off_ikh[a], ikh[a].data(), ikh[a].size() * sizeof(T)});
+  }
+  size_t total = 0;
+  for (auto& p : pieces) total += align_up(p.bytes, 64);
+  void* stage = nullptr;
+  const int sid = b->staging.get(total, &stage);
+  if (sid < 0) return fail(OIDT_ENOMEM, "pinned staging allocation failed");
+  size_t so = 0;
+  for (auto& p : pieces) {
+    std::memcpy(static_cast<char*>(stage) + so, p.src, p.bytes);
+    HIP_TRY(hipMemcpyAsync(base + p.off, static_cast<char*>(stage) + so, p.bytes, hipMemcpyHostToDevice, b->stream));
+    so += align_up(p.bytes, 64);
+  }
+  if (b->staging.submit(sid, b->stream) != 0) return fail(OIDT_EHIP, "event record failed");
+
+  // Descriptor.
+  Slot<T> s{};
+  s.nx = d.nx; s.ny = d.ny; s.nz = d.nz; s.N = N;
+  s.ib = s.jb = d.offset; s.kb = d.scheme == SCHEME_3D ? d.offset : 0;
+  s.ie = d.nx - d.offset; s.je = d.ny - d.offset; s.ke = d.scheme == SCHEME_3D ? d.nz - d.offset : 1;
+  s.nt = d.nt; s.nf = d.nf; s.base = 0; s.active = 1; s.scheme = d.scheme;
+  s.src_field = d.src_field;
+  s.sx0 = d.src_box[0]; s.sx1 = d.src_box[1]; s.sy0 = d.src_box[2]; s.sy1 = d.src_box[3];
+  s.sz0 = d.src_box[4]; s.sz1 = d.src_box[5];
+  s.port[0] = d.port[0]; s.port[1] = d.port[1];
+  s.wx0 = L.wx0; s.wx1 = L.wx1; s.wy0 = L.wy0; s.wy1 = L.wy1;
+  s.prow = L.prow; s.pcells = L.pcells; s.wcells = L.wcells; s.nplanes = L.nplanes; s.nq = L.nq;
+  s.db = (T)d.db;
+  s.dt = d.dt;
+  for (int f = 0; f < 6; ++f) s.f[f] = L.off_f[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f[f]);
+  for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi[i]);
+  s.mat = reinterpret_cast<const uint8_t*>(base + L.off_mat);
+  s.cbt = reinterpret_cast<const T*>(base + L.off_cbt);
+  for (int a = 0; a < 3; ++a) {
+    s.ike[a] = reinterpret_cast<const T*>(base + L.off_ike[a]);
+    s.ikh[a] = reinterpret_cast<const T*>(base + L.off_ikh[a]);
+  }
+  s.be = reinterpret_cast<const T*>(base + L.off_be);
+  s.ceo = reinterpret_cast<const T*>(base + L.off_ceo);
+  s.bh = reinterpret_cast<const T*>(base + L.off_bh);
+  s.cho = reinterpret_cast<const T*>(base + L.off_cho);
+  s.jsrc = reinterpret_cast<const T*>(base + L.off_jsrc);
+  s.ring = reinterpret_cast<T*>(base + L.off_ring);
+  s.acc = reinterpret_cast<double*>(base + L.off_acc);
+  s.w = reinterpret_cast<const double*>(base + L.off_w);
+  sh.slot.assign(reinterpret_cast<unsigned char*>(&s), reinterpret_cast<unsigned char*>(&s) + sizeof(s));
+  sh.nplanes = L.nplanes; sh.nq = L.nq; sh.wcells = L.wcells; sh.pcells = L.pcells; sh.wyn = L.wy1 - L.wy0;
+  return OIDT_OK;
+}
+
+int upload_bytes(oidt_batch* b, void* dst, const void* src, size_t bytes) {
+  void* stage = nullptr;
+  const int sid = b->staging.get(bytes, &stage);
+  if (sid < 0) return fail(OIDT_ENOMEM, "pinned staging allocation failed");
+  std::memcpy(stage, src, bytes);
+  HIP_TRY(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, b->stream));
+  if (b->staging.submit(sid, b->stream) != 0) return fail(OIDT_EHIP, "event record failed");
+  return OIDT_OK;
+}
+
+int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
+  if (need <= *cap) return OIDT_OK;
+  size_t nc = std::max(need, *cap * 2);
+  int4* nb = nullptr;
+  if (hipMallocAsync(reinterpret_cast<void**>(&nb), nc * sizeof(int4), b->stream) != hipSuccess)
+    return fail(OIDT_ENOMEM, "tile list allocation failed");
+  if (*buf) HIP_TRY(hipFreeAsync(*buf, b->stream));
+  *buf = nb;
+  *cap = nc;
+  return OIDT_OK;
+}
+
+// Rebuild the tile and DFT work lists for the resident slots and upload them.
+int rebuild_lists(oidt_batch* b) {
+  std::vector<int4> tiles, work;
+  const int xc = b->xchunk;
+  for (int si = 0; si < (int)b->slots.size(); ++si) {
+    const SlotHost& sh = b->slots[si];
+    if (!sh.used) continue;
+    const oidt_run_desc& d = sh.desc;
+    if (d.scheme == SCHEME_3D) {
+      for (int x0 = 0; x0 < d.nx; x0 += xc)
+        for (int y0 = 0; y0 < d.ny; y0 += kBY)
+          for (int z0 = 0; z0 < d.nz; z0 += kBX) tiles.push_back(make_int4(si, x0, y0, z0));
+    } else {
+      for (int x0 = 0; x0 < d.nx; x0 += xc * kBY)
+        for (int y0 = 0; y0 < d.ny; y0 += kBX) tiles.push_back(make_int4(si, x0, y0, 0));
+    }
+    const int cblocks = (sh.wcells + kThreads - 1) / kThreads;
+    const int fblocks = (d.nf + kFB - 1) / kFB;
+    for (int q = 0; q < sh.nq; ++q)
+      for (int cb = 0; cb < cblocks; ++cb)
+        for (int fb = 0; fb < fblocks; ++fb) work.push_back(make_int4(si, q, cb, fb));
+  }
+  int rc;
+  if ((rc = grow(b, &b->d_tiles, &b->tiles_cap, std::max<size_t>(tiles.size(), 1))) != OIDT_OK) return rc;
+  if ((rc = grow(b, &b->d_work, &b->work_cap, std::max<size_t>(work.size(), 1))) != OIDT_OK) return rc;
+  if (!tiles.empty() && (rc = upload_bytes(b, b->d_tiles, tiles.data(), tiles.size() * sizeof(int4))) != OIDT_OK) return rc;
+  if (!work.empty() && (rc = upload_bytes(b, b->d_work, work.data(), work.size() * sizeof(int4))) != OIDT_OK) return rc;
+  int counts[2] = {(int)tiles.size(), (int)work.size()};
+  if ((rc = upload_bytes(b, b->d_counts, counts, sizeof(counts))) != OIDT_OK) return rc;
+  b->ntiles = counts[0];
+  b->nwork = counts[1];
+  return OIDT_OK;
+}
+
+int grid_for(int items, int cap) {
+  int g = std::min(std::max(items, 1), cap);
+  return (g + 7) / 8 * 8;
+}
+
+template <typename T, int XC>
+void launch_stencils(oidt_batch* b, int gs, int tstep) {
+  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
+  dim3 blk(kBX, kBY);
+  switch (b->scheme) {
+    case SCHEME_TMZ:
+      hipLaunchKernelGGL((k_tmz_e<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      hipLaunchKernelGGL((k_tmz_h<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      break;
+    case SCHEME_TEZ:
+      hipLaunchKernelGGL((k_tez_e<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      hipLaunchKernelGGL((k_tez_h<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      break;
+    default:
+      hipLaunchKernelGGL((k_3d_e<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      hipLaunchKernelGGL((k_3d_h<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      break;
+  }
+}
+
+template <typename T>
+int enqueue_chunk(oidt_batch* b) {
+  const int cap = b->grid_override > 0 ? b->grid_override : 8192;
+  const int gs = grid_for(b->ntiles, cap);
+  const int gd = grid_for(b->nwork, 8192);
+  for (int t = 0; t < kChunk; ++t) {
+    switch (b->xchunk) {
+      case 4: launch_stencils<T, 4>(b, gs, t); break;
+      case 16: launch_stencils<T, 16>(b, gs, t); break;
+      case 32: launch_stencils<T, 32>(b, gs, t); break;
+      default: launch_stencils<T, 8>(b, gs, t); break;
+    }
+  }
+  hipLaunchKernelGGL((k_dft<T>), dim3(gd), dim3(kThreads), 0, b->stream,
+                     static_cast<const Slot<T>*>(b->d_slots), b->d_work, b->d_counts);
+  hipLaunchKernelGGL((k_advance<T>), dim3((b->max_slots + 255) / 256), dim3(256), 0, b->stream,
+                     static_cast<Slot<T>*>(b->d_slots), b->max_slots);
+  return hipGetLastError() == hipSuccess ? OIDT_OK : fail(OIDT_EHIP, "kernel launch failed");
+}
+
+int capture_graph(oidt_batch* b) {
+  if (b->gexec) {
+    HIP_TRY(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+  }
+  if (b->graph) {
+    HIP_TRY(hipGraphDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = b->precision == OIDT_F64 ? enqueue_chunk<double>(b) : enqueue_chunk<float>(b);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc != OIDT_OK) return rc;
+  if (e != hipSuccess) return fail(OIDT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  b->dirty = false;
+  return OIDT_OK;
+}
+
+void reap_marks(oidt_batch* b) {
+  while (!b->marks.empty()) {
+    auto& m = b->marks.front();
+    if (hipEventQuery(m.first) != hipSuccess) break;
+    b->chunks_done = m.second;
+    b->free_events.push_back(m.first);
+    b->marks.pop_front();
+  }
+}
+
+int slot_ok(oidt_batch* b, int slot) {
+  if (!b) return fail(OIDT_EINVAL, "null batch");
+  if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return fail(OIDT_EINVAL, "no run in slot");
+  return OIDT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t oidt_abi_version(void) { return OIDT_ABI_VERSION; }
+
+const char* oidt_last_error(void) { return g_err.c_str(); }
+
+int32_t oidt_chunk_steps(void) { return kChunk; }
+
+int oidt_device_info_get(oidt_device_info* out) {
+  if (!out) return fail(OIDT_EINVAL, "null output");
+  std::memset(out, 0, sizeof(*out));
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+  std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+  out->compute_units = p.multiProcessorCount;
+  out->device_id = dev;
+  size_t fr = 0, tot = 0;
+  HIP_TRY(hipMemGetInfo(&fr, &tot));
+  out->hbm_free_bytes = fr;
+  out->hbm_total_bytes = tot;
+  return OIDT_OK;
+}
+
+int oidt_run_footprint(const oidt_run_desc* desc, uint64_t* bytes) {
+  int rc = validate(desc);
+  if (rc != OIDT_OK) return rc;
+  if (!bytes) return fail(OIDT_EINVAL, "null output");
+  *bytes = footprint(*desc);
+  return OIDT_OK;
+}
+
+int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt_batch** out) {
+  if (!out) return fail(OIDT_EINVAL, "null output");
+  *out = nullptr;
+  if (scheme < 0 || scheme > 2) return fail(OIDT_EINVAL, "unknown scheme");
+  if (precision != OIDT_F32 && precision != OIDT_F64) return fail(OIDT_EINVAL, "unknown precision");
+  if (max_slots < 1 || max_slots > 4096) return fail(OIDT_EINVAL, "max_slots out of range");
+  oidt_batch* b = new (std::nothrow) oidt_batch();
+  if (!b) return fail(OIDT_ENOMEM, "host allocation failed");
+  b->scheme = scheme;
+  b->precision = precision;
+  b->max_slots = max_slots;
+  b->slots.resize(max_slots);
+  auto bad = [&](hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    oidt_batch_destroy(b);
+    return OIDT_EHIP;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "hipStreamCreate");
+  if ((e = hipMalloc(&b->d_slots, (size_t)max_slots * b->slot_bytes())) != hipSuccess) return bad(e, "hipMalloc slots");
+  if ((e = hipMemsetAsync(b->d_slots, 0, (size_t)max_slots * b->slot_bytes(), b->stream)) != hipSuccess) return bad(e, "hipMemset slots");
+  if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_counts), 2 * sizeof(int))) != hipSuccess) return bad(e, "hipMalloc counts");
+  if ((e = hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), b->stream)) != hipSuccess) return bad(e, "hipMemset counts");
+  *out = b;
+  return OIDT_OK;
+}
+
+int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inputs* in, int32_t* slot) {
+  if (!b || !in || !slot) return fail(OIDT_EINVAL, "null argument");
+  int rc = validate(desc);
+  if (rc != OIDT_OK) return rc;
+  if (desc->scheme != b->scheme || desc->precision != b->precision)
+    return fail(OIDT_EINVAL, "run scheme/precision differs from the batch");
+  int si = -1;
+  for (int i = 0; i < b->max_slots; ++i)
+    if (!b->slots[i].used) { si = i; break; }
+  if (si < 0) return fail(OIDT_EFULL, "batch is full");
+  SlotHost& sh = b->slots[si];
+  sh = SlotHost();
+  sh.desc = *desc;
+  rc = b->precision == OIDT_F64 ? build_slot<double>(b, sh, *desc, in) : build_slot<float>(b, sh, *desc, in);
+  if (rc != OIDT_OK) {
+    if (sh.dmem) hipFreeAsync(sh.dmem, b->stream);
+    sh = SlotHost();
+    return rc;
+  }
+  rc = upload_bytes(b, static_cast<char*>(b->d_slots) + (size_t)si * b->slot_bytes(), sh.slot.data(), sh.slot.size());
+  if (rc != OIDT_OK) return rc;
+  sh.used = true;
+  sh.admit_chunk = b->chunks_enqueued;
+  b->dirty = true;
+  *slot = si;
+  return OIDT_OK;
+}
+
+int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks) {
+  if (!b) return fail(OIDT_EINVAL, "null batch");
+  if (xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32) return fail(OIDT_EINVAL, "xchunk must be 4, 8, 16 or 32");
+  if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
+  b->xchunk = xchunk;
+  b->grid_override = grid_blocks;
+  b->dirty = true;
+  return OIDT_OK;
+}
+
+int oidt_batch_advance(oidt_batch* b, int32_t nchunks) {
+  if (!b) return fail(OIDT_EINVAL, "null batch");
+  if (nchunks < 0) return fail(OIDT_EINVAL, "negative chunk count");
+  if (nchunks == 0) return OIDT_OK;
+  int rc;
+  if (b->dirty) {
+    if ((rc = rebuild_lists(b)) != OIDT_OK) return rc;
+    if ((rc = capture_graph(b)) != OIDT_OK) return rc;
+  }
+  for (int c = 0; c < nchunks; ++c) HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+  b->chunks_enqueued += nchunks;
+  hipEvent_t ev;
+  if (!b->free_events.empty()) {
+    ev = b->free_events.back();
+    b->free_events.pop_back();
+  } else {
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventRecord(ev, b->stream));
+  b->marks.emplace_back(ev, b->chunks_enqueued);
+  return OIDT_OK;
+}
+
+int oidt_batch_sync(oidt_batch* b) {
+  if (!b) return fail(OIDT_EINVAL, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  reap_marks(b);
+  return OIDT_OK;
+}
+
+int oidt_batch_poll(oidt_batch* b, int32_t slot, int32_t* steps_done) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  if (!steps_done) return fail(OIDT_EINVAL, "null output");
+  reap_marks(b);
+  const SlotHost& sh = b->slots[slot];
+  const int64_t ch = std::max<int64_t>(0, b->chunks_done - sh.admit_chunk);
+  *steps_done = (int32_t)std::min<int64_t>(sh.desc.nt, ch * kChunk);
+  return OIDT_OK;
+}
+
+int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp, double* re, double* im) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  const SlotHost& sh = b->slots[slot];
+  const int ncomp = sh.nq / 2;
+  if (port < 0 || port > 1 || comp < 0 || comp >= ncomp) return fail(OIDT_EINVAL, "bad port or component");
+  if (!re || !im) return fail(OIDT_EINVAL, "null output");
+  const oidt_run_desc& d = sh.desc;
+  const size_t n = (size_t)d.nf * sh.wcells;
+  if (n == 0) return OIDT_OK;
+  const size_t off = d.precision == OIDT_F64 ? layout<double>(d).off_acc : layout<float>(d).off_acc;
+  const int q = port * ncomp + comp;
+  std::vector<double> buf(2 * n);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(buf.data(), static_cast<char*>(sh.dmem) + off + (size_t)q * n * 2 * sizeof(double),
+                    2 * n * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t e = 0; e < n; ++e) {
+    re[e] = buf[2 * e];
+    im[e] = buf[2 * e + 1];
+  }
+  return OIDT_OK;
+}
+
+int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host, uint64_t nbytes) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  const SlotHost& sh = b->slots[slot];
+  const oidt_run_desc& d = sh.desc;
+  if (field < 0 || field > 5 || !field_used(d.scheme, field)) return fail(OIDT_EINVAL, "field not part of this scheme");
+  const size_t es = d.precision == OIDT_F64 ? 8 : 4;
+  const size_t bytes = (size_t)d.nx * d.ny * d.nz * es;
+  if (!host || nbytes != bytes) return fail(OIDT_EINVAL, "host buffer size mismatch");
+  const size_t off = d.precision == OIDT_F64 ? layout<double>(d).off_f[field] : layout<float>(d).off_f[field];
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(host, static_cast<char*>(sh.dmem) + off, bytes, hipMemcpyDeviceToHost));
+  return OIDT_OK;
+}
+
+int oidt_batch_remove(oidt_batch* b, int32_t slot) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  SlotHost& sh = b->slots[slot];
+  // Deactivate in stream order, then free after all enqueued work on it.
+  std::vector<unsigned char> zero(b->slot_bytes(), 0);
+  rc = upload_bytes(b, static_cast<char*>(b->d_slots) + (size_t)slot * b->slot_bytes(), zero.data(), zero.size());
+  if (rc != OIDT_OK) return rc;
+  if (sh.dmem) HIP_TRY(hipFreeAsync(sh.dmem, b->stream));
+  sh = SlotHost();
+  b->dirty = true;
+  return OIDT_OK;
+}
+
+int oidt_batch_destroy(oidt_batch* b) {
+  if (!b) return OIDT_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (auto& sh : b->slots)
+    if (sh.dmem) hipFree(sh.dmem);
+  if (b->gexec) hipGraphExecDestroy(b->gexec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->d_tiles) hipFree(b->d_tiles);
+  if (b->d_work) hipFree(b->d_work);
+  if (b->d_counts) hipFree(b->d_counts);
+  if (b->d_slots) hipFree(b->d_slots);
+  for (auto& m : b->marks) hipEventDestroy(m.first);
+  for (auto e : b->free_events) hipEventDestroy(e);
+  b->staging.release();
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return OIDT_OK;
+}
+
+// ---- single run = batch of one -------------------------------------------
+
+int oidt_solver_create(const oidt_run_desc* desc, const oidt_run_inputs* in, oidt_solver** out) {
+  if (!out) return fail(OIDT_EINVAL, "null output");
+  *out = nullptr;
+  int rc = validate(desc);
+  if (rc != OIDT_OK) return rc;
+  oidt_solver* s = new (std::nothrow) oidt_solver();
+  if (!s) return fail(OIDT_ENOMEM, "host allocation failed");
+  if ((rc = oidt_batch_create(desc->scheme, desc->precision, 1, &s->b)) != OIDT_OK) {
+    delete s;
+    return rc;
+  }
+  if ((rc = oidt_batch_add(s->b, desc, in, &s->slot)) != OIDT_OK) {
+    oidt_batch_destroy(s->b);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return OIDT_OK;
+}
+
+int oidt_solver_advance(oidt_solver* s, int32_t nsteps) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  if (nsteps < 0) return fail(OIDT_EINVAL, "negative step count");
+  const SlotHost& sh = s->b->slots[s->slot];
+  const int64_t total = (sh.desc.nt + kChunk - 1) / kChunk;
+  const int64_t done = s->b->chunks_enqueued - sh.admit_chunk;
+  const int64_t want = std::min<int64_t>(total - done, (nsteps + kChunk - 1) / kChunk);
+  return want > 0 ? oidt_batch_advance(s->b, (int32_t)want) : OIDT_OK;
+}
+
+int oidt_solver_sync(oidt_solver* s) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_sync(s->b);
+}
+
+int oidt_solver_steps(oidt_solver* s, int32_t* steps) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_poll(s->b, s->slot, steps);
+}
+
+int oidt_solver_read_dft(oidt_solver* s, int32_t port, int32_t comp, double* re, double* im) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_read_dft(s->b, s->slot, port, comp, re, im);
+}
+
+int oidt_solver_read_field(oidt_solver* s, int32_t field, void* host, uint64_t nbytes) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_read_field(s->b, s->slot, field, host, nbytes);
+}
+
+int oidt_solver_destroy(oidt_solver* s) {
+  if (!s) return OIDT_OK;
+  const int rc = oidt_batch_destroy(s->b);
+  delete s;
+  return rc;
+}
+
+}  // extern "C"

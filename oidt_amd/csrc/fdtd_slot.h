@@ -1,0 +1,69 @@
+// Device-resident description of one run ("slot") and the launch constants
+// shared by the kernels and the host library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oidt {
+
+// Steps per DFT chunk: the stencil kernels record port-plane samples into a
+// ring of this many steps; the DFT kernel folds a whole chunk into the FP64
+// accumulators at once.
+constexpr int kChunk = 32;
+// Stencil block: 64 lanes along the fastest array axis (z in 3D, y in 2D)
+// times 4 rows.
+constexpr int kBX = 64;
+constexpr int kBY = 4;
+constexpr int kThreads = kBX * kBY;
+// Frequencies per DFT work item.
+constexpr int kFB = 16;
+// Material table entries (distinct (cbx, cby, cbz) tuples per run).
+constexpr int kMatMax = 256;
+
+enum : int { SCHEME_TMZ = 0, SCHEME_TEZ = 1, SCHEME_3D = 2 };
+enum : int { F_EX = 0, F_EY = 1, F_EZ = 2, F_HX = 3, F_HY = 4, F_HZ = 5 };
+
+// psi arrays are addressed as psi[(component * 3 + axis) * 2 + side]
+// (component 0..5 = Ex..Hz, axis 0..2 = x..z, side 0 = low, 1 = high).
+// Layouts follow the reference allocation (fdtd_main.py:128-137, 3D :147-175):
+//   x-slab [N][ny][nz], y-slab [nx][N][nz], z-slab [nx][ny][N].
+__host__ __device__ constexpr int psi_id(int comp, int axis, int side) {
+  return (comp * 3 + axis) * 2 + side;
+}
+
+template <typename T>
+struct Slot {
+  int nx, ny, nz, N;
+  int ib, jb, kb, ie, je, ke;
+  int nt, nf;
+  int base;    // first step of the chunk in flight (advanced on the device)
+  int active;
+  int scheme;
+  int src_field;  // -1: none
+  int sx0, sx1, sy0, sy1, sz0, sz1;
+  int port[2];
+  int wx0, wx1, wy0, wy1;  // DFT window (half-open); wx unused in 2D
+  int prow;                // raw capture plane row length (wy1 - wy0 + 1)
+  int pcells;              // raw capture plane cells
+  int wcells;              // window cells
+  int nplanes;             // raw planes per step
+  int nq;                  // DFT sample components
+  T db;
+  double dt;
+  T* f[6];
+  T* psi[36];
+  const uint8_t* mat;   // per-cell index into cbt
+  const T* cbt;         // [kMatMax][3] cb of (Ex, Ey, Ez)
+  const T* ike[3];      // 1 / (kappa_e * dx) per axis
+  const T* ikh[3];      // 1 / (kappa_h * dx) per axis
+  const T* be;
+  const T* ceo;         // ce / dx
+  const T* bh;
+  const T* cho;         // ch / dx
+  const T* jsrc;        // [nt]
+  T* ring;              // [kChunk][nplanes][pcells]
+  double* acc;          // [nq][nf][wcells][2]
+  const double* w;      // [nf] angular frequency 2*pi*f
+};
+
+}  // namespace oidt

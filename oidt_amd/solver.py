@@ -1,0 +1,271 @@
+"""Device runs through the C ABI: one run (``Solver``) or many (``Batch``).
+
+``run_arrays`` assembles exactly the arrays the reference driver hands its
+kernels (FDTD/GPU/FDTD_2D_TM/s-parameter_extraction/fdtd_main.py:83-178,
+FDTD_3D fdtd_main.py:86-221) plus the source box of update_source
+(fdtd_main.py:280-286; TE :291-297; 3D :364-374).  ``Solver.dft`` returns the
+port phasors in the reference's layout (``EZF1 = EZ1r + 1j*EZ1i``,
+fdtd_main.py:359-363; 3D [nf, nx, ny], FDTD_3D fdtd_main.py:462-470).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from . import materials as M
+from ._lib import check, lib, ptr
+
+
+class RunArrays:
+    """Host arrays of one run, kept alive while the library reads them."""
+
+    def __init__(self, s, fg, direction='f', line=0):
+        if direction not in ('f', 'b'):
+            raise ValueError("direction must be 'f' or 'b'")
+        P = s.precision
+        self.setup = s
+        self.direction = direction
+        self.line = line
+        g = M.cpml_grading(s.num_cpml, s.m_grade, s.m_alpha, s.kappa_max, s.alpha_max, s.sigma_factor,
+                           s.eta, s.delta_x, s.delta_t, s.eps, P)
+        self.grading = g
+        o = s.offset
+        dims = (s.nx, s.ny, s.nz) if s.scheme == '3d' else (s.nx, s.ny)
+        self.kappa = [M.kappa_axes(n, s.num_cpml, o, g['KE'], g['KH'], s.delta_x, P) for n in dims]
+        self.cb = [np.ascontiguousarray(a, dtype=P) for a in M.cb_maps(s, fg)]
+        self.jsrc = np.ascontiguousarray(s.J_SRC, dtype=P)
+        self.freqs = np.ascontiguousarray(s.sparam_freqs, dtype=np.float64)
+        self.db = P(float(s.delta_t) / s.mu)   # DB = delta_t / mu (fdtd_main.py:86), narrowed at launch
+        self.src_field, self.src_box = source_box(s, direction, line)
+
+    def desc(self):
+        s = self.setup
+        d = _lib.RunDesc()
+        d.abi_version = _lib.ABI_VERSION
+        d.scheme = _lib.SCHEME_ID[s.scheme]
+        d.precision = _lib.F32 if s.precision == np.float32 else _lib.F64
+        d.nx, d.ny = s.nx, s.ny
+        d.nz = s.nz if s.scheme == '3d' else 1
+        d.num_cpml = s.num_cpml
+        d.offset = s.offset
+        d.nt = s.nt
+        d.nf = len(self.freqs)
+        d.src_field = self.src_field
+        for a in range(6):
+            d.src_box[a] = self.src_box[a]
+        d.port[0], d.port[1] = s.ports
+        d.dx = float(s.delta_x)
+        d.dt = float(s.delta_t)
+        d.db = float(self.db)
+        return d
+
+    def inputs(self):
+        r = _lib.RunInputs()
+        for c, a in enumerate(self.cb):
+            r.cb[c] = ptr(a)
+        for a, (ke, kh) in enumerate(self.kappa):
+            r.kappa_e[a] = ptr(ke)
+            r.kappa_h[a] = ptr(kh)
+        g = self.grading
+        r.be, r.ce, r.bh, r.ch = ptr(g['BE']), ptr(g['CE']), ptr(g['BH']), ptr(g['CH'])
+        r.jsrc = ptr(self.jsrc)
+        r.freqs = ptr(self.freqs)
+        return r
+
+
+def source_box(s, direction, line):
+    """(field id, [x0,x1,y0,y1,z0,z1]) of update_source for this run."""
+    shift = line * s.pp
+    if s.scheme in ('tm', 'te'):
+        m = 1 if s.scheme == 'tm' else 2
+        y0, y1 = s.by + shift + m, s.by + shift + s.ny_swg - m
+        x0, x1 = (s.sx, s.sx + 1) if direction == 'f' else (s.nx - s.sx - 1, s.nx - s.sx)
+        box = [x0, x1, y0, y1, 0, 1]
+        fid = _lib.EZ if s.scheme == 'tm' else _lib.HZ
+        lim = (s.nx, s.ny, 1)
+    else:
+        m = 1 if s.source_condition == 'TE' else 2
+        x0, x1 = s.bx + shift + m, s.bx + shift + s.nx_swg - m
+        y0, y1 = s.by + m, s.by + s.ny_swg - m
+        z0, z1 = (s.sz, s.sz + 2) if direction == 'f' else (s.nz - s.sz - 1, s.nz - s.sz)
+        box = [x0, x1, y0, y1, z0, z1]
+        fid = _lib.EY if s.source_condition == 'TE' else _lib.HY
+        lim = (s.nx, s.ny, s.nz)
+    for a in range(3):   # the reference kernel guard only reaches cells inside the grid
+        box[2 * a] = min(max(box[2 * a], 0), lim[a])
+        box[2 * a + 1] = min(max(box[2 * a + 1], box[2 * a]), lim[a])
+    return fid, box
+
+
+def _window(s):
+    N = s.num_cpml
+    if s.scheme == '3d':
+        return (N, s.nx - N), (N, s.ny - N)
+    return None, (N, s.ny - N)
+
+
+def expand_dft(s, comp_arrays):
+    """Compact window phasors -> the reference's full-plane arrays (zeros outside)."""
+    nf = len(s.sparam_freqs)
+    wx, wy = _window(s)
+    out = {}
+    for key, z in comp_arrays.items():
+        if s.scheme == '3d':
+            full = np.zeros((nf, s.nx, s.ny), dtype=np.complex128)
+            full[:, wx[0]:wx[1], wy[0]:wy[1]] = z.reshape(nf, wx[1] - wx[0], wy[1] - wy[0])
+        else:
+            full = np.zeros((nf, s.ny), dtype=np.complex128)
+            full[:, wy[0]:wy[1]] = z.reshape(nf, wy[1] - wy[0])
+        out[key] = full
+    return out
+
+
+COMP_KEYS = {'tm': ('e', 'h'), 'te': ('e', 'h'), '3d': ('ex', 'ey', 'hx', 'hy')}
+FIELDS = {'tm': ('ez', 'hx', 'hy'), 'te': ('ex', 'ey', 'hz'), '3d': ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')}
+
+
+def _window_cells(s):
+    wx, wy = _window(s)
+    return (wy[1] - wy[0]) * ((wx[1] - wx[0]) if wx else 1)
+
+
+class Solver:
+    """One run on the device (a batch of one inside the library)."""
+
+    def __init__(self, setup, fg, direction='f', line=0):
+        self.arrays = RunArrays(setup, fg, direction, line)
+        self.setup = setup
+        self._desc = self.arrays.desc()
+        self._in = self.arrays.inputs()
+        h = C.c_void_p()
+        check(lib().oidt_solver_create(C.byref(self._desc), C.byref(self._in), C.byref(h)))
+        self.h = h
+
+    def advance(self, nsteps):
+        check(lib().oidt_solver_advance(self.h, int(nsteps)))
+        return self
+
+    def sync(self):
+        check(lib().oidt_solver_sync(self.h))
+        return self
+
+    def run(self):
+        return self.advance(self.setup.nt).sync()
+
+    def steps(self):
+        n = C.c_int32()
+        check(lib().oidt_solver_steps(self.h, C.byref(n)))
+        return n.value
+
+    def read_dft(self, port, comp):
+        n = len(self.setup.sparam_freqs) * _window_cells(self.setup)
+        re = np.zeros(n)
+        im = np.zeros(n)
+        check(lib().oidt_solver_read_dft(self.h, port, comp, ptr(re), ptr(im)))
+        return re + 1j * im
+
+    def dft(self):
+        """{(component key, port 1|2): complex [nf, ny] (2D) or [nf, nx, ny] (3D)}."""
+        keys = COMP_KEYS[self.setup.scheme]
+        raw = {(k, p + 1): self.read_dft(p, c) for p in (0, 1) for c, k in enumerate(keys)}
+        return expand_dft(self.setup, raw)
+
+    def field(self, name):
+        s = self.setup
+        a = np.zeros(s.shape, dtype=s.precision)
+        check(lib().oidt_solver_read_field(self.h, _lib.FIELD_ID[name], ptr(a), a.nbytes))
+        return a
+
+    def fields(self):
+        return {k: self.field(k) for k in FIELDS[self.setup.scheme]}
+
+    def close(self):
+        if getattr(self, 'h', None):
+            check(lib().oidt_solver_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many runs of one scheme/precision advanced by the same launches."""
+
+    def __init__(self, scheme, precision, max_slots=64):
+        self.scheme = scheme
+        self.precision = precision
+        h = C.c_void_p()
+        check(lib().oidt_batch_create(_lib.SCHEME_ID[scheme], _lib.F32 if precision == np.float32 else _lib.F64,
+                                      int(max_slots), C.byref(h)))
+        self.h = h
+        self.runs = {}
+
+    def add(self, setup, fg, direction='f', line=0):
+        arr = RunArrays(setup, fg, direction, line)
+        d, r = arr.desc(), arr.inputs()
+        slot = C.c_int32()
+        check(lib().oidt_batch_add(self.h, C.byref(d), C.byref(r), C.byref(slot)))
+        self.runs[slot.value] = setup
+        return slot.value
+
+    def set_launch(self, xchunk=8, grid_blocks=0):
+        check(lib().oidt_batch_set_launch(self.h, int(xchunk), int(grid_blocks)))
+
+    def advance(self, nchunks):
+        check(lib().oidt_batch_advance(self.h, int(nchunks)))
+
+    def sync(self):
+        check(lib().oidt_batch_sync(self.h))
+
+    def poll(self, slot):
+        n = C.c_int32()
+        check(lib().oidt_batch_poll(self.h, int(slot), C.byref(n)))
+        return n.value
+
+    def read_dft(self, slot, port, comp):
+        s = self.runs[slot]
+        n = len(s.sparam_freqs) * _window_cells(s)
+        re = np.zeros(n)
+        im = np.zeros(n)
+        check(lib().oidt_batch_read_dft(self.h, int(slot), port, comp, ptr(re), ptr(im)))
+        return re + 1j * im
+
+    def dft(self, slot):
+        s = self.runs[slot]
+        keys = COMP_KEYS[s.scheme]
+        raw = {(k, p + 1): self.read_dft(slot, p, c) for p in (0, 1) for c, k in enumerate(keys)}
+        return expand_dft(s, raw)
+
+    def field(self, slot, name):
+        s = self.runs[slot]
+        a = np.zeros(s.shape, dtype=s.precision)
+        check(lib().oidt_batch_read_field(self.h, int(slot), _lib.FIELD_ID[name], ptr(a), a.nbytes))
+        return a
+
+    def remove(self, slot):
+        check(lib().oidt_batch_remove(self.h, int(slot)))
+        self.runs.pop(slot, None)
+
+    def close(self):
+        if getattr(self, 'h', None):
+            check(lib().oidt_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(setup, direction='f', line=0):
+    """Device bytes a run of ``setup`` holds (admission by footprint)."""
+    fg = np.zeros(setup.shape, dtype=bool)
+    arr = RunArrays(setup, fg, direction, line)
+    d = arr.desc()
+    b = C.c_uint64()
+    check(lib().oidt_run_footprint(C.byref(d), C.byref(b)))
+    return b.value
