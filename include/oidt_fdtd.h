@@ -174,11 +174,27 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
 /* Release a slot (its device memory is returned in stream order). */
 int oidt_batch_remove(oidt_batch* b, int32_t slot);
 
-/* Launch geometry knobs for the autotuner: x-chunk depth of the stencil tiles
- * (cells marched per thread) and the persistent grid size (0 = automatic). */
-int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks);
+/* Launch knobs for the autotuner: x-chunk depth of the stencil tiles (cells per
+ * thread along x: 1, 2, 4, 8, 16 or 32; default 4), the persistent grid size
+ * (0 = automatic) and the stencil kernel form (1 = per-plane march, 2 =
+ * batched loads per x-chunk, the default). */
+int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant);
 
 int oidt_batch_destroy(oidt_batch* b);
+
+/* ---- measurement -------------------------------------------------------- */
+/* No counterpart in the reference, which times the whole loop with time()
+ * (fdtd_main.py:271-341).  These run real work and advance the runs. */
+
+/* Enqueue nchunks chunks as individual launches with a HIP event pair around
+ * every kernel on the batch stream, wait, and return the mean duration (ms) of
+ * out_ms4[0] E half-step, [1] H half-step, [2] DFT fold, [3] chunk advance. */
+int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4);
+int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4);
+
+/* Mean duration (ms) of a plain device copy kernel moving nbytes (read nbytes,
+ * write nbytes), over reps launches: the reference point for HBM counters. */
+int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms);
 
 #ifdef __cplusplus
 }

@@ -61,8 +61,11 @@ SIGNATURES = {
     'oidt_batch_read_dft': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     'oidt_batch_read_field': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_uint64]),
     'oidt_batch_remove': (C.c_int, [C.c_void_p, C.c_int32]),
-    'oidt_batch_set_launch': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    'oidt_batch_set_launch': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     'oidt_batch_destroy': (C.c_int, [C.c_void_p]),
+    'oidt_batch_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    'oidt_solver_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    'oidt_copy_probe': (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 _LIB = None
@@ -106,6 +109,12 @@ def device_info():
     check(lib().oidt_device_info_get(C.byref(d)))
     return {'name': d.name.decode(), 'arch': d.arch.decode(), 'compute_units': d.compute_units,
             'hbm_total_bytes': d.hbm_total_bytes, 'hbm_free_bytes': d.hbm_free_bytes}
+
+
+def copy_probe(nbytes, reps=20):
+    ms = C.c_double()
+    check(lib().oidt_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def chunk_steps():

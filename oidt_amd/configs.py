@@ -79,7 +79,7 @@ def config3(count=16, nt=5000):
     out = []
     for r in range(count):
         nz = int(rng.integers(192, 513))
-        out.append(dict(strip3d(nz, nx=96, ny=64, nt=nt), rough_std=float(rng.uniform(3e-9, 8e-9)),
+        out.append(dict(strip3d(nz, nx=96, ny=64, nt=nt, port_len=40), rough_std=float(rng.uniform(3e-9, 8e-9)),
                         rough_acl=float(rng.uniform(30e-9, 80e-9)), seed=2000 + r))
     return out
 

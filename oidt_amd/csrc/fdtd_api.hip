@@ -159,7 +159,7 @@ int validate(const oidt_run_desc* d) {
 
 template <typename T>
 struct Layout {
-  size_t off_f[6], off_psi[36], off_mat, off_cbt, off_ike[3], off_ikh[3], off_be, off_ceo, off_bh, off_cho,
+  size_t off_f[6], off_psi[36], off_mat, off_ike[3], off_ikh[3], off_be, off_ceo, off_bh, off_cho,
       off_jsrc, off_ring, off_acc, off_w, total;
   int nplanes, nq, wcells, pcells, prow, wx0, wx1, wy0, wy1;
 };
@@ -181,7 +181,6 @@ Layout<T> layout(const oidt_run_desc& d) {
     for (int side = 0; side < 2; ++side)
       L.off_psi[psi_id(pr.first, pr.second, side)] = take(psi_elems(d, pr.second) * sizeof(T));
   L.off_mat = take(cells);
-  L.off_cbt = take(kMatMax * 3 * sizeof(T));
   for (int a = 0; a < 3; ++a) {
     const int n = a == 0 ? d.nx : (a == 1 ? d.ny : d.nz);
     L.off_ike[a] = take(n * sizeof(T));
@@ -238,8 +237,9 @@ struct oidt_batch {
   hipGraphExec_t gexec = nullptr;
   bool dirty = true;
   int ntiles = 0, nwork = 0;
-  int xchunk = 8;
+  int xchunk = 4;
   int grid_override = 0;
+  int variant = 2;
   int64_t chunks_enqueued = 0;
   int64_t chunks_done = 0;
   std::deque<std::pair<hipEvent_t, int64_t>> marks;
@@ -270,36 +270,32 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   if (d.nt > 0 && !in->jsrc) return fail(OIDT_EINVAL, "missing J_SRC");
   if (d.nf > 0 && !in->freqs) return fail(OIDT_EINVAL, "missing freqs");
 
-  // Material compression: distinct (cbx, cby, cbz) tuples -> uint8 index.
-  std::vector<uint8_t> mat(cells);
-  std::vector<T> cbt(kMatMax * 3, T(0));
-  {
-    const T* cbp[3] = {nullptr, nullptr, nullptr};
-    for (int c = 0; c < ncb; ++c) cbp[cb_comp[d.scheme][c]] = static_cast<const T*>(in->cb[c]);
-    std::map<std::vector<T>, int> index;
-    std::vector<T> key(3);
+  // Material compression: per E component, the distinct cb values (at most
+  // kMatMax) and a 2-bit index per cell, packed as bits 2c..2c+1 of one byte.
+  std::vector<uint8_t> mat(cells, 0);
+  T cbv[3][kMatMax] = {};
+  for (int c = 0; c < ncb; ++c) {
+    const int comp = cb_comp[d.scheme][c];
+    const T* src = static_cast<const T*>(in->cb[c]);
+    int nv = 0;
+    T vals[kMatMax] = {};
     int last = -1;
-    std::vector<T> last_key;
     for (size_t e = 0; e < cells; ++e) {
-      for (int c = 0; c < 3; ++c) key[c] = cbp[c] ? cbp[c][e] : T(0);
-      if (last >= 0 && key == last_key) {
-        mat[e] = (uint8_t)last;
-        continue;
+      const T v = src[e];
+      int id = -1;
+      if (last >= 0 && vals[last] == v) id = last;
+      for (int q = 0; id < 0 && q < nv; ++q)
+        if (vals[q] == v) id = q;
+      if (id < 0) {
+        if (nv == kMatMax)
+          return fail(OIDT_EINVAL, "cb map of component " + std::to_string(comp) + " has more than 4 distinct values");
+        vals[nv] = v;
+        id = nv++;
       }
-      auto it = index.find(key);
-      int id;
-      if (it == index.end()) {
-        id = (int)index.size();
-        if (id >= kMatMax) return fail(OIDT_EINVAL, "more than 256 distinct cb tuples");
-        index.emplace(key, id);
-        for (int c = 0; c < 3; ++c) cbt[id * 3 + c] = key[c];
-      } else {
-        id = it->second;
-      }
-      mat[e] = (uint8_t)id;
       last = id;
-      last_key = key;
+      mat[e] |= (uint8_t)(id << (2 * comp));
     }
+    for (int q = 0; q < kMatMax; ++q) cbv[comp][q] = vals[q];
   }
   // Reciprocal kappa arrays and folded CPML c / dx.
   auto recip = [](const void* src, int n, std::vector<T>& out) {
@@ -338,7 +334,6 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   struct Piece { size_t off; const void* src; size_t bytes; };
   std::vector<Piece> pieces = {
       {L.off_mat, mat.data(), cells},
-      {L.off_cbt, cbt.data(), cbt.size() * sizeof(T)},
       {L.off_be, be.data(), N * sizeof(T)},
       {L.off_ceo, ceo.data(), N * sizeof(T)},
       {L.off_bh, bh.data(), N * sizeof(T)},
@@ -380,7 +375,8 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   for (int f = 0; f < 6; ++f) s.f[f] = L.off_f[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f[f]);
   for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi[i]);
   s.mat = reinterpret_cast<const uint8_t*>(base + L.off_mat);
-  s.cbt = reinterpret_cast<const T*>(base + L.off_cbt);
+  for (int c = 0; c < 3; ++c)
+    for (int q = 0; q < kMatMax; ++q) s.cbv[c][q] = cbv[c][q];
   for (int a = 0; a < 3; ++a) {
     s.ike[a] = reinterpret_cast<const T*>(base + L.off_ike[a]);
     s.ikh[a] = reinterpret_cast<const T*>(base + L.off_ikh[a]);
@@ -460,43 +456,123 @@ int grid_for(int items, int cap) {
 }
 
 template <typename T, int XC>
-void launch_stencils(oidt_batch* b, int gs, int tstep) {
+void launch_half(oidt_batch* b, int gs, int tstep, bool e_half) {
   const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
-  dim3 blk(kBX, kBY);
+  const dim3 blk(kBX, kBY), grd(gs);
+  hipStream_t st = b->stream;
+  const bool v1 = b->variant == 1;
   switch (b->scheme) {
     case SCHEME_TMZ:
-      hipLaunchKernelGGL((k_tmz_e<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
-      hipLaunchKernelGGL((k_tmz_h<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      if (e_half) {
+        if (v1) hipLaunchKernelGGL((k_tmz_e<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+        else hipLaunchKernelGGL((k_tmz_e2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      } else {
+        if (v1) hipLaunchKernelGGL((k_tmz_h<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+        else hipLaunchKernelGGL((k_tmz_h2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      }
       break;
     case SCHEME_TEZ:
-      hipLaunchKernelGGL((k_tez_e<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
-      hipLaunchKernelGGL((k_tez_h<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      if (e_half) {
+        if (v1) hipLaunchKernelGGL((k_tez_e<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+        else hipLaunchKernelGGL((k_tez_e2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      } else {
+        if (v1) hipLaunchKernelGGL((k_tez_h<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+        else hipLaunchKernelGGL((k_tez_h2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      }
       break;
     default:
-      hipLaunchKernelGGL((k_3d_e<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
-      hipLaunchKernelGGL((k_3d_h<T, XC>), dim3(gs), blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+      if (e_half) {
+        if (v1) hipLaunchKernelGGL((k_3d_e<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+        else hipLaunchKernelGGL((k_3d_e2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      } else {
+        if (v1) hipLaunchKernelGGL((k_3d_h<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+        else hipLaunchKernelGGL((k_3d_h2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      }
       break;
   }
 }
 
 template <typename T>
-int enqueue_chunk(oidt_batch* b) {
-  const int cap = b->grid_override > 0 ? b->grid_override : 8192;
-  const int gs = grid_for(b->ntiles, cap);
-  const int gd = grid_for(b->nwork, 8192);
-  for (int t = 0; t < kChunk; ++t) {
-    switch (b->xchunk) {
-      case 4: launch_stencils<T, 4>(b, gs, t); break;
-      case 16: launch_stencils<T, 16>(b, gs, t); break;
-      case 32: launch_stencils<T, 32>(b, gs, t); break;
-      default: launch_stencils<T, 8>(b, gs, t); break;
-    }
+void launch_stencil(oidt_batch* b, int gs, int tstep, bool e_half) {
+  switch (b->xchunk) {
+    case 1: launch_half<T, 1>(b, gs, tstep, e_half); break;
+    case 2: launch_half<T, 2>(b, gs, tstep, e_half); break;
+    case 4: launch_half<T, 4>(b, gs, tstep, e_half); break;
+    case 16: launch_half<T, 16>(b, gs, tstep, e_half); break;
+    case 32: launch_half<T, 32>(b, gs, tstep, e_half); break;
+    default: launch_half<T, 8>(b, gs, tstep, e_half); break;
   }
+}
+
+template <typename T>
+void launch_dft(oidt_batch* b, int gd) {
   hipLaunchKernelGGL((k_dft<T>), dim3(gd), dim3(kThreads), 0, b->stream,
                      static_cast<const Slot<T>*>(b->d_slots), b->d_work, b->d_counts);
+}
+
+template <typename T>
+void launch_advance(oidt_batch* b) {
   hipLaunchKernelGGL((k_advance<T>), dim3((b->max_slots + 255) / 256), dim3(256), 0, b->stream,
                      static_cast<Slot<T>*>(b->d_slots), b->max_slots);
+}
+
+int stencil_grid(const oidt_batch* b) { return grid_for(b->ntiles, b->grid_override > 0 ? b->grid_override : 8192); }
+int dft_grid(const oidt_batch* b) { return grid_for(b->nwork, 8192); }
+
+template <typename T>
+int enqueue_chunk(oidt_batch* b) {
+  const int gs = stencil_grid(b), gd = dft_grid(b);
+  for (int t = 0; t < kChunk; ++t) {
+    launch_stencil<T>(b, gs, t, true);
+    launch_stencil<T>(b, gs, t, false);
+  }
+  launch_dft<T>(b, gd);
+  launch_advance<T>(b);
   return hipGetLastError() == hipSuccess ? OIDT_OK : fail(OIDT_EHIP, "kernel launch failed");
+}
+
+// Direct (non-graph) launches with an event pair around every kernel.
+template <typename T>
+int profile_chunks(oidt_batch* b, int nchunks, double* out) {
+  const int gs = stencil_grid(b), gd = dft_grid(b);
+  const int per = 2 * kChunk + 2;
+  std::vector<hipEvent_t> ev(2 * per * nchunks);
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  int k = 0;
+  for (int c = 0; c < nchunks; ++c) {
+    for (int t = 0; t < kChunk; ++t)
+      for (int h = 0; h < 2; ++h) {
+        HIP_TRY(hipEventRecord(ev[k++], b->stream));
+        launch_stencil<T>(b, gs, t, h == 0);
+        HIP_TRY(hipEventRecord(ev[k++], b->stream));
+      }
+    HIP_TRY(hipEventRecord(ev[k++], b->stream));
+    launch_dft<T>(b, gd);
+    HIP_TRY(hipEventRecord(ev[k++], b->stream));
+    HIP_TRY(hipEventRecord(ev[k++], b->stream));
+    launch_advance<T>(b);
+    HIP_TRY(hipEventRecord(ev[k++], b->stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  double sum[4] = {0, 0, 0, 0};
+  int cnt[4] = {0, 0, 0, 0};
+  k = 0;
+  for (int c = 0; c < nchunks; ++c) {
+    for (int i = 0; i < per; ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      k += 2;
+      const int kind = i < 2 * kChunk ? (i & 1) : (i == 2 * kChunk ? 2 : 3);
+      sum[kind] += ms;
+      cnt[kind] += 1;
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  for (int i = 0; i < 4; ++i) out[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
+  b->chunks_enqueued += nchunks;
+  b->chunks_done = b->chunks_enqueued;
+  return OIDT_OK;
 }
 
 int capture_graph(oidt_batch* b) {
@@ -627,9 +703,12 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
   return OIDT_OK;
 }
 
-int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks) {
+int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
-  if (xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32) return fail(OIDT_EINVAL, "xchunk must be 4, 8, 16 or 32");
+  if (variant != 1 && variant != 2) return fail(OIDT_EINVAL, "variant must be 1 or 2");
+  b->variant = variant;
+  if (xchunk != 1 && xchunk != 2 && xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32)
+    return fail(OIDT_EINVAL, "xchunk must be 1, 2, 4, 8, 16 or 32");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   b->xchunk = xchunk;
   b->grid_override = grid_blocks;
@@ -728,6 +807,55 @@ int oidt_batch_remove(oidt_batch* b, int32_t slot) {
   sh = SlotHost();
   b->dirty = true;
   return OIDT_OK;
+}
+
+int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4) {
+  if (!b || !out_ms4) return fail(OIDT_EINVAL, "null argument");
+  if (nchunks < 1) return fail(OIDT_EINVAL, "need at least one chunk");
+  int rc;
+  if (b->dirty) {
+    if ((rc = rebuild_lists(b)) != OIDT_OK) return rc;
+    if ((rc = capture_graph(b)) != OIDT_OK) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  reap_marks(b);
+  return b->precision == OIDT_F64 ? profile_chunks<double>(b, nchunks, out_ms4)
+                                  : profile_chunks<float>(b, nchunks, out_ms4);
+}
+
+int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms) {
+  if (!ms || nbytes < 16 || reps < 1) return fail(OIDT_EINVAL, "bad copy probe arguments");
+  const size_t n4 = nbytes / 16;
+  float4* a = nullptr;
+  float4* c = nullptr;
+  HIP_TRY(hipMalloc(&a, n4 * 16));
+  if (hipMalloc(&c, n4 * 16) != hipSuccess) {
+    hipFree(a);
+    return fail(OIDT_ENOMEM, "copy probe allocation failed");
+  }
+  HIP_TRY(hipMemset(a, 0, n4 * 16));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int blocks = (int)std::min<size_t>((n4 + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, nullptr, a, c, n4);
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, nullptr, a, c, n4);
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  *ms = t / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(c);
+  return OIDT_OK;
+}
+
+int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_profile(s->b, nchunks, out_ms4);
 }
 
 int oidt_batch_destroy(oidt_batch* b) {

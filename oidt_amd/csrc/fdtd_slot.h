@@ -17,8 +17,9 @@ constexpr int kBY = 4;
 constexpr int kThreads = kBX * kBY;
 // Frequencies per DFT work item.
 constexpr int kFB = 16;
-// Material table entries (distinct (cbx, cby, cbz) tuples per run).
-constexpr int kMatMax = 256;
+// Distinct cb values per E component (2-bit index per component in the
+// material byte).
+constexpr int kMatMax = 4;
 
 enum : int { SCHEME_TMZ = 0, SCHEME_TEZ = 1, SCHEME_3D = 2 };
 enum : int { F_EX = 0, F_EY = 1, F_EZ = 2, F_HX = 3, F_HY = 4, F_HZ = 5 };
@@ -52,8 +53,8 @@ struct Slot {
   double dt;
   T* f[6];
   T* psi[36];
-  const uint8_t* mat;   // per-cell index into cbt
-  const T* cbt;         // [kMatMax][3] cb of (Ex, Ey, Ez)
+  T cbv[3][kMatMax];    // distinct cb values of Ex, Ey, Ez
+  const uint8_t* mat;   // per cell: bits 2c..2c+1 index cbv[c]
   const T* ike[3];      // 1 / (kappa_e * dx) per axis
   const T* ikh[3];      // 1 / (kappa_h * dx) per axis
   const T* be;

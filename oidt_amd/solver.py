@@ -211,8 +211,8 @@ class Batch:
         self.runs[slot.value] = setup
         return slot.value
 
-    def set_launch(self, xchunk=8, grid_blocks=0):
-        check(lib().oidt_batch_set_launch(self.h, int(xchunk), int(grid_blocks)))
+    def set_launch(self, xchunk=8, grid_blocks=0, variant=2):
+        check(lib().oidt_batch_set_launch(self.h, int(xchunk), int(grid_blocks), int(variant)))
 
     def advance(self, nchunks):
         check(lib().oidt_batch_advance(self.h, int(nchunks)))
