@@ -64,17 +64,104 @@ def workload(name):
     raise SystemExit('unknown workload ' + name)
 
 
+def bench_serving(args):
+    """Serving mix: plain one-at-a-time path, saturated continuous batching, and
+    Poisson arrivals at 0.8x the saturated capacity (latency percentiles)."""
+    from oidt_amd import configs, serve
+    descs = configs.serving_requests(count=args.requests)
+    be = serve.DeviceBackend(max_slots=64, xchunk=args.xchunk, variant=args.variant)
+    reqs = serve.make_requests(descs, [0.0] * len(descs))
+    t0 = time.perf_counter()
+    for r in reqs:
+        serve.prepare_request(r, be)
+    prep = time.perf_counter() - t0
+    total_cu = sum(r.cell_updates for r in reqs)
+    # plain: each request alone, back to back
+    t0 = time.perf_counter()
+    plain = serve.serve_plain(reqs)
+    plain_wall = time.perf_counter() - t0
+    plain_rate = total_cu / plain_wall / 1e6
+    # saturated continuous batching (everything queued at t = 0)
+    hbm = int(0.8 * __import__('oidt_amd._lib', fromlist=['x']).device_info()['hbm_free_bytes'])
+    for r in reqs:
+        r.result = None
+    srv = serve.Server(be, hbm_budget=hbm, chunk_group=4, prepare=lambda r, b: None)
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    sat_wall = time.perf_counter() - t0
+    sat = serve.summarize(done, sat_wall)
+    cap_rps = len(reqs) / sat_wall
+    # Poisson arrivals at 0.8x the saturated request capacity
+    rate = 0.8 * cap_rps
+    arr = serve.poisson_arrivals(len(reqs), rate, seed=4)
+    preqs = serve.make_requests(descs, arr)
+    for a, b_ in zip(preqs, reqs):
+        a.setup, a.fg, a.footprint = b_.setup, b_.fg, b_.footprint
+    srv2 = serve.Server(be, hbm_budget=hbm, chunk_group=4, prepare=lambda r, b: None)
+    t0 = time.perf_counter()
+    pdone = srv2.run(preqs)
+    pw = time.perf_counter() - t0
+    pois = serve.summarize(pdone, pw)
+    be.close()
+    n2d = sum(1 for r in reqs if r.setup.scheme == 'tm')
+    line = {
+        'metric': 'Mcell-updates/s', 'value': round(sat['mcell_updates_per_s'], 1), 'unit': 'Mcell-updates/s',
+        'n_gpus': 1, 'steps': int(sum(r.setup.nt for r in reqs)), 'warmup': 0,
+        'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+        'data': 'synthetic',
+        'config': {'workload': 'serving mix: %d requests, %d 2D TMz (nx 400..4000, ny 200..240) and %d 3D '
+                               '((128..384)x(64..128)x(48..96)), 2000..8000 steps, FP32' % (len(reqs), n2d, len(reqs) - n2d),
+                   'max_slots_per_batch': 64, 'hbm_budget_bytes': hbm},
+        'plain_mcell_updates_per_s': round(plain_rate, 1),
+        'served_over_plain': round(sat['mcell_updates_per_s'] / plain_rate, 3),
+        'saturated': {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sat.items()},
+        'poisson': dict({k: (round(v, 4) if isinstance(v, float) else v) for k, v in pois.items()},
+                        rate_per_s=round(rate, 3)),
+        'peak_resident_runs': srv.peak_resident, 'host_prepare_s': round(prep, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_plain_ensemble(args, descs, label):
+    """The ensemble one run at a time (each alone on the device), same steps."""
+    from oidt_amd import configs, materials
+    from oidt_amd.solver import Solver
+    K = args.steps
+    runs = []
+    for p in descs:
+        p = dict(p, nt=K)
+        s, up, lo = configs.realise(p)
+        runs.append((s, materials.fg_mask(s, up, lo)))
+    # warm the device on the first run
+    Solver(*runs[0]).advance(64).sync().close()
+    t0 = time.perf_counter()
+    for s, fg in runs:
+        sv = Solver(s, fg)
+        sv.run()
+        sv.close()
+    wall = time.perf_counter() - t0
+    cells = sum(s.cells for s, _ in runs)
+    print(json.dumps({'metric': 'Mcell-updates/s', 'value': round(cells * K / wall / 1e6, 1),
+                      'unit': 'Mcell-updates/s', 'n_gpus': 1, 'steps': K, 'warmup': 0,
+                      'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True, 'scaling': 'weak',
+                      'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+                      'config': {'workload': label + ', plain: one run at a time incl. create/upload',
+                                 'runs': len(runs)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=6016)
     ap.add_argument('--warmup', type=int, default=256)
     ap.add_argument('--workload', default='c1')
-    ap.add_argument('--xchunk', type=int, default=8)
+    ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=2)
     ap.add_argument('--profile-chunks', type=int, default=4)
     ap.add_argument('--cpu-steps', type=int, default=20)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--plain', action='store_true', help='ensembles: one run at a time instead of one batch')
+    ap.add_argument('--requests', type=int, default=48, help='serving workload: number of requests')
     ap.add_argument('--copy-probe', type=float, default=0.0,
                     help='only run the copy kernel on this many MiB (counter calibration) and exit')
     args = ap.parse_args()
@@ -94,7 +181,13 @@ def main():
         print(json.dumps({'copy_probe_bytes': nbytes, 'ms': ms, 'GBps_rw': 2 * nbytes / ms / 1e6}))
         return
 
+    if args.workload == 'c4':
+        bench_serving(args)
+        return
     descs, label = workload(args.workload)
+    if args.plain:
+        bench_plain_ensemble(args, descs, label)
+        return
     chunk = _lib.chunk_steps()
     warm = -(-args.warmup // chunk) * chunk
     K = args.steps

@@ -168,6 +168,11 @@ int oidt_batch_sync(oidt_batch* b);
  * when equal to the run's nt). */
 int oidt_batch_poll(oidt_batch* b, int32_t slot, int32_t* steps_done);
 
+/* Chunks enqueued so far and chunks the GPU has completed (non-blocking). */
+int oidt_batch_progress(oidt_batch* b, int64_t* chunks_enqueued, int64_t* chunks_done);
+
+/* DFT of one slot (layout as oidt_solver_read_dft).  A run whose last chunk
+ * has completed is read without waiting for the rest of the batch. */
 int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp, double* re, double* im);
 int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host, uint64_t nbytes);
 
@@ -175,7 +180,8 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
 int oidt_batch_remove(oidt_batch* b, int32_t slot);
 
 /* Launch knobs for the autotuner: x-chunk depth of the stencil tiles (cells per
- * thread along x: 1, 2, 4, 8, 16 or 32; default 4), the persistent grid size
+ * thread along x: 1, 2, 4, 8, 16 or 32; 0 keeps the current value; default 1
+ * for 3D and 4 for 2D), the persistent grid size
  * (0 = automatic) and the stencil kernel form (1 = per-plane march, 2 =
  * batched loads per x-chunk, the default). */
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant);

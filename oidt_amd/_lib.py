@@ -58,6 +58,7 @@ SIGNATURES = {
     'oidt_batch_advance': (C.c_int, [C.c_void_p, C.c_int32]),
     'oidt_batch_sync': (C.c_int, [C.c_void_p]),
     'oidt_batch_poll': (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    'oidt_batch_progress': (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'oidt_batch_read_dft': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     'oidt_batch_read_field': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_uint64]),
     'oidt_batch_remove': (C.c_int, [C.c_void_p, C.c_int32]),

@@ -122,6 +122,21 @@ def small(scheme, precision='float64', nt=None, **over):
     return p
 
 
+def incident(p):
+    """Description of the incident (kind 'i') run paired with ``p``: the guide is
+    shortened to the two port settling lengths and both port planes fall on
+    the same cell, while nt is kept (fdtd_auto_setup.py:82-91, :161-162)."""
+    q = dict(p)
+    q.pop('rough_std', None)
+    q.pop('rough_acl', None)
+    port_len = p['p1'] - (p['sz'] if p['scheme'] == '3d' else p['sx'])
+    b = p['bz'] if p['scheme'] == '3d' else p['bx']
+    n = 2 * b + 2 * port_len
+    q['nz' if p['scheme'] == '3d' else 'nx'] = n
+    q['p1'] = q['p2'] = b + port_len
+    return q
+
+
 # --- building a realisation ------------------------------------------------
 
 def realise(p):

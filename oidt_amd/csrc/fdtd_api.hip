@@ -606,6 +606,13 @@ void reap_marks(oidt_batch* b) {
   }
 }
 
+// True once the GPU has completed every chunk the run needs (later chunks skip it).
+bool slot_finished(oidt_batch* b, const SlotHost& sh) {
+  reap_marks(b);
+  const int64_t need = (sh.desc.nt + kChunk - 1) / kChunk;
+  return b->chunks_done - sh.admit_chunk >= need;
+}
+
 int slot_ok(oidt_batch* b, int slot) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
   if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) return fail(OIDT_EINVAL, "no run in slot");
@@ -659,6 +666,9 @@ int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt
   b->scheme = scheme;
   b->precision = precision;
   b->max_slots = max_slots;
+  // measured defaults on MI355X: 3D is best with one cell per thread along x
+  // (54 VGPRs, full occupancy), 2D with four
+  b->xchunk = scheme == SCHEME_3D ? 1 : 4;
   b->slots.resize(max_slots);
   auto bad = [&](hipError_t e, const char* what) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -706,11 +716,11 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
   if (variant != 1 && variant != 2) return fail(OIDT_EINVAL, "variant must be 1 or 2");
-  b->variant = variant;
-  if (xchunk != 1 && xchunk != 2 && xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32)
-    return fail(OIDT_EINVAL, "xchunk must be 1, 2, 4, 8, 16 or 32");
+  if (xchunk != 0 && xchunk != 1 && xchunk != 2 && xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32)
+    return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
-  b->xchunk = xchunk;
+  b->variant = variant;
+  if (xchunk) b->xchunk = xchunk;
   b->grid_override = grid_blocks;
   b->dirty = true;
   return OIDT_OK;
@@ -757,6 +767,14 @@ int oidt_batch_poll(oidt_batch* b, int32_t slot, int32_t* steps_done) {
   return OIDT_OK;
 }
 
+int oidt_batch_progress(oidt_batch* b, int64_t* chunks_enqueued, int64_t* chunks_done) {
+  if (!b || !chunks_enqueued || !chunks_done) return fail(OIDT_EINVAL, "null argument");
+  reap_marks(b);
+  *chunks_enqueued = b->chunks_enqueued;
+  *chunks_done = b->chunks_done;
+  return OIDT_OK;
+}
+
 int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp, double* re, double* im) {
   int rc = slot_ok(b, slot);
   if (rc != OIDT_OK) return rc;
@@ -770,7 +788,9 @@ int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp,
   const size_t off = d.precision == OIDT_F64 ? layout<double>(d).off_acc : layout<float>(d).off_acc;
   const int q = port * ncomp + comp;
   std::vector<double> buf(2 * n);
-  HIP_TRY(hipStreamSynchronize(b->stream));
+  // A finished run's accumulators are final while later chunks run for the
+  // others: read without waiting for the batch stream.
+  if (!slot_finished(b, sh)) HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpy(buf.data(), static_cast<char*>(sh.dmem) + off + (size_t)q * n * 2 * sizeof(double),
                     2 * n * sizeof(double), hipMemcpyDeviceToHost));
   for (size_t e = 0; e < n; ++e) {

@@ -220,6 +220,12 @@ class Batch:
     def sync(self):
         check(lib().oidt_batch_sync(self.h))
 
+    def progress(self):
+        """(chunks enqueued, chunks completed by the GPU), non-blocking."""
+        e, d = C.c_int64(), C.c_int64()
+        check(lib().oidt_batch_progress(self.h, C.byref(e), C.byref(d)))
+        return e.value, d.value
+
     def poll(self, slot):
         n = C.c_int32()
         check(lib().oidt_batch_poll(self.h, int(slot), C.byref(n)))

@@ -1,0 +1,81 @@
+"""The C ABI library loads and exports every symbol include/oidt_fdtd.h
+declares; host-side validation errors need no GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oidt_amd import _lib, configs, params
+from oidt_amd.solver import RunArrays, footprint
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, 'include', 'oidt_fdtd.h')).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return set(re.findall(r'\b(oidt_[a-z_0-9]+)\s*\(', text))
+
+
+def test_header_and_binding_agree():
+    assert declared() == set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    for name in declared():
+        assert hasattr(L, name), name
+    assert L.oidt_abi_version() == _lib.ABI_VERSION
+    assert _lib.chunk_steps() == 32
+
+
+def _desc(kind='tm', **over):
+    p = configs.small(kind, 'float32')
+    s = params.explicit_setup(p)
+    ra = RunArrays(s, np.zeros(s.shape, bool))
+    d = ra.desc()
+    for k, v in over.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize('field,value,msg', [
+    ('abi_version', 99, 'abi_version'), ('scheme', 7, 'scheme'), ('precision', 5, 'precision'),
+    ('nz', 4, 'nz == 1'), ('num_cpml', 60, 'CPML'), ('nt', -1, 'negative')])
+def test_validation_errors(field, value, msg):
+    d = _desc(**{field: value})
+    b = C.c_uint64()
+    rc = _lib.lib().oidt_run_footprint(C.byref(d), C.byref(b))
+    assert rc == _lib.EINVAL
+    assert msg in _lib.lib().oidt_last_error().decode()
+
+
+def test_bad_port_and_source_rejected():
+    d = _desc()
+    d.port[1] = d.nx - 1
+    assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.EINVAL
+    d = _desc()
+    d.src_field = _lib.HZ          # TMz drives Ez only
+    assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.EINVAL
+
+
+def test_3d_tm_source_box_in_cpml_stencil_rejected():
+    p = configs.small('3d', 'float32', source_condition='TM')
+    s = params.explicit_setup(p)
+    ra = RunArrays(s, np.zeros(s.shape, bool))
+    d = ra.desc()
+    assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.OK
+    d.src_box[4], d.src_box[5] = 3, 5      # inside the low z CPML slab
+    assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.EINVAL
+    assert 'CPML' in _lib.lib().oidt_last_error().decode()
+
+
+def test_footprint_scales_with_grid():
+    a = footprint(params.explicit_setup(configs.slab2d(800, precision='float32', nt=100)))
+    b = footprint(params.explicit_setup(configs.slab2d(1600, precision='float32', nt=100)))
+    assert 1.6 < b / a < 2.1      # fixed per-run state (DFT ring and accumulators) on top
+    c = footprint(params.explicit_setup(configs.config1()))
+    # 6 FP32 fields + 1-byte material map + psi + DFT state of the 128x96x320 run
+    assert 100e6 < c < 250e6

@@ -1,0 +1,47 @@
+"""Serving on the device: continuous batching admits requests while others run;
+every result is bit-identical to the same request solved alone."""
+import numpy as np
+import pytest
+
+from oidt_amd import configs, serve
+from oidt_amd.solver import Solver
+
+
+def _requests():
+    descs = []
+    for i, (scheme, extra) in enumerate([('tm', 0), ('tm', 30), ('3d', 0), ('tm', 60), ('3d', 4), ('tm', 90)]):
+        if scheme == 'tm':
+            p = configs.small('tm', 'float32', nt=300 + 64 * i)
+            p['nx'] += extra
+            p['p2'] += extra
+        else:
+            p = configs.small('3d', 'float32', nt=200 + 32 * i)
+            p['nz'] += extra
+            p['p2'] += extra
+        p.update(rough_std=(5 + 2 * i) * 1e-9, rough_acl=40e-9, seed=100 + i)
+        descs.append(p)
+    return descs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('budget_runs', [100, 2])
+def test_results_identical_alone_and_in_mix(budget_runs):
+    descs = _requests()
+    reqs = serve.make_requests(descs, [0.0, 0.002, 0.004, 0.006, 0.02, 0.03])
+    be = serve.DeviceBackend(max_slots=8)
+    fp = max(be.footprint(configs.realise(d)[0]) for d in descs)
+    srv = serve.Server(be, hbm_budget=budget_runs * fp, chunk_group=2)
+    done = srv.run(reqs)
+    be.close()
+    assert all(r.error is None for r in done)
+    if budget_runs == 2:
+        assert srv.peak_resident <= 2
+    else:
+        assert srv.peak_resident >= 3            # requests overlapped on the device
+    for r in done:
+        print('request', r.rid, r.setup.scheme, r.setup.shape, r.setup.nt, 'latency %.3f' % r.latency, flush=True)
+        alone = Solver(r.setup, r.fg).run()
+        ref = alone.dft()
+        alone.close()
+        for k, z in ref.items():
+            assert np.array_equal(z, r.result[k]), (r.rid, k)
