@@ -34,8 +34,9 @@ def test_results_identical_alone_and_in_mix(budget_runs):
     done = srv.run(reqs)
     be.close()
     assert all(r.error is None for r in done)
+    assert srv.peak_bytes <= budget_runs * fp
     if budget_runs == 2:
-        assert srv.peak_resident <= 2
+        assert srv.peak_resident < len(reqs)      # admission waited for memory
     else:
         assert srv.peak_resident >= 3            # requests overlapped on the device
     for r in done:
