@@ -224,7 +224,9 @@ def main():
     pb.close()
     es = 4 if prec == np.float32 else 8
     be, bh = algorithmic_bytes(scheme, es)
-    if ms_e >= ms_h:
+    if args.variant == 3:
+        kname, ms_k, bytes_k = ('fused step', ms_e, (be + bh) * cells)
+    elif ms_e >= ms_h:
         kname, ms_k, bytes_k = ('E half-step', ms_e, be * cells)
     else:
         kname, ms_k, bytes_k = ('H half-step', ms_h, bh * cells)

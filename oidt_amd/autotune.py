@@ -1,0 +1,92 @@
+"""Launch-shape autotuner for the stencil kernels.
+
+For a set of runs (one batch: scheme, precision, grid sizes) it times every
+candidate (kernel form, x-chunk depth) with HIP events on the batch's own
+stream (``oidt_batch_profile``) on a copy of the real batch, and keeps the one
+with the smallest time per step.  Results are cached per workload class in a
+JSON file so a serving process tunes each class once.
+
+Kernel forms: 1 = per-plane march, 2 = batched loads (two launches per step),
+3 = fused Yee step (one launch per step; TMz).  The x-chunk depth is the number
+of cells each thread owns along x (the z-block depth of the tiles in 3D).
+"""
+import ctypes as C
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from .solver import Batch
+
+CANDIDATES = {
+    'tm': [(2, 2), (2, 4), (2, 8), (3, 1), (3, 2), (3, 4)],
+    'te': [(2, 2), (2, 4), (2, 8), (1, 4)],
+    '3d': [(2, 1), (2, 2), (2, 4), (1, 4)],
+}
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'oidt_amd', 'autotune.json')
+
+
+def workload_class(setups):
+    """Key of a batch for the cache: scheme, precision, run count and total
+    cells rounded to a power of two (the tuning depends on occupancy, not on
+    exact sizes)."""
+    s0 = setups[0]
+    cells = sum(s.cells for s in setups)
+    return '%s-%s-n%d-c%d' % (s0.scheme, np.dtype(s0.precision).name, len(setups),
+                              int(round(np.log2(max(cells, 1)))))
+
+
+def step_time(runs, variant, xchunk, chunks=2):
+    """Mean device time per time step (ms) of one candidate on ``runs``
+    [(setup, fg), ...]: stencil launches of a chunk plus its DFT fold."""
+    s0 = runs[0][0]
+    b = Batch(s0.scheme, s0.precision, max_slots=len(runs))
+    try:
+        b.set_launch(xchunk, 0, variant)
+        for s, fg in runs:
+            b.add(s, fg)
+        out = (C.c_double * 4)()
+        _lib.check(_lib.lib().oidt_batch_profile(b.h, 1, out))            # warm-up chunk
+        _lib.check(_lib.lib().oidt_batch_profile(b.h, int(chunks), out))
+        chunk = _lib.chunk_steps()
+        return out[0] + out[1] + (out[2] + out[3]) / chunk
+    finally:
+        b.close()
+
+
+def tune(runs, candidates=None, cache_path=DEFAULT_CACHE, use_cache=True, verbose=False):
+    """Best (variant, xchunk) for ``runs``; returns (choice, {candidate: ms/step})."""
+    setups = [s for s, _ in runs]
+    key = workload_class(setups)
+    cache = {}
+    if use_cache and cache_path and os.path.exists(cache_path):
+        try:
+            with open(cache_path) as fh:
+                cache = json.load(fh)
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            return tuple(cache[key]['choice']), cache[key]['times']
+    scheme = setups[0].scheme
+    cands = candidates or CANDIDATES[scheme]
+    # short runs: tune on the first chunks of each run
+    short = [(s, fg) for s, fg in runs]
+    times = {}
+    for variant, xc in cands:
+        t0 = time.perf_counter()
+        ms = step_time(short, variant, xc)
+        times['%d,%d' % (variant, xc)] = ms
+        if verbose:
+            print('autotune %s variant %d xchunk %d: %.4f ms/step (%.1f s)' % (key, variant, xc, ms,
+                                                                               time.perf_counter() - t0), flush=True)
+    best = min(times, key=times.get)
+    choice = tuple(int(v) for v in best.split(','))
+    if cache_path:
+        cache[key] = {'choice': list(choice), 'times': times}
+        os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+        with open(cache_path, 'w') as fh:
+            json.dump(cache, fh, indent=1, sort_keys=True)
+    return choice, times

@@ -89,6 +89,7 @@ struct SlotHost {
   size_t bytes = 0;
   std::vector<unsigned char> slot;  // host image of Slot<T>
   int64_t admit_chunk = 0;          // batch chunk counter when admitted
+  bool fused = false;               // advanced by the fused step kernel (alternating field sets)
   int nplanes = 0, nq = 0, wcells = 0, pcells = 0, wyn = 0;
 };
 
@@ -159,7 +160,7 @@ int validate(const oidt_run_desc* d) {
 
 template <typename T>
 struct Layout {
-  size_t off_f[6], off_psi[36], off_mat, off_ike[3], off_ikh[3], off_be, off_ceo, off_bh, off_cho,
+  size_t off_f[6], off_psi[36], off_f2[6], off_psi2[36], off_mat, off_ike[3], off_ikh[3], off_be, off_ceo, off_bh, off_cho,
       off_jsrc, off_ring, off_acc, off_w, total;
   int nplanes, nq, wcells, pcells, prow, wx0, wx1, wy0, wy1;
 };
@@ -175,11 +176,17 @@ Layout<T> layout(const oidt_run_desc& d) {
     off = align_up(off + bytes, 256);
     return o;
   };
+  // TMz runs hold a second set of fields and psi: the fused step form reads
+  // one set and writes the other.  The other schemes update in place.
+  const bool two = d.scheme == SCHEME_TMZ;
   for (int f = 0; f < 6; ++f) L.off_f[f] = field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
-  for (int i = 0; i < 36; ++i) L.off_psi[i] = (size_t)-1;
+  for (int f = 0; f < 6; ++f) L.off_f2[f] = two && field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
+  for (int i = 0; i < 36; ++i) L.off_psi[i] = L.off_psi2[i] = (size_t)-1;
   for (auto pr : psi_pairs(d.scheme))
-    for (int side = 0; side < 2; ++side)
+    for (int side = 0; side < 2; ++side) {
       L.off_psi[psi_id(pr.first, pr.second, side)] = take(psi_elems(d, pr.second) * sizeof(T));
+      if (two) L.off_psi2[psi_id(pr.first, pr.second, side)] = take(psi_elems(d, pr.second) * sizeof(T));
+    }
   L.off_mat = take(cells);
   for (int a = 0; a < 3; ++a) {
     const int n = a == 0 ? d.nx : (a == 1 ? d.ny : d.nz);
@@ -374,6 +381,8 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   s.dt = d.dt;
   for (int f = 0; f < 6; ++f) s.f[f] = L.off_f[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f[f]);
   for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi[i]);
+  for (int f = 0; f < 6; ++f) s.f2[f] = L.off_f2[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f2[f]);
+  for (int i = 0; i < 36; ++i) s.psi2[i] = L.off_psi2[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi2[i]);
   s.mat = reinterpret_cast<const uint8_t*>(base + L.off_mat);
   for (int c = 0; c < 3; ++c)
     for (int q = 0; q < kMatMax; ++q) s.cbv[c][q] = cbv[c][q];
@@ -460,6 +469,10 @@ void launch_half(oidt_batch* b, int gs, int tstep, bool e_half) {
   const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
   const dim3 blk(kBX, kBY), grd(gs);
   hipStream_t st = b->stream;
+  if (b->variant == 3) {   // fused step: the whole step in the "E" launch
+    if (e_half) hipLaunchKernelGGL((k_tmz_step<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+    return;
+  }
   const bool v1 = b->variant == 1;
   switch (b->scheme) {
     case SCHEME_TMZ:
@@ -524,7 +537,7 @@ int enqueue_chunk(oidt_batch* b) {
   const int gs = stencil_grid(b), gd = dft_grid(b);
   for (int t = 0; t < kChunk; ++t) {
     launch_stencil<T>(b, gs, t, true);
-    launch_stencil<T>(b, gs, t, false);
+    if (b->variant != 3) launch_stencil<T>(b, gs, t, false);
   }
   launch_dft<T>(b, gd);
   launch_advance<T>(b);
@@ -564,6 +577,7 @@ int profile_chunks(oidt_batch* b, int nchunks, double* out) {
       HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
       k += 2;
       const int kind = i < 2 * kChunk ? (i & 1) : (i == 2 * kChunk ? 2 : 3);
+      if (b->variant == 3 && kind == 1) continue;   // fused step: no separate H launch
       sum[kind] += ms;
       cnt[kind] += 1;
     }
@@ -708,6 +722,7 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
   if (rc != OIDT_OK) return rc;
   sh.used = true;
   sh.admit_chunk = b->chunks_enqueued;
+  sh.fused = b->variant == 3;
   b->dirty = true;
   *slot = si;
   return OIDT_OK;
@@ -715,11 +730,17 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
-  if (variant != 1 && variant != 2) return fail(OIDT_EINVAL, "variant must be 1 or 2");
+  if (variant < 1 || variant > 3) return fail(OIDT_EINVAL, "variant must be 1, 2 or 3");
+  if (variant == 3 && b->scheme != SCHEME_TMZ) return fail(OIDT_EINVAL, "the fused step form exists for TMz only");
+  for (const auto& sh : b->slots)
+    if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.fused != (variant == 3))
+      return fail(OIDT_ESTATE, "cannot switch between fused and two-kernel forms while a run is in progress");
   if (xchunk != 0 && xchunk != 1 && xchunk != 2 && xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32)
     return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   b->variant = variant;
+  for (auto& sh : b->slots)
+    if (sh.used && b->chunks_enqueued == sh.admit_chunk) sh.fused = variant == 3;
   if (xchunk) b->xchunk = xchunk;
   b->grid_override = grid_blocks;
   b->dirty = true;
@@ -809,8 +830,14 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
   const size_t es = d.precision == OIDT_F64 ? 8 : 4;
   const size_t bytes = (size_t)d.nx * d.ny * d.nz * es;
   if (!host || nbytes != bytes) return fail(OIDT_EINVAL, "host buffer size mismatch");
-  const size_t off = d.precision == OIDT_F64 ? layout<double>(d).off_f[field] : layout<float>(d).off_f[field];
   HIP_TRY(hipStreamSynchronize(b->stream));
+  reap_marks(b);
+  // The fused step alternates field sets: after an odd number of completed
+  // steps the state is in set B.
+  const int64_t steps = std::min<int64_t>(d.nt, std::max<int64_t>(0, b->chunks_done - sh.admit_chunk) * kChunk);
+  const bool in_b = sh.fused && (steps & 1);
+  const size_t off = d.precision == OIDT_F64 ? (in_b ? layout<double>(d).off_f2[field] : layout<double>(d).off_f[field])
+                                             : (in_b ? layout<float>(d).off_f2[field] : layout<float>(d).off_f[field]);
   HIP_TRY(hipMemcpy(host, static_cast<char*>(sh.dmem) + off, bytes, hipMemcpyDeviceToHost));
   return OIDT_OK;
 }

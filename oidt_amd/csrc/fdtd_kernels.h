@@ -1140,6 +1140,206 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
 
 #undef OIDT_TILE_PROLOGUE
 
+
+// ---------------------------------------------------------- fused Yee step ---
+// One launch per time step for TMz: E and H half-steps together, reading
+// field/psi set (n & 1) and writing the other set, so no tile races another.
+// A block owns 64 aligned rows (y) x 4*XC planes (x); wave w owns planes
+// i0 + w*XC .. +XC.  New Ez of the block goes to LDS; the H half-step needs it
+// at j-1 and i-1, so wave 0 also computes the plane below the block (i0-1) and
+// wave 1 the row below it (j0-1), neither stored.  Halo values come from the
+// same per-cell code as the owners', so they are bit-identical to them.  Per
+// cell and step each field is read once and written once (25 B in 2D FP32
+// instead of 37 B for the two half-step kernels).
+
+// New Ez of one cell (bulk curl, source, psi x, psi y), reading set A only.
+template <typename T>
+__device__ __forceinline__ T tmz_e_cell(const Slot<T>& s, int i, int j, int n, const T* __restrict__ EzA,
+                                        const T* __restrict__ HxA, const T* __restrict__ HyA, T* const* psiA,
+                                        T* const* psiB, bool store) {
+  const int nx = s.nx, ny = s.ny, N = s.N;
+  if (i < 0 || j < 0 || i >= nx || j >= ny) return T(0);
+  const unsigned idx = (unsigned)i * ny + j;
+  T v = EzA[idx];
+  const bool bulk = i >= s.ib && i < s.ie && j >= s.jb && j < s.je;
+  const bool src = s.src_field == F_EZ && i >= s.sx0 && i < s.sx1 && j >= s.sy0 && j < s.sy1;
+  const T jv = src ? s.jsrc[n] : T(0);
+  if (bulk) {
+    const T cb = cbsel(s, s.mat[idx], 2);
+    const T dHy = HyA[idx + ny] - HyA[idx];
+    const T dHx = HxA[idx + 1] - HxA[idx];
+    v = v + cb * (dHy * s.ike[0][i] - dHx * s.ike[1][j]);
+    if (src) v -= jv;
+    int dx = 0, cx = 0, dy = 0, cy = 0;
+    const int sx = slab_e(i, s.ib, s.ie, N, dx, cx);
+    const int sy = slab_e(j, s.jb, s.je, N, dy, cy);
+    if (sx >= 0) {
+      const T p = s.be[cx] * psiA[psi_id(F_EZ, 0, sx)][(size_t)dx * ny + j] + s.ceo[cx] * dHy;
+      if (store) psiB[psi_id(F_EZ, 0, sx)][(size_t)dx * ny + j] = p;
+      v += cb * p;
+    }
+    if (sy >= 0) {
+      const T p = s.be[cy] * psiA[psi_id(F_EZ, 1, sy)][(size_t)i * N + dy] + s.ceo[cy] * dHx;
+      if (store) psiB[psi_id(F_EZ, 1, sy)][(size_t)i * N + dy] = p;
+      v -= cb * p;
+    }
+  } else if (src) {
+    v -= jv;
+  }
+  return v;
+}
+
+template <typename T, int XC>
+__global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict__ slots,
+                                                       const int4* __restrict__ tiles,
+                                                       const int* __restrict__ counts, int tstep) {
+  // eL[r][c]: new Ez of plane i0 - 1 + r, row j0 - 1 + c
+  __shared__ T eL[kBY * XC + 1][kBX + 1];
+  const int ntiles = counts[0];
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const int nx = s.nx, ny = s.ny, N = s.N;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int j = tl.z + lane;
+    const int jc = min(j, ny - 1);
+    const bool jok = j < ny;
+    const int i0 = tl.y;                 // first plane of the block
+    const int iw = i0 + w * XC;          // first plane of this wave
+    const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je;
+    const bool odd = (n & 1) != 0;
+    const T* __restrict__ EzA = odd ? s.f2[F_EZ] : s.f[F_EZ];
+    const T* __restrict__ HxA = odd ? s.f2[F_HX] : s.f[F_HX];
+    const T* __restrict__ HyA = odd ? s.f2[F_HY] : s.f[F_HY];
+    T* __restrict__ EzB = odd ? s.f[F_EZ] : s.f2[F_EZ];
+    T* __restrict__ HxB = odd ? s.f[F_HX] : s.f2[F_HX];
+    T* __restrict__ HyB = odd ? s.f[F_HY] : s.f2[F_HY];
+    T* const* psiA = odd ? s.psi2 : s.psi;
+    T* const* psiB = odd ? s.psi : s.psi2;
+    const T db = s.db;
+    const bool jin = jok && j >= jb && j < je, jin1 = jok && j > jb && j < je;
+    int dye = 0, cye = 0, dyh = 0, cyh = 0;
+    const int sye = jin ? slab_e(j, jb, je, N, dye, cye) : -1;
+    const int syh = jin1 ? slab_h(j, jb, je, N, dyh, cyh) : -1;
+    const T ikye = jin ? s.ike[1][j] : T(0);
+    const T ikyh = jin1 ? s.ikh[1][j] : T(0);
+    const T bey = sye >= 0 ? s.be[cye] : T(0), cey = sye >= 0 ? s.ceo[cye] : T(0);
+    const T bhy = syh >= 0 ? s.bh[cyh] : T(0), chy = syh >= 0 ? s.cho[cyh] : T(0);
+    const bool jsrc = jok && s.src_field == F_EZ && j >= s.sy0 && j < s.sy1;
+    const T jv = jsrc ? s.jsrc[n] : T(0);
+    const bool jcap = jok && j >= s.wy0 && j <= s.wy1;
+    const unsigned offj = (jc + 1 < ny) ? 1u : 0u;
+    // ---- loads of the wave's XC planes (clamped, all before any store)
+    T ez[XC], hx[XC], hxj[XC], hy[XC + 1], pzx[XC], pzy[XC], pxy[XC], pyx[XC];
+    uint8_t m[XC];
+#pragma unroll
+    for (int u = 0; u <= XC; ++u) {
+      const int i = min(iw + u, nx - 1);
+      const unsigned idx = (unsigned)i * ny + jc;
+      hy[u] = HyA[idx];
+      if (u < XC) {
+        ez[u] = EzA[idx];
+        hx[u] = HxA[idx];
+        hxj[u] = HxA[idx + offj];
+        m[u] = s.mat[idx];
+        pzy[u] = sye >= 0 ? psiA[psi_id(F_EZ, 1, sye)][(size_t)i * N + dye] : T(0);
+        pxy[u] = syh >= 0 ? psiA[psi_id(F_HX, 1, syh)][(size_t)i * N + dyh] : T(0);
+        int dx = 0, cx = 0;
+        const int sxe = slab_e(i, ib, ie, N, dx, cx);
+        pzx[u] = sxe >= 0 ? psiA[psi_id(F_EZ, 0, sxe)][(size_t)dx * ny + jc] : T(0);
+        const int sxh = slab_h(i, ib, ie, N, dx, cx);
+        pyx[u] = sxh >= 0 ? psiA[psi_id(F_HY, 0, sxh)][(size_t)dx * ny + jc] : T(0);
+      }
+    }
+    // ---- E half-step of the owned cells
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      const int i = iw + u;
+      const bool iok = i < nx;
+      const unsigned idx = (unsigned)min(i, nx - 1) * ny + jc;
+      const bool bulk = iok && jin && i >= ib && i < ie;
+      const bool src = iok && jsrc && i >= s.sx0 && i < s.sx1;
+      T v = ez[u];
+      if (bulk) {
+        const T cb = cbsel(s, m[u], 2);
+        const T dHy = hy[u + 1] - hy[u];
+        const T dHx = hxj[u] - hx[u];
+        v = v + cb * (dHy * s.ike[0][i] - dHx * ikye);
+        if (src) v -= jv;
+        int dx = 0, cx = 0;
+        const int sx = slab_e(i, ib, ie, N, dx, cx);
+        if (sx >= 0) {
+          const T p = s.be[cx] * pzx[u] + s.ceo[cx] * dHy;
+          psiB[psi_id(F_EZ, 0, sx)][(size_t)dx * ny + j] = p;
+          v += cb * p;
+        }
+        if (sye >= 0) {
+          const T p = bey * pzy[u] + cey * dHx;
+          psiB[psi_id(F_EZ, 1, sye)][(size_t)i * N + dye] = p;
+          v -= cb * p;
+        }
+        EzB[idx] = v;
+      } else if (src) {
+        v -= jv;
+        EzB[idx] = v;
+      }
+      eL[1 + w * XC + u][1 + lane] = v;
+      if (jcap && iok) {
+        if (i == s.port[0]) capture(s, tstep, 0, j - s.wy0, v);
+        if (i == s.port[1]) capture(s, tstep, 3, j - s.wy0, v);
+      }
+    }
+    // ---- halo: plane i0-1 (wave 0) and row j0-1 (wave 1), not stored
+    if (w == 0) eL[0][1 + lane] = tmz_e_cell(s, i0 - 1, j, n, EzA, HxA, HyA, psiA, psiB, false);
+    if (w == 1)
+      for (int r = lane; r <= kBY * XC; r += kBX)
+        eL[r][0] = tmz_e_cell(s, i0 - 1 + r, tl.z - 1, n, EzA, HxA, HyA, psiA, psiB, false);
+    __syncthreads();
+    // ---- H half-step of the owned cells
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      const int i = iw + u;
+      if (!jok || i >= nx) continue;
+      const unsigned idx = (unsigned)i * ny + j;
+      const int r = 1 + w * XC + u;
+      const T ezc = eL[r][1 + lane];
+      if (i >= ib && i < ie && jin1) {
+        const T dEz = ezc - eL[r][lane];
+        T v = hx[u] + db * (-dEz) * ikyh;
+        if (syh >= 0) {
+          const T p = bhy * pxy[u] + chy * dEz;
+          psiB[psi_id(F_HX, 1, syh)][(size_t)i * N + dyh] = p;
+          v -= db * p;
+        }
+        HxB[idx] = v;
+      }
+      T v = hy[u];
+      if (i > ib && i < ie && jin) {
+        const T dEz = ezc - eL[r - 1][1 + lane];
+        v = v + db * dEz * s.ikh[0][i];
+        int dx = 0, cx = 0;
+        const int sx = slab_h(i, ib, ie, N, dx, cx);
+        if (sx >= 0) {
+          const T p = s.bh[cx] * pyx[u] + s.cho[cx] * dEz;
+          psiB[psi_id(F_HY, 0, sx)][(size_t)dx * ny + j] = p;
+          v += db * p;
+        }
+        HyB[idx] = v;
+      }
+      if (jcap) {
+        const int c = j - s.wy0;
+        if (i == s.port[0]) capture(s, tstep, 1, c, v);
+        if (i == s.port[0] + 1) capture(s, tstep, 2, c, v);
+        if (i == s.port[1]) capture(s, tstep, 4, c, v);
+        if (i == s.port[1] + 1) capture(s, tstep, 5, c, v);
+      }
+    }
+    __syncthreads();   // eL is reused by the next tile
+  }
+}
+
 // ------------------------------------------------------------------- DFT ---
 // Work item (slot, q, cell block, frequency block).  Folds the kChunk samples
 // of the chunk in flight into acc[q][f][cell] (re += s cos, im -= s sin, as

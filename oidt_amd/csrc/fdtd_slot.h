@@ -51,8 +51,10 @@ struct Slot {
   int nq;                  // DFT sample components
   T db;
   double dt;
-  T* f[6];
+  T* f[6];       // field set A (current state of the two-kernel forms)
   T* psi[36];
+  T* f2[6];      // field set B: the fused step reads one set and writes the other
+  T* psi2[36];
   T cbv[3][kMatMax];    // distinct cb values of Ex, Ey, Ez
   const uint8_t* mat;   // per cell: bits 2c..2c+1 index cbv[c]
   const T* ike[3];      // 1 / (kappa_e * dx) per axis

@@ -56,8 +56,10 @@ def prepare(scheme, usr, line, direction, kind, workdir='.'):
     s = params.auto_setup(usr, scheme, kind)
     mode = profile_mode(usr, line, direction, kind)
     length = (s.p2z - s.p1z) if scheme == '3d' else (s.p2x - s.p1x)
-    up_name, lo_name = roughness.profile_names(os.path.join(workdir, usr.get('prof_dir', 'rough_profiles')),
-                                               usr['rough_std'], usr['rough_acl'], usr.get('profile_number', 0))
+    up_rel, lo_rel = roughness.profile_names(usr.get('prof_dir', 'rough_profiles'), usr['rough_std'],
+                                             usr['rough_acl'], usr.get('profile_number', 0))
+    up_name = os.path.normpath(os.path.join(workdir, up_rel))
+    lo_name = os.path.normpath(os.path.join(workdir, lo_rel))
     if mode == 'smooth':
         up, lo = roughness.generate_profiles('smooth', length, s, usr)
     elif mode == 'gen':

@@ -115,7 +115,9 @@ def generate_profiles(mode, length, setup, usr, upper_name='', lower_name='', se
         raise Exception('Choose a valid roughness profile mode')
     # The reference also saves in 'smooth' mode, where the names are '' and numpy
     # writes a stray '.npy' into the working directory; empty names are skipped.
-    os.makedirs(usr.get('prof_dir', 'rough_profiles'), exist_ok=True)
+    for name in (upper_name, lower_name):
+        if name and os.path.dirname(name):
+            os.makedirs(os.path.dirname(name), exist_ok=True)
     if upper_name:
         np.save(upper_name, up)
     if lower_name:

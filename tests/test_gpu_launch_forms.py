@@ -9,6 +9,7 @@ from oidt_amd.solver import Batch
 from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
+FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step (TMz)
 
 
 @pytest.mark.gpu
@@ -21,7 +22,7 @@ def test_all_launch_forms_match_oracle(scheme):
     ref_f = o.fields()
     od = o.dft()
     w = window(scheme, o.c)
-    for variant, xc in FORMS:
+    for variant, xc in FORMS + (FUSED if scheme == 'tm' else []):
         print('case', scheme, 'variant', variant, 'xchunk', xc, flush=True)
         b = Batch(scheme, np.float32, max_slots=1)
         b.set_launch(xc, 0, variant)
