@@ -36,19 +36,61 @@ def algorithmic_bytes(scheme, es):
     return (6 + 3) * es, (6 + 3) * es
 
 
-def cpu_baseline(p, steps):
-    """Oracle (FP64 NumPy restatement, one thread) on the same grid, bounded sample."""
+def cpu_baseline(p, seconds):
+    """Oracle (FP64 NumPy restatement, one thread) on the same grid: whole
+    steps until ``seconds`` of CPU work have run (a bounded sample)."""
     from oracle import fdtd as ofdtd
     from oracle import geometry as ogeo
     from oracle import setup as osetup
     c = osetup.explicit_setup(p)
     o = ofdtd.make(c, ogeo.fg_mask(c))
     o.run(1)
+    steps = 0
     t0 = time.perf_counter()
-    o.run(steps)
-    dt = time.perf_counter() - t0
+    while True:
+        o.run(1)
+        steps += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
     cells = c.nx * c.ny * (c.nz if c.scheme == '3d' else 1)
-    return cells * steps / dt / 1e6, dt
+    return cells * steps / dt / 1e6, dt, steps
+
+
+def committed_traffic(kernel, workload_key, variant, xchunk):
+    """HBM bytes per launch of ``kernel`` from the rocprofv3 counter summaries
+    under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the
+    copy-kernel calibration); None when no summary matches this launch form.
+    A summary states it on a line
+    ``traffic: kernel=K workload=W variant=V xchunk=X bytes_per_launch=B``."""
+    d = os.path.join(ROOT, 'profiles')
+    if not os.path.isdir(d):
+        return None, None
+    want = {'kernel': kernel, 'workload': workload_key, 'variant': str(variant), 'xchunk': str(xchunk)}
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(('.md', '.txt')):
+            continue
+        with open(os.path.join(d, name)) as fh:
+            for line in fh:
+                line = line.strip().strip('`')
+                if not line.startswith('traffic:'):
+                    continue
+                kv = dict(t.split('=', 1) for t in line[len('traffic:'):].split() if '=' in t)
+                if all(kv.get(k) == v for k, v in want.items()):
+                    return int(kv['bytes_per_launch']), 'profiles/' + name
+    return None, None
+
+
+def choose_launch(args, runs):
+    """(variant, xchunk, autotune record) for the batch of ``runs``: the
+    command-line form, or the autotuner's pick when --variant is 0."""
+    if args.variant:
+        return args.variant, args.xchunk, None
+    from oidt_amd import autotune
+    t0 = time.perf_counter()
+    (variant, xc), times = autotune.tune(runs, use_cache=False, cache_path=None)
+    return variant, xc, {'ms_per_step': {k: round(v, 5) for k, v in times.items()},
+                         'seconds': round(time.perf_counter() - t0, 2)}
 
 
 def workload(name):
@@ -69,18 +111,26 @@ def bench_serving(args):
     Poisson arrivals at 0.8x the saturated capacity (latency percentiles)."""
     from oidt_amd import configs, serve
     descs = configs.serving_requests(count=args.requests)
-    be = serve.DeviceBackend(max_slots=64, xchunk=args.xchunk, variant=args.variant)
+    be = serve.DeviceBackend(max_slots=64, xchunk=args.xchunk, variant=args.variant or 2)
     reqs = serve.make_requests(descs, [0.0] * len(descs))
     t0 = time.perf_counter()
     for r in reqs:
         serve.prepare_request(r, be)
     prep = time.perf_counter() - t0
-    total_cu = sum(r.cell_updates for r in reqs)
-    # plain: each request alone, back to back
+    tuned = {}
+    if not args.variant:
+        # autotune each scheme once on (up to) 16 of its requests
+        for scheme in sorted({r.setup.scheme for r in reqs}):
+            sub = [(r.setup, r.fg) for r in reqs if r.setup.scheme == scheme][:16]
+            variant, xchunk, rec = choose_launch(args, sub)
+            be.launch[scheme] = (variant, xchunk)
+            tuned[scheme] = dict(rec, choice=[variant, xchunk])
+    # plain: each request alone, back to back, same kernel forms
     t0 = time.perf_counter()
-    plain = serve.serve_plain(reqs)
+    plain = serve.serve_plain(reqs, be.launch)
     plain_wall = time.perf_counter() - t0
     plain_rate = total_cu / plain_wall / 1e6
+    total_cu = sum(r.cell_updates for r in reqs)
     # saturated continuous batching (everything queued at t = 0)
     hbm = int(0.8 * __import__('oidt_amd._lib', fromlist=['x']).device_info()['hbm_free_bytes'])
     for r in reqs:
@@ -111,7 +161,8 @@ def bench_serving(args):
         'data': 'synthetic',
         'config': {'workload': 'serving mix: %d requests, %d 2D TMz (nx 400..4000, ny 200..240) and %d 3D '
                                '((128..384)x(64..128)x(48..96)), 2000..8000 steps, FP32' % (len(reqs), n2d, len(reqs) - n2d),
-                   'max_slots_per_batch': 64, 'hbm_budget_bytes': hbm},
+                   'max_slots_per_batch': 64, 'hbm_budget_bytes': hbm, 'launch': be.launch or None,
+                   'autotune': tuned or None},
         'plain_mcell_updates_per_s': round(plain_rate, 1),
         'served_over_plain': round(sat['mcell_updates_per_s'] / plain_rate, 3),
         'saturated': {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sat.items()},
@@ -123,7 +174,9 @@ def bench_serving(args):
 
 
 def bench_plain_ensemble(args, descs, label):
-    """The ensemble one run at a time (each alone on the device), same steps."""
+    """The ensemble one run at a time (each alone on the device), same steps,
+    same kernel form as the batched path (autotuned on the first run when
+    --variant is 0)."""
     from oidt_amd import configs, materials
     from oidt_amd.solver import Solver
     K = args.steps
@@ -132,11 +185,13 @@ def bench_plain_ensemble(args, descs, label):
         p = dict(p, nt=K)
         s, up, lo = configs.realise(p)
         runs.append((s, materials.fg_mask(s, up, lo)))
+    variant, xchunk, tuned = choose_launch(args, runs[:1])
     # warm the device on the first run
-    Solver(*runs[0]).advance(64).sync().close()
+    Solver(*runs[0]).set_launch(xchunk, 0, variant).advance(64).sync().close()
     t0 = time.perf_counter()
     for s, fg in runs:
         sv = Solver(s, fg)
+        sv.set_launch(xchunk, 0, variant)
         sv.run()
         sv.close()
     wall = time.perf_counter() - t0
@@ -146,7 +201,8 @@ def bench_plain_ensemble(args, descs, label):
                       'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True, 'scaling': 'weak',
                       'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
                       'config': {'workload': label + ', plain: one run at a time incl. create/upload',
-                                 'runs': len(runs)}}), flush=True)
+                                 'runs': len(runs), 'variant': variant, 'xchunk': xchunk, 'autotune': tuned}}),
+          flush=True)
 
 
 def main():
@@ -156,9 +212,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=256)
     ap.add_argument('--workload', default='c1')
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
-    ap.add_argument('--variant', type=int, default=2)
+    ap.add_argument('--variant', type=int, default=0,
+                    help='kernel form: 1 march, 2 batched loads, 3 fused step; 0: autotuner picks form and xchunk')
     ap.add_argument('--profile-chunks', type=int, default=4)
-    ap.add_argument('--cpu-steps', type=int, default=20)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0, help='length of the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--plain', action='store_true', help='ensembles: one run at a time instead of one batch')
     ap.add_argument('--requests', type=int, default=48, help='serving workload: number of requests')
@@ -197,8 +254,10 @@ def main():
         s, up, lo = configs.realise(p)
         runs.append((p, s, materials.fg_mask(s, up, lo)))
     scheme, prec = runs[0][1].scheme, runs[0][1].precision
+    variant, xchunk, tuned = choose_launch(args, [(s, fg) for _, s, fg in runs])
     b = Batch(scheme, prec, max_slots=len(runs))
-    b.set_launch(args.xchunk, 0, args.variant)
+    b.set_launch(xchunk, 0, variant)
+    variant, xchunk, _ = b.launch()
     for p, s, fg in runs:
         b.add(s, fg)
     b.advance(warm // chunk)
@@ -214,7 +273,7 @@ def main():
     # Per-kernel durations with HIP events on the batch stream (a second batch of
     # the same runs, so the timed run above is untouched).
     pb = Batch(scheme, prec, max_slots=len(runs))
-    pb.set_launch(args.xchunk, 0, args.variant)
+    pb.set_launch(xchunk, 0, variant)
     for p, s, fg in runs:
         pb.add(s, fg)
     import ctypes as C
@@ -224,15 +283,18 @@ def main():
     pb.close()
     es = 4 if prec == np.float32 else 8
     be, bh = algorithmic_bytes(scheme, es)
-    if args.variant == 3:
-        kname, ms_k, bytes_k = ('fused step', ms_e, (be + bh) * cells)
+    stem = {'tm': 'k_tmz', 'te': 'k_tez', '3d': 'k_3d'}[scheme]
+    if variant == 3:
+        kname, ms_k, bytes_k = (stem + '_step', ms_e, (be + bh) * cells)
     elif ms_e >= ms_h:
-        kname, ms_k, bytes_k = ('E half-step', ms_e, be * cells)
+        kname, ms_k, bytes_k = (stem + ('_e2' if variant == 2 else '_e'), ms_e, be * cells)
     else:
-        kname, ms_k, bytes_k = ('H half-step', ms_h, bh * cells)
+        kname, ms_k, bytes_k = (stem + ('_h2' if variant == 2 else '_h'), ms_h, bh * cells)
     achieved = bytes_k / (ms_k * 1e-3) / 1e9
+    traffic, traffic_src = committed_traffic(kname, args.workload, variant, xchunk)
     roof = {'bound': 'hbm', 'kernel': kname, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
-            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
+            'traffic_source': traffic_src,
             'bytes_per_launch': bytes_k, 'ms_per_launch': round(ms_k, 5),
             'ms_e': round(ms_e, 5), 'ms_h': round(ms_h, 5), 'ms_dft': round(ms_dft, 5), 'ms_advance': round(ms_adv, 5)}
     line = {
@@ -240,7 +302,8 @@ def main():
         'steps': K, 'warmup': warm, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32' if es == 4 else 'f64', 'data': 'synthetic',
         'config': {'workload': label, 'runs': len(runs), 'cells_per_step': cells,
-                   'steps_per_run': K, 'xchunk': args.xchunk, 'variant': args.variant, 'parallelism': 'single GPU'},
+                   'steps_per_run': K, 'xchunk': xchunk, 'variant': variant, 'parallelism': 'single GPU',
+                   'autotune': tuned},
         'hbm_roofline_frac_step': round(value * 1e6 * (be + bh) / (HBM_PEAK_GBS * 1e9), 4),
         'algorithmic_bytes_per_cell_update': be + bh,
         'roofline': roof,
@@ -249,10 +312,10 @@ def main():
     if not args.no_cpu:
         # CPU leg: the oracle on the first run's grid, bounded sample.
         p0 = dict(descs[0])
-        rate, secs = cpu_baseline(p0, args.cpu_steps)
+        rate, secs, nsteps = cpu_baseline(p0, args.cpu_seconds)
         line['cpu_baseline'] = {'value': round(rate, 3), 'unit': 'Mcell-updates/s', 'cores': 1, 'kind': 'port',
                                 'sample': '%d steps (after 1 warm-up) of the FP64 NumPy oracle on the %dx%dx%d grid '
-                                          'of the first run, %.1f s' % (args.cpu_steps, p0['nx'], p0['ny'],
+                                          'of the first run, %.1f s' % (nsteps, p0['nx'], p0['ny'],
                                                                          p0.get('nz', 1), secs)}
     print(json.dumps(line), flush=True)
 

@@ -125,6 +125,11 @@ int32_t oidt_chunk_steps(void);
  * Replaces fdtd_main.py:216-261 (cuda.to_device of every array). */
 int oidt_solver_create(const oidt_run_desc* desc, const oidt_run_inputs* in, oidt_solver** out);
 
+/* Launch knobs of a single run, as oidt_batch_set_launch; call before the
+ * first oidt_solver_advance to change between fused and two-kernel forms.
+ * No counterpart in the reference (fixed launch shapes, fdtd_main.py:263-273). */
+int oidt_solver_set_launch(oidt_solver* s, int32_t xchunk, int32_t grid_blocks, int32_t variant);
+
 /* Enqueue up to nsteps further steps (rounded up to whole chunks, never past
  * nt).  Asynchronous.  Replaces the body of the time loop fdtd_main.py:275-304. */
 int oidt_solver_advance(oidt_solver* s, int32_t nsteps);
@@ -181,10 +186,16 @@ int oidt_batch_remove(oidt_batch* b, int32_t slot);
 
 /* Launch knobs for the autotuner: x-chunk depth of the stencil tiles (cells per
  * thread along x: 1, 2, 4, 8, 16 or 32; 0 keeps the current value; default 1
- * for 3D and 4 for 2D), the persistent grid size
- * (0 = automatic) and the stencil kernel form (1 = per-plane march, 2 =
- * batched loads per x-chunk, the default). */
+ * for 3D and 4 for 2D), the persistent grid size (0 = automatic) and the
+ * stencil kernel form (1 = per-plane march, 2 = batched loads per x-chunk, the
+ * default; 3 = fused Yee step, one launch per step, TMz and 3D with xchunk at
+ * most 8 in 3D).  The kernel form of a run cannot change between the fused
+ * and the two-kernel forms once the run has started (OIDT_ESTATE).
+ * No counterpart in the reference (fixed launch shapes, fdtd_main.py:263-273). */
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant);
+
+/* Current launch knobs (any pointer may be NULL). */
+int oidt_batch_get_launch(const oidt_batch* b, int32_t* xchunk, int32_t* grid_blocks, int32_t* variant);
 
 int oidt_batch_destroy(oidt_batch* b);
 

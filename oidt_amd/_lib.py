@@ -48,6 +48,7 @@ SIGNATURES = {
     'oidt_chunk_steps': (C.c_int32, []),
     'oidt_solver_create': (C.c_int, [C.POINTER(RunDesc), C.POINTER(RunInputs), C.POINTER(C.c_void_p)]),
     'oidt_solver_advance': (C.c_int, [C.c_void_p, C.c_int32]),
+    'oidt_solver_set_launch': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     'oidt_solver_sync': (C.c_int, [C.c_void_p]),
     'oidt_solver_steps': (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     'oidt_solver_read_dft': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
@@ -63,6 +64,7 @@ SIGNATURES = {
     'oidt_batch_read_field': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_uint64]),
     'oidt_batch_remove': (C.c_int, [C.c_void_p, C.c_int32]),
     'oidt_batch_set_launch': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    'oidt_batch_get_launch': (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     'oidt_batch_destroy': (C.c_int, [C.c_void_p]),
     'oidt_batch_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     'oidt_solver_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),

@@ -7,7 +7,7 @@ with the smallest time per step.  Results are cached per workload class in a
 JSON file so a serving process tunes each class once.
 
 Kernel forms: 1 = per-plane march, 2 = batched loads (two launches per step),
-3 = fused Yee step (one launch per step; TMz).  The x-chunk depth is the number
+3 = fused Yee step (one launch per step; TMz and 3D).  The x-chunk depth is the number
 of cells each thread owns along x (the z-block depth of the tiles in 3D).
 """
 import ctypes as C
@@ -23,7 +23,7 @@ from .solver import Batch
 CANDIDATES = {
     'tm': [(2, 2), (2, 4), (2, 8), (3, 1), (3, 2), (3, 4)],
     'te': [(2, 2), (2, 4), (2, 8), (1, 4)],
-    '3d': [(2, 1), (2, 2), (2, 4), (1, 4)],
+    '3d': [(2, 1), (2, 2), (2, 4), (1, 4), (3, 1), (3, 2), (3, 4)],
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'oidt_amd', 'autotune.json')

@@ -176,9 +176,9 @@ Layout<T> layout(const oidt_run_desc& d) {
     off = align_up(off + bytes, 256);
     return o;
   };
-  // TMz runs hold a second set of fields and psi: the fused step form reads
-  // one set and writes the other.  The other schemes update in place.
-  const bool two = d.scheme == SCHEME_TMZ;
+  // TMz and 3D runs hold a second set of fields and psi: the fused step form
+  // reads one set and writes the other.  TEz updates in place.
+  const bool two = d.scheme != SCHEME_TEZ;
   for (int f = 0; f < 6; ++f) L.off_f[f] = field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
   for (int f = 0; f < 6; ++f) L.off_f2[f] = two && field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
   for (int i = 0; i < 36; ++i) L.off_psi[i] = L.off_psi2[i] = (size_t)-1;
@@ -470,7 +470,12 @@ void launch_half(oidt_batch* b, int gs, int tstep, bool e_half) {
   const dim3 blk(kBX, kBY), grd(gs);
   hipStream_t st = b->stream;
   if (b->variant == 3) {   // fused step: the whole step in the "E" launch
-    if (e_half) hipLaunchKernelGGL((k_tmz_step<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+    if (!e_half) return;
+    if (b->scheme == SCHEME_TMZ) {
+      hipLaunchKernelGGL((k_tmz_step<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+    } else if constexpr (XC <= kFused3dMaxXc) {
+      hipLaunchKernelGGL((k_3d_step<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+    }
     return;
   }
   const bool v1 = b->variant == 1;
@@ -731,19 +736,29 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
   if (variant < 1 || variant > 3) return fail(OIDT_EINVAL, "variant must be 1, 2 or 3");
-  if (variant == 3 && b->scheme != SCHEME_TMZ) return fail(OIDT_EINVAL, "the fused step form exists for TMz only");
+  if (variant == 3 && b->scheme == SCHEME_TEZ) return fail(OIDT_EINVAL, "the fused step form exists for TMz and 3D only");
   for (const auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.fused != (variant == 3))
       return fail(OIDT_ESTATE, "cannot switch between fused and two-kernel forms while a run is in progress");
   if (xchunk != 0 && xchunk != 1 && xchunk != 2 && xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32)
     return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
+  if (variant == 3 && b->scheme == SCHEME_3D && (xchunk ? xchunk : b->xchunk) > kFused3dMaxXc)
+    return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4 or 8");
   b->variant = variant;
   for (auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued == sh.admit_chunk) sh.fused = variant == 3;
   if (xchunk) b->xchunk = xchunk;
   b->grid_override = grid_blocks;
   b->dirty = true;
+  return OIDT_OK;
+}
+
+int oidt_batch_get_launch(const oidt_batch* b, int32_t* xchunk, int32_t* grid_blocks, int32_t* variant) {
+  if (!b) return fail(OIDT_EINVAL, "null batch");
+  if (xchunk) *xchunk = b->xchunk;
+  if (grid_blocks) *grid_blocks = b->grid_override;
+  if (variant) *variant = b->variant;
   return OIDT_OK;
 }
 
@@ -944,6 +959,11 @@ int oidt_solver_create(const oidt_run_desc* desc, const oidt_run_inputs* in, oid
   }
   *out = s;
   return OIDT_OK;
+}
+
+int oidt_solver_set_launch(oidt_solver* s, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_set_launch(s->b, xchunk, grid_blocks, variant);
 }
 
 int oidt_solver_advance(oidt_solver* s, int32_t nsteps) {

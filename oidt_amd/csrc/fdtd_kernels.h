@@ -1340,6 +1340,347 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
   }
 }
 
+// ------------------------------------------------------- fused 3D step ---
+// One launch per time step for 3D, as k_tmz_step: it reads field/psi set
+// (n & 1) and writes the other.  A block owns 64 z-cells (lanes) x 4 y-rows
+// (waves) x XC x-planes.  The H half-step at (i,j,k) needs new E at i-1
+// (Ey, Ez), j-1 (Ex, Ez) and k-1 (Ex, Ey).  Each wave keeps its new E per
+// plane in registers (the i-1 neighbour inside the tile) and computes the
+// plane below the tile for its own row; the row below the tile (j0-1) is
+// spread over the waves by plane; the column below the tile (k0-1, 4 x XC
+// cells) is one partial pass of one wave.  Ex, Ey, Ez of the tile and of
+// those halos go through LDS for the j-1 and k-1 reads.  Halo cells use the
+// same load and update code as their owners, so they are bit-identical.
+
+// Old-set values one cell's new E reads (only the MASK components' inputs).
+template <typename T>
+struct E3In {
+  T ex, ey, ez, hx, hy, hz, hzj, hyk, hxk, hzi, hyi, hxj;
+  T qxy, qxz, qyx, qyz, qzx, qzy;
+  uint8_t m;
+};
+
+// New E of one cell; hx, hy, hz are the cell's old H (for the H half-step).
+template <typename T>
+struct E3Out {
+  T ex, ey, ez, hx, hy, hz;
+  T pxy, pxz, pyx, pyz, pzx, pzy;
+};
+
+// Flags of one cell for the E half-step (3D index ranges of
+// fdtd_funcs.py:44-80, FDTD_3D; CPML slabs :82-126).
+struct E3Where {
+  bool bx, by, bz, src;
+  int si, sj, sk, di, dj, dk, ci, cj, ck;
+};
+
+template <typename T>
+__device__ __forceinline__ E3Where e3_where(const Slot<T>& s, int i, int j, int k) {
+  E3Where f;
+  const bool in = i >= 0 && j >= 0 && k >= 0 && i < s.nx && j < s.ny && k < s.nz;
+  const bool iin = in && i >= s.ib && i < s.ie, iin1 = in && i > s.ib && i < s.ie;
+  const bool jin = in && j >= s.jb && j < s.je, jin1 = in && j > s.jb && j < s.je;
+  const bool kin = in && k >= s.kb && k < s.ke, kin1 = in && k > s.kb && k < s.ke;
+  f.bx = iin1 && jin && kin;
+  f.by = iin && jin1 && kin;
+  f.bz = iin && jin && kin1;
+  f.src = in && s.src_field == F_EY && i >= s.sx0 && i < s.sx1 && j >= s.sy0 && j < s.sy1 && k >= s.sz0 &&
+          k < s.sz1;
+  f.di = f.dj = f.dk = f.ci = f.cj = f.ck = 0;
+  f.si = in ? slab_e(i, s.ib, s.ie, s.N, f.di, f.ci) : -1;
+  f.sj = in ? slab_e(j, s.jb, s.je, s.N, f.dj, f.cj) : -1;
+  f.sk = in ? slab_e(k, s.kb, s.ke, s.N, f.dk, f.ck) : -1;
+  return f;
+}
+
+// Loads (indexes clamped into the grid; psi only inside its slab).
+template <typename T, int MASK>
+__device__ __forceinline__ E3In<T> e3_load(const Slot<T>& s, int i, int j, int k, const E3Where& f,
+                                           const T* const* A, T* const* psiA) {
+  E3In<T> a{};
+  const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+  const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
+  const unsigned pl = (unsigned)ny * nz;
+  const unsigned idx = (unsigned)ic * pl + (unsigned)jc * nz + kc;
+  const unsigned oi = ic + 1 < nx ? pl : 0u, oj = jc + 1 < ny ? (unsigned)nz : 0u, ok = kc + 1 < nz ? 1u : 0u;
+  const size_t iy = ((size_t)ic * N + f.dj) * nz + kc, iz = ((size_t)ic * ny + jc) * N + f.dk;
+  const size_t ix = ((size_t)f.di * ny + jc) * nz + kc;
+  a.m = s.mat[idx];
+  if (MASK & 1) {
+    a.ex = A[F_EX][idx];
+    a.hzj = A[F_HZ][idx + oj];
+    a.hyk = A[F_HY][idx + ok];
+    if (f.bx && f.sj >= 0) a.qxy = psiA[psi_id(F_EX, 1, f.sj)][iy];
+    if (f.bx && f.sk >= 0) a.qxz = psiA[psi_id(F_EX, 2, f.sk)][iz];
+  }
+  if (MASK & 2) {
+    a.ey = A[F_EY][idx];
+    a.hxk = A[F_HX][idx + ok];
+    a.hzi = A[F_HZ][idx + oi];
+    if (f.by && f.si >= 0) a.qyx = psiA[psi_id(F_EY, 0, f.si)][ix];
+    if (f.by && f.sk >= 0) a.qyz = psiA[psi_id(F_EY, 2, f.sk)][iz];
+  }
+  if (MASK & 4) {
+    a.ez = A[F_EZ][idx];
+    a.hyi = A[F_HY][idx + oi];
+    a.hxj = A[F_HX][idx + oj];
+    if (f.bz && f.si >= 0) a.qzx = psiA[psi_id(F_EZ, 0, f.si)][ix];
+    if (f.bz && f.sj >= 0) a.qzy = psiA[psi_id(F_EZ, 1, f.sj)][iy];
+  }
+  if (MASK & 6) a.hx = A[F_HX][idx];
+  if (MASK & 5) a.hy = A[F_HY][idx];
+  if (MASK & 3) a.hz = A[F_HZ][idx];
+  return a;
+}
+
+// The E update of one cell in the reference order: bulk curl, source, psi of
+// the first then the second transverse axis (fdtd_funcs.py:44-126, FDTD_3D).
+template <typename T, int MASK>
+__device__ __forceinline__ E3Out<T> e3_update(const Slot<T>& s, int i, int j, int k, int n, const E3Where& f,
+                                              const E3In<T>& a) {
+  E3Out<T> r;
+  r.ex = a.ex;
+  r.ey = a.ey;
+  r.ez = a.ez;
+  r.hx = a.hx;
+  r.hy = a.hy;
+  r.hz = a.hz;
+  r.pxy = r.pxz = r.pyx = r.pyz = r.pzx = r.pzy = T(0);
+  if ((MASK & 1) && f.bx) {
+    const T cb = cbsel(s, a.m, 0);
+    const T dHz = a.hzj - a.hz;
+    const T dHy = a.hyk - a.hy;
+    T v = a.ex + cb * (dHz * s.ike[1][j] - dHy * s.ike[2][k]);
+    if (f.sj >= 0) { r.pxy = s.be[f.cj] * a.qxy + s.ceo[f.cj] * dHz; v += cb * r.pxy; }
+    if (f.sk >= 0) { r.pxz = s.be[f.ck] * a.qxz + s.ceo[f.ck] * dHy; v -= cb * r.pxz; }
+    r.ex = v;
+  }
+  if (MASK & 2) {
+    if (f.by) {
+      const T cb = cbsel(s, a.m, 1);
+      const T dHx = a.hxk - a.hx;
+      const T dHz = a.hzi - a.hz;
+      T v = a.ey + cb * (dHx * s.ike[2][k] - dHz * s.ike[0][i]);
+      if (f.src) v -= s.jsrc[n];
+      if (f.si >= 0) { r.pyx = s.be[f.ci] * a.qyx + s.ceo[f.ci] * dHz; v -= cb * r.pyx; }
+      if (f.sk >= 0) { r.pyz = s.be[f.ck] * a.qyz + s.ceo[f.ck] * dHx; v += cb * r.pyz; }
+      r.ey = v;
+    } else if (f.src) {
+      r.ey = a.ey - s.jsrc[n];
+    }
+  }
+  if ((MASK & 4) && f.bz) {
+    const T cb = cbsel(s, a.m, 2);
+    const T dHy = a.hyi - a.hy;
+    const T dHx = a.hxj - a.hx;
+    T v = a.ez + cb * (dHy * s.ike[0][i] - dHx * s.ike[1][j]);
+    if (f.si >= 0) { r.pzx = s.be[f.ci] * a.qzx + s.ceo[f.ci] * dHy; v += cb * r.pzx; }
+    if (f.sj >= 0) { r.pzy = s.be[f.cj] * a.qzy + s.ceo[f.cj] * dHx; v -= cb * r.pzy; }
+    r.ez = v;
+  }
+  return r;
+}
+
+template <typename T, int XC>
+__global__ __launch_bounds__(kThreads) void k_3d_step(const Slot<T>* __restrict__ slots,
+                                                      const int4* __restrict__ tiles,
+                                                      const int* __restrict__ counts, int tstep) {
+  // sE?[u][q][p]: new E of plane i0 + u, row j0 - 1 + q, column k0 - 1 + p
+  __shared__ T sEx[XC][kBY + 1][kBX + 1];
+  __shared__ T sEy[XC][kBY + 1][kBX + 1];
+  __shared__ T sEz[XC][kBY + 1][kBX + 1];
+  constexpr int kColWave = XC >= kBY ? 0 : XC;   // the wave with the fewest row-halo planes
+  const int ntiles = counts[0];
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int i0 = tl.y, j0 = tl.z, k0 = tl.w;
+    const int j = j0 + w, k = k0 + lane;
+    const bool odd = (n & 1) != 0;
+    const T db = s.db;
+    const T* const* A = odd ? s.f2 : s.f;
+    T* const* B = odd ? s.f : s.f2;
+    T* const* psiA = odd ? s.psi2 : s.psi;
+    T* const* psiB = odd ? s.psi : s.psi2;
+    // ---- loads of the owned cells, all issued before any update
+    E3Where fw[XC];
+    E3In<T> in[XC];
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      fw[u] = e3_where(s, i0 + u, j, k);
+      in[u] = e3_load<T, 7>(s, i0 + u, j, k, fw[u], A, psiA);
+    }
+    // ---- halos: plane i0-1 of this row (Ey, Ez), row j0-1 (Ex, Ez), column k0-1 (Ex, Ey)
+    E3Out<T> ph;
+    {
+      const E3Where f = e3_where(s, i0 - 1, j, k);
+      ph = e3_update<T, 6>(s, i0 - 1, j, k, n, f, e3_load<T, 6>(s, i0 - 1, j, k, f, A, psiA));
+    }
+    for (int u = w; u < XC; u += kBY) {
+      const E3Where f = e3_where(s, i0 + u, j0 - 1, k);
+      const E3Out<T> r = e3_update<T, 5>(s, i0 + u, j0 - 1, k, n, f, e3_load<T, 5>(s, i0 + u, j0 - 1, k, f, A, psiA));
+      sEx[u][0][1 + lane] = r.ex;
+      sEz[u][0][1 + lane] = r.ez;
+    }
+    if (w == kColWave && lane < kBY * XC) {
+      const int u = lane / kBY, q = lane % kBY;
+      const E3Where f = e3_where(s, i0 + u, j0 + q, k0 - 1);
+      const E3Out<T> r =
+          e3_update<T, 3>(s, i0 + u, j0 + q, k0 - 1, n, f, e3_load<T, 3>(s, i0 + u, j0 + q, k0 - 1, f, A, psiA));
+      sEx[u][1 + q][0] = r.ex;
+      sEy[u][1 + q][0] = r.ey;
+    }
+    // ---- E half-step of the owned cells
+    const bool jcap = j >= s.wy0 && j <= s.wy1;
+    const int cap = !jcap ? -1 : (k == s.port[0] ? 0 : (k == s.port[1] ? 1 : -1));
+    const bool cap2 = jcap && s.port[0] == s.port[1] && k == s.port[0];
+    const unsigned pl = (unsigned)ny * nz;
+    E3Out<T> e[XC];
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      const int i = i0 + u;
+      const E3Where& f = fw[u];
+      e[u] = e3_update<T, 7>(s, i, j, k, n, f, in[u]);
+      sEx[u][1 + w][1 + lane] = e[u].ex;
+      sEy[u][1 + w][1 + lane] = e[u].ey;
+      sEz[u][1 + w][1 + lane] = e[u].ez;
+      const unsigned idx = (unsigned)i * pl + (unsigned)j * nz + k;
+      if (f.bx) {
+        B[F_EX][idx] = e[u].ex;
+        if (f.sj >= 0) psiB[psi_id(F_EX, 1, f.sj)][((size_t)i * N + f.dj) * nz + k] = e[u].pxy;
+        if (f.sk >= 0) psiB[psi_id(F_EX, 2, f.sk)][((size_t)i * ny + j) * N + f.dk] = e[u].pxz;
+      }
+      if (f.by) {
+        B[F_EY][idx] = e[u].ey;
+        if (f.si >= 0) psiB[psi_id(F_EY, 0, f.si)][((size_t)f.di * ny + j) * nz + k] = e[u].pyx;
+        if (f.sk >= 0) psiB[psi_id(F_EY, 2, f.sk)][((size_t)i * ny + j) * N + f.dk] = e[u].pyz;
+      } else if (f.src) {
+        B[F_EY][idx] = e[u].ey;
+      }
+      if (f.bz) {
+        B[F_EZ][idx] = e[u].ez;
+        if (f.si >= 0) psiB[psi_id(F_EZ, 0, f.si)][((size_t)f.di * ny + j) * nz + k] = e[u].pzx;
+        if (f.sj >= 0) psiB[psi_id(F_EZ, 1, f.sj)][((size_t)i * N + f.dj) * nz + k] = e[u].pzy;
+      }
+      if (cap >= 0 && i < nx && i >= s.wx0 && i <= s.wx1) {
+        const int c = (i - s.wx0) * s.prow + (j - s.wy0);
+        capture(s, tstep, cap * 6 + 0, c, e[u].ex);
+        capture(s, tstep, cap * 6 + 1, c, e[u].ey);
+        if (cap2) {
+          capture(s, tstep, 6, c, e[u].ex);
+          capture(s, tstep, 7, c, e[u].ey);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- H half-step of the owned cells (index ranges fdtd_funcs.py:84-126, FDTD_3D)
+    const bool jok = j < ny, kok = k < nz;
+    const bool jin = jok && j >= s.jb && j < s.je, jin1 = jok && j > s.jb && j < s.je;
+    const bool kin = kok && k >= s.kb && k < s.ke, kin1 = kok && k > s.kb && k < s.ke;
+    int dj = 0, cj = 0, dk = 0, ck = 0;
+    const int sj = jin1 ? slab_h(j, s.jb, s.je, N, dj, cj) : -1;
+    const int sk = kin1 ? slab_h(k, s.kb, s.ke, N, dk, ck) : -1;
+    const T iky = jin1 ? s.ikh[1][j] : T(0);
+    const T ikz = kin1 ? s.ikh[2][k] : T(0);
+    const T bhj = sj >= 0 ? s.bh[cj] : T(0), chj = sj >= 0 ? s.cho[cj] : T(0);
+    const T bhk = sk >= 0 ? s.bh[ck] : T(0), chk = sk >= 0 ? s.cho[ck] : T(0);
+    const bool jksrc = s.src_field == F_HY && jok && kok && j >= s.sy0 && j < s.sy1 && k >= s.sz0 && k < s.sz1;
+    const T jv = jksrc ? s.jsrc[n] : T(0);
+    int capA = -1, capB = -1, capC = -1, capD = -1;
+    if (jcap && kok) {
+      if (k == s.port[0]) capA = 0;
+      if (k == s.port[0] + 1) capB = 0;
+      if (k == s.port[1]) capC = 1;
+      if (k == s.port[1] + 1) capD = 1;
+    }
+    const bool anycap = capA >= 0 || capB >= 0 || capC >= 0 || capD >= 0;
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      const int i = i0 + u;
+      if (!jok || !kok || i >= nx) continue;
+      const bool iin = i >= s.ib && i < s.ie, iin1 = i > s.ib && i < s.ie;
+      const bool bx = iin && jin1 && kin1;
+      const bool by = iin1 && jin && kin1;
+      const bool bz = iin1 && jin1 && kin;
+      const bool src = jksrc && i >= s.sx0 && i < s.sx1;
+      const bool icap = anycap && i >= s.wx0 && i <= s.wx1;
+      if (!(bx || by || bz || src || icap)) continue;
+      int di = 0, ci = 0;
+      const int si = slab_h(i, s.ib, s.ie, N, di, ci);
+      const size_t iy = ((size_t)i * N + dj) * nz + k, iz = ((size_t)i * ny + j) * N + dk;
+      const size_t ix = ((size_t)di * ny + j) * nz + k;
+      const unsigned idx = (unsigned)i * pl + (unsigned)j * nz + k;
+      const T ex = e[u].ex, ey = e[u].ey, ez = e[u].ez;
+      const T exk = sEx[u][1 + w][lane], eyk = sEy[u][1 + w][lane];     // k-1
+      const T exj = sEx[u][w][1 + lane], ezj = sEz[u][w][1 + lane];     // j-1
+      const T eyi = u ? e[u - 1].ey : ph.ey, ezi = u ? e[u - 1].ez : ph.ez;   // i-1
+      T vx = e[u].hx;
+      if (bx) {
+        const T dEy = ey - eyk;
+        const T dEz = ez - ezj;
+        vx = vx + db * (dEy * ikz - dEz * iky);
+        if (sj >= 0) {
+          const T p = bhj * psiA[psi_id(F_HX, 1, sj)][iy] + chj * dEz;
+          psiB[psi_id(F_HX, 1, sj)][iy] = p;
+          vx -= db * p;
+        }
+        if (sk >= 0) {
+          const T p = bhk * psiA[psi_id(F_HX, 2, sk)][iz] + chk * dEy;
+          psiB[psi_id(F_HX, 2, sk)][iz] = p;
+          vx += db * p;
+        }
+        B[F_HX][idx] = vx;
+      }
+      T vy = e[u].hy;
+      if (src) vy -= jv;
+      if (by) {
+        const T dEz = ez - ezi;
+        const T dEx = ex - exk;
+        vy = vy + db * (dEz * s.ikh[0][i] - dEx * ikz);
+        if (si >= 0) {
+          const T p = s.bh[ci] * psiA[psi_id(F_HY, 0, si)][ix] + s.cho[ci] * dEz;
+          psiB[psi_id(F_HY, 0, si)][ix] = p;
+          vy += db * p;
+        }
+        if (sk >= 0) {
+          const T p = bhk * psiA[psi_id(F_HY, 2, sk)][iz] + chk * dEx;
+          psiB[psi_id(F_HY, 2, sk)][iz] = p;
+          vy -= db * p;
+        }
+      }
+      if (by || src) B[F_HY][idx] = vy;
+      if (bz) {
+        const T dEx = ex - exj;
+        const T dEy = ey - eyi;
+        T v = e[u].hz + db * (dEx * iky - dEy * s.ikh[0][i]);
+        if (si >= 0) {
+          const T p = s.bh[ci] * psiA[psi_id(F_HZ, 0, si)][ix] + s.cho[ci] * dEy;
+          psiB[psi_id(F_HZ, 0, si)][ix] = p;
+          v -= db * p;
+        }
+        if (sj >= 0) {
+          const T p = bhj * psiA[psi_id(F_HZ, 1, sj)][iy] + chj * dEx;
+          psiB[psi_id(F_HZ, 1, sj)][iy] = p;
+          v += db * p;
+        }
+        B[F_HZ][idx] = v;
+      }
+      if (icap) {
+        const int c = (i - s.wx0) * s.prow + (j - s.wy0);
+        if (capA >= 0) { capture(s, tstep, 2, c, vx); capture(s, tstep, 4, c, vy); }
+        if (capB >= 0) { capture(s, tstep, 3, c, vx); capture(s, tstep, 5, c, vy); }
+        if (capC >= 0) { capture(s, tstep, 8, c, vx); capture(s, tstep, 10, c, vy); }
+        if (capD >= 0) { capture(s, tstep, 9, c, vx); capture(s, tstep, 11, c, vy); }
+      }
+    }
+    __syncthreads();   // the LDS planes are reused by the next tile
+  }
+}
+
 // ------------------------------------------------------------------- DFT ---
 // Work item (slot, q, cell block, frequency block).  Folds the kChunk samples
 // of the chunk in flight into acc[q][f][cell] (re += s cos, im -= s sin, as

@@ -61,9 +61,12 @@ class Request:
 class DeviceBackend:
     """The GPU: one oidt batch per (scheme, precision)."""
 
-    def __init__(self, max_slots=64, xchunk=0, variant=2):
+    def __init__(self, max_slots=64, xchunk=0, variant=2, launch=None):
+        """``launch`` maps a scheme ('tm', 'te', '3d') to its (variant, xchunk),
+        e.g. the autotuner's picks; other schemes use ``variant``/``xchunk``."""
         self.max_slots = max_slots
         self.xchunk, self.variant = xchunk, variant
+        self.launch = dict(launch or {})
         self.batches = {}
         self.resident = {}
 
@@ -73,7 +76,10 @@ class DeviceBackend:
     def batch(self, key, setup):
         if key not in self.batches:
             b = S.Batch(setup.scheme, setup.precision, self.max_slots)
-            b.set_launch(self.xchunk, 0, self.variant)
+            variant, xchunk = self.launch.get(setup.scheme, (self.variant, self.xchunk))
+            if variant == 3 and setup.scheme == 'te':
+                variant = 2
+            b.set_launch(xchunk, 0, variant)
             self.batches[key] = b
             self.resident[key] = 0
         return self.batches[key]
@@ -211,9 +217,10 @@ def summarize(done, wall):
             'mcell_updates_per_s': cu / wall / 1e6 if wall > 0 else 0.0, 'cell_updates': cu}
 
 
-def serve_plain(requests):
+def serve_plain(requests, launch=None):
     """Plain path: one solve at a time, each alone on the device (queueing in
-    arrival order).  Returns the requests with latencies."""
+    arrival order), with the kernel form of ``launch`` ({scheme: (variant,
+    xchunk)}) where given.  Returns the requests with latencies."""
     clock = time.perf_counter
     start = clock()
     for r in sorted(requests):
@@ -223,6 +230,9 @@ def serve_plain(requests):
         if r.setup is None:
             prepare_request(r, DeviceBackend())
         sv = S.Solver(r.setup, r.fg)
+        if launch and r.setup.scheme in launch:
+            variant, xchunk = launch[r.setup.scheme]
+            sv.set_launch(xchunk, 0, variant)
         sv.run()
         r.result = sv.dft()
         sv.close()

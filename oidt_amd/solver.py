@@ -141,6 +141,11 @@ class Solver:
         check(lib().oidt_solver_create(C.byref(self._desc), C.byref(self._in), C.byref(h)))
         self.h = h
 
+    def set_launch(self, xchunk=0, grid_blocks=0, variant=2):
+        """Kernel form and x-chunk depth (before the first advance)."""
+        check(lib().oidt_solver_set_launch(self.h, int(xchunk), int(grid_blocks), int(variant)))
+        return self
+
     def advance(self, nsteps):
         check(lib().oidt_solver_advance(self.h, int(nsteps)))
         return self
@@ -213,6 +218,12 @@ class Batch:
 
     def set_launch(self, xchunk=8, grid_blocks=0, variant=2):
         check(lib().oidt_batch_set_launch(self.h, int(xchunk), int(grid_blocks), int(variant)))
+
+    def launch(self):
+        """Current (variant, xchunk, grid_blocks) of the batch."""
+        xc, g, v = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().oidt_batch_get_launch(self.h, C.byref(xc), C.byref(g), C.byref(v)))
+        return v.value, xc.value, g.value
 
     def advance(self, nchunks):
         check(lib().oidt_batch_advance(self.h, int(nchunks)))
