@@ -181,6 +181,17 @@ int oidt_batch_progress(oidt_batch* b, int64_t* chunks_enqueued, int64_t* chunks
 int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp, double* re, double* im);
 int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host, uint64_t nbytes);
 
+/* Port power on the device, per frequency: the sum over rows [row0, row0 +
+ * nrows) of the DFT window of the time-averaged normal Poynting density of
+ * port `port` of run `slot` (2D: one row = one cell along y; 3D: one row =
+ * one x index with every y of the window), after subtracting the phasors of
+ * port `sub_port` of run `sub_slot` (sub_slot = -1: nothing subtracted).
+ * TMz -0.5 Ez Hy*, TEz 0.5 Ey Hz*, 3D 0.5 (Ex Hy* - Ey Hx*), in FP64.
+ * Replaces the host sums of extract_sparams.py:88-104 (TM, TE) and
+ * FDTD_3D/.../extract_sparams.py:114-131.  re/im receive nf values. */
+int oidt_batch_port_power(oidt_batch* b, int32_t slot, int32_t port, int32_t sub_slot, int32_t sub_port,
+                          int32_t row0, int32_t nrows, double* re, double* im);
+
 /* Release a slot (its device memory is returned in stream order). */
 int oidt_batch_remove(oidt_batch* b, int32_t slot);
 

@@ -251,6 +251,9 @@ struct oidt_batch {
   int64_t chunks_done = 0;
   std::deque<std::pair<hipEvent_t, int64_t>> marks;
   std::vector<hipEvent_t> free_events;
+  hipStream_t aux = nullptr;   // reductions over finished runs, off the stepping stream
+  double* d_pow = nullptr;     // [nf][2] port power output
+  size_t pow_cap = 0;
   size_t slot_bytes() const { return precision == OIDT_F64 ? sizeof(Slot<double>) : sizeof(Slot<float>); }
 };
 
@@ -836,6 +839,58 @@ int oidt_batch_read_dft(oidt_batch* b, int32_t slot, int32_t port, int32_t comp,
   return OIDT_OK;
 }
 
+int oidt_batch_port_power(oidt_batch* b, int32_t slot, int32_t port, int32_t sub_slot, int32_t sub_port,
+                          int32_t row0, int32_t nrows, double* re, double* im) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  const SlotHost& sh = b->slots[slot];
+  const oidt_run_desc& d = sh.desc;
+  if (port < 0 || port > 1) return fail(OIDT_EINVAL, "bad port");
+  if (!re || !im) return fail(OIDT_EINVAL, "null output");
+  const SlotHost* ss = nullptr;
+  if (sub_slot >= 0) {
+    if ((rc = slot_ok(b, sub_slot)) != OIDT_OK) return rc;
+    ss = &b->slots[sub_slot];
+    if (sub_port < 0 || sub_port > 1) return fail(OIDT_EINVAL, "bad port of the subtracted run");
+    if (ss->desc.nf != d.nf || ss->wcells != sh.wcells || ss->wyn != sh.wyn)
+      return fail(OIDT_EINVAL, "the subtracted run has other port planes or frequencies");
+  }
+  const int rowlen = d.scheme == SCHEME_3D ? sh.wyn : 1;
+  const int rows = rowlen > 0 ? sh.wcells / rowlen : 0;
+  if (row0 < 0 || nrows < 0 || row0 + nrows > rows) return fail(OIDT_EINVAL, "port window outside the DFT window");
+  if (d.nf == 0) return OIDT_OK;
+  // Accumulators of finished runs are final; otherwise wait for the batch.
+  if (!slot_finished(b, sh) || (ss && !slot_finished(b, *ss))) HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!b->aux) HIP_TRY(hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking));
+  if (b->pow_cap < (size_t)d.nf) {
+    if (b->d_pow) HIP_TRY(hipFree(b->d_pow));
+    b->d_pow = nullptr;
+    b->pow_cap = 0;
+    HIP_TRY(hipMalloc(&b->d_pow, (size_t)d.nf * 2 * sizeof(double)));
+    b->pow_cap = d.nf;
+  }
+  const size_t off = d.precision == OIDT_F64 ? layout<double>(d).off_acc : layout<float>(d).off_acc;
+  const double* a = reinterpret_cast<const double*>(static_cast<char*>(sh.dmem) + off);
+  const double* sa = nullptr;
+  if (ss) {
+    const size_t soff = ss->desc.precision == OIDT_F64 ? layout<double>(ss->desc).off_acc
+                                                      : layout<float>(ss->desc).off_acc;
+    sa = reinterpret_cast<const double*>(static_cast<char*>(ss->dmem) + soff);
+  }
+  const int ncomp = sh.nq / 2;
+  hipLaunchKernelGGL(k_port_power, dim3(d.nf), dim3(kThreads), 0, b->aux, a, sa, d.scheme, port * ncomp,
+                     ss ? sub_port * ncomp : 0, d.nf, sh.wcells, row0 * rowlen, (row0 + nrows) * rowlen, b->d_pow);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> buf(2 * (size_t)d.nf);
+  HIP_TRY(hipMemcpyAsync(buf.data(), b->d_pow, buf.size() * sizeof(double), hipMemcpyDeviceToHost, b->aux));
+  HIP_TRY(hipStreamSynchronize(b->aux));
+  for (int f = 0; f < d.nf; ++f) {
+    re[f] = buf[2 * f];
+    im[f] = buf[2 * f + 1];
+  }
+  return OIDT_OK;
+}
+
 int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host, uint64_t nbytes) {
   int rc = slot_ok(b, slot);
   if (rc != OIDT_OK) return rc;
@@ -934,6 +989,9 @@ int oidt_batch_destroy(oidt_batch* b) {
   for (auto& m : b->marks) hipEventDestroy(m.first);
   for (auto e : b->free_events) hipEventDestroy(e);
   b->staging.release();
+  if (b->aux) hipStreamSynchronize(b->aux);
+  if (b->d_pow) hipFree(b->d_pow);
+  if (b->aux) hipStreamDestroy(b->aux);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return OIDT_OK;

@@ -1768,6 +1768,86 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
   }
 }
 
+// ------------------------------------------------------------ port power ---
+// Time-averaged normal Poynting power of one port plane per frequency, summed
+// over window cells [c0, c1) (the port-window sums of extract_sparams.py:88-104,
+// FDTD_3D :114-131).  a / sub: the FP64 accumulators of the run and of the
+// run whose phasors are subtracted first (sub may be null; the incident-field
+// correction of S_ii).  q0 / qs: first accumulator row (port * components) of
+// each.  One block per frequency; wave reductions with cross-lane shuffles,
+// then LDS across the four waves.  out[f] = (re, im).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_port_power(const double* __restrict__ a,
+                                                         const double* __restrict__ sub, int scheme, int q0,
+                                                         int qs, int nf, int wcells, int c0, int c1,
+                                                         double* __restrict__ out) {
+  __shared__ double part[2][kThreads / 64];
+  const int f = blockIdx.x;
+  const int tid = threadIdx.x;
+  auto ph = [&](const double* base, int q, int c, double& re, double& im) {
+    const double* p = base + (((size_t)q * nf + f) * wcells + c) * 2;
+    re = p[0];
+    im = p[1];
+  };
+  double sr = 0.0, si = 0.0;
+  for (int c = c0 + tid; c < c1; c += kThreads) {
+    if (scheme == SCHEME_3D) {
+      // components ex, ey, hx, hy: S = 0.5 (Ex Hy* - Ey Hx*)
+      double v[4][2];
+      for (int k = 0; k < 4; ++k) {
+        ph(a, q0 + k, c, v[k][0], v[k][1]);
+        if (sub) {
+          double r, i;
+          ph(sub, qs + k, c, r, i);
+          v[k][0] -= r;
+          v[k][1] -= i;
+        }
+      }
+      sr += 0.5 * ((v[0][0] * v[3][0] + v[0][1] * v[3][1]) - (v[1][0] * v[2][0] + v[1][1] * v[2][1]));
+      si += 0.5 * ((v[0][1] * v[3][0] - v[0][0] * v[3][1]) - (v[1][1] * v[2][0] - v[1][0] * v[2][1]));
+    } else {
+      double er, ei, hr, hi;
+      ph(a, q0, c, er, ei);
+      ph(a, q0 + 1, c, hr, hi);
+      if (sub) {
+        double r, i;
+        ph(sub, qs, c, r, i);
+        er -= r;
+        ei -= i;
+        ph(sub, qs + 1, c, r, i);
+        hr -= r;
+        hi -= i;
+      }
+      // TMz: -0.5 Ez Hy*; TEz: 0.5 Ey Hz*
+      const double sgn = scheme == SCHEME_TMZ ? -0.5 : 0.5;
+      sr += sgn * (er * hr + ei * hi);
+      si += sgn * (ei * hr - er * hi);
+    }
+  }
+  sr = wave_sum(sr);
+  si = wave_sum(si);
+  const int w = tid >> 6;
+  if ((tid & 63) == 0) {
+    part[0][w] = sr;
+    part[1][w] = si;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double r = 0.0, i = 0.0;
+    for (int k = 0; k < kThreads / 64; ++k) {
+      r += part[0][k];
+      i += part[1][k];
+    }
+    out[2 * f] = r;
+    out[2 * f + 1] = i;
+  }
+}
+
 // End-of-chunk: every resident run moves to its next chunk.
 template <typename T>
 __global__ void k_advance(Slot<T>* __restrict__ slots, int nslots) {

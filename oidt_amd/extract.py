@@ -48,6 +48,24 @@ def port_power(scheme, s, sav, line):
     return sav[:, start:start + s.pp].sum(axis=1)
 
 
+def s_pairs(scheme, num_lines):
+    """(i, j, iline, jline, inc_tag, ref_tag, cor_tag or None) of every S entry
+    the reference computes, in its order (extract_sparams.py:52-104; 3D
+    FDTD_3D extract_sparams.py:52-131)."""
+    P = 2 * num_lines
+    out = []
+    for i in range(P):
+        for j in (range(P) if scheme == '3d' else range(i, P)):
+            inc_tag = 'fi' if j % 2 == 0 else 'bi'
+            if i % 2 == j % 2:
+                ref_tag = 'ft' if i % 2 == 0 else 'bt'
+            else:
+                ref_tag = 'fr' if i % 2 == 0 else 'br'
+            cor_tag = ('fi' if i % 2 == 0 else 'bi') if i == j else None
+            out.append((i, j, i // 2 + 1, j // 2 + 1, inc_tag, ref_tag, cor_tag))
+    return out
+
+
 def s_matrix(scheme, s, load, num_lines, nf):
     """S[nf, 2L, 2L] (complex128) from ``load(comp, tag, line)`` -> phasor array.
 
@@ -57,22 +75,42 @@ def s_matrix(scheme, s, load, num_lines, nf):
     P = 2 * num_lines
     S = np.zeros((nf, P, P), dtype=np.complex128)
     comps = COMPS[scheme]
-    pairs = [(i, j) for i in range(P) for j in (range(P) if scheme == '3d' else range(i, P))]
-    for i, j in pairs:
-        iline, jline = i // 2 + 1, j // 2 + 1
-        inc_tag = 'fi' if j % 2 == 0 else 'bi'
-        if i % 2 == j % 2:
-            ref_tag = 'ft' if i % 2 == 0 else 'bt'
-        else:
-            ref_tag = 'fr' if i % 2 == 0 else 'br'
+    for i, j, iline, jline, inc_tag, ref_tag, cor_tag in s_pairs(scheme, num_lines):
         inc = {c: np.asarray(load(c, inc_tag, jline), dtype=np.complex128) for c in comps}
         ref = {c: np.array(load(c, ref_tag, jline), dtype=np.complex128) for c in comps}
-        if i == j:
-            cor_tag = 'fi' if i % 2 == 0 else 'bi'
+        if cor_tag is not None:
             for c in comps:
                 ref[c] -= np.asarray(load(c, cor_tag, iline), dtype=np.complex128)
         pinc = port_power(scheme, s, poynting(scheme, inc), jline)
         pref = port_power(scheme, s, poynting(scheme, ref), iline)
+        S[:, i, j] = S[:, j, i] = np.sqrt(pref / pinc)
+    return S
+
+
+# run (direction, kind) and port index (0: port 1, 1: port 2) of each phasor tag
+TAG_RUN = {'fi': (('f', 'i'), 0), 'bi': (('b', 'i'), 1), 'ft': (('f', 'r'), 0), 'br': (('f', 'r'), 1),
+           'fr': (('b', 'r'), 0), 'bt': (('b', 'r'), 1)}
+
+
+def s_matrix_device(scheme, s, batch, slots, num_lines, nf):
+    """S[nf, 2L, 2L] as ``s_matrix``, with the port-window power sums done on
+    the device from the FP64 accumulators (Batch.port_power); ``slots`` maps
+    (line index 0.., direction, kind) to the batch slot of that run.  Only the
+    per-frequency powers leave the device."""
+    P = 2 * num_lines
+    S = np.zeros((nf, P, P), dtype=np.complex128)
+    N = s.num_cpml
+
+    def where(tag, line):
+        (d, k), port = TAG_RUN[tag]
+        return slots[(line - 1, d, k)], port
+
+    for i, j, iline, jline, inc_tag, ref_tag, cor_tag in s_pairs(scheme, num_lines):
+        si, pi = where(inc_tag, jline)
+        sr, pr = where(ref_tag, jline)
+        sub = where(cor_tag, iline) if cor_tag is not None else None
+        pinc = batch.port_power(si, pi, window_start(scheme, s, jline) - N, s.pp)
+        pref = batch.port_power(sr, pr, window_start(scheme, s, iline) - N, s.pp, sub=sub)
         S[:, i, j] = S[:, j, i] = np.sqrt(pref / pinc)
     return S
 

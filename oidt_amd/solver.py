@@ -250,6 +250,18 @@ class Batch:
         check(lib().oidt_batch_read_dft(self.h, int(slot), port, comp, ptr(re), ptr(im)))
         return re + 1j * im
 
+    def port_power(self, slot, port, row0, nrows, sub=None):
+        """Complex port power [nf] summed on the device over DFT-window rows
+        [row0, row0 + nrows) of ``port`` (0 or 1) of run ``slot``, after
+        subtracting the phasors of ``sub`` = (slot, port) if given."""
+        nf = len(self.runs[slot].sparam_freqs)
+        re = np.zeros(nf)
+        im = np.zeros(nf)
+        ss, sp = sub if sub is not None else (-1, 0)
+        check(lib().oidt_batch_port_power(self.h, int(slot), int(port), int(ss), int(sp), int(row0), int(nrows),
+                                          ptr(re), ptr(im)))
+        return re + 1j * im
+
     def dft(self, slot):
         s = self.runs[slot]
         keys = COMP_KEYS[s.scheme]

@@ -33,4 +33,6 @@ def test_protocol_writes_touchstone(tmp_path, scheme):
     np.testing.assert_array_equal(S, S2)
     names = sorted(os.listdir(seq / 'Results'))
     assert 'NEXT_STEPS_INSTRUCTIONS.txt' in names
-    assert len([n for n in names if n.endswith('.npy')]) == (8 if scheme != '3d' else 16) + 0
+    # fi and bi runs save port 1 / port 2, the two rough runs both ports:
+    # 6 (port, tag) pairs x 2 components (2D) or 4 components (3D)
+    assert len([n for n in names if n.endswith('.npy')]) == (12 if scheme != '3d' else 24)
