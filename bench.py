@@ -214,6 +214,7 @@ def main():
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=0,
                     help='kernel form: 1 march, 2 batched loads, 3 fused step; 0: autotuner picks form and xchunk')
+    ap.add_argument('--grid', type=int, default=0, help='persistent grid size cap of the stencil launches (0: 8192)')
     ap.add_argument('--profile-chunks', type=int, default=4)
     ap.add_argument('--cpu-seconds', type=float, default=15.0, help='length of the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
@@ -256,7 +257,7 @@ def main():
     scheme, prec = runs[0][1].scheme, runs[0][1].precision
     variant, xchunk, tuned = choose_launch(args, [(s, fg) for _, s, fg in runs])
     b = Batch(scheme, prec, max_slots=len(runs))
-    b.set_launch(xchunk, 0, variant)
+    b.set_launch(xchunk, args.grid, variant)
     variant, xchunk, _ = b.launch()
     for p, s, fg in runs:
         b.add(s, fg)
@@ -273,7 +274,7 @@ def main():
     # Per-kernel durations with HIP events on the batch stream (a second batch of
     # the same runs, so the timed run above is untouched).
     pb = Batch(scheme, prec, max_slots=len(runs))
-    pb.set_launch(xchunk, 0, variant)
+    pb.set_launch(xchunk, args.grid, variant)
     for p, s, fg in runs:
         pb.add(s, fg)
     import ctypes as C
@@ -302,7 +303,8 @@ def main():
         'steps': K, 'warmup': warm, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32' if es == 4 else 'f64', 'data': 'synthetic',
         'config': {'workload': label, 'runs': len(runs), 'cells_per_step': cells,
-                   'steps_per_run': K, 'xchunk': xchunk, 'variant': variant, 'parallelism': 'single GPU',
+                   'steps_per_run': K, 'xchunk': xchunk, 'variant': variant, 'grid_cap': args.grid or 8192,
+                   'parallelism': 'single GPU',
                    'autotune': tuned},
         'hbm_roofline_frac_step': round(value * 1e6 * (be + bh) / (HBM_PEAK_GBS * 1e9), 4),
         'algorithmic_bytes_per_cell_update': be + bh,

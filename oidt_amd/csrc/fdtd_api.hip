@@ -75,8 +75,8 @@ struct Staging {
   }
   void release() {
     for (auto& b : bufs) {
-      if (b.ev) hipEventSynchronize(b.ev), hipEventDestroy(b.ev);
-      if (b.p) hipHostFree(b.p);
+      if (b.ev) { (void)hipEventSynchronize(b.ev); (void)hipEventDestroy(b.ev); }
+      if (b.p) (void)hipHostFree(b.p);
     }
     bufs.clear();
   }
@@ -428,6 +428,9 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
   return OIDT_OK;
 }
 
+// Kernels whose tiles are per wavefront (four tiles per block).
+bool march_tiles(const oidt_batch* b) { return b->variant == 3 && b->scheme == SCHEME_TMZ; }
+
 // Rebuild the tile and DFT work lists for the resident slots and upload them.
 int rebuild_lists(oidt_batch* b) {
   std::vector<int4> tiles, work;
@@ -441,7 +444,10 @@ int rebuild_lists(oidt_batch* b) {
         for (int y0 = 0; y0 < d.ny; y0 += kBY)
           for (int z0 = 0; z0 < d.nz; z0 += kBX) tiles.push_back(make_int4(si, x0, y0, z0));
     } else {
-      for (int x0 = 0; x0 < d.nx; x0 += xc * kBY)
+      // the fused TMz step gives each wavefront its own tile of xc planes;
+      // the other 2D forms give a block 4 waves x xc planes
+      const int xstep = march_tiles(b) ? xc : xc * kBY;
+      for (int x0 = 0; x0 < d.nx; x0 += xstep)
         for (int y0 = 0; y0 < d.ny; y0 += kBX) tiles.push_back(make_int4(si, x0, y0, 0));
     }
     const int cblocks = (sh.wcells + kThreads - 1) / kThreads;
@@ -537,7 +543,10 @@ void launch_advance(oidt_batch* b) {
                      static_cast<Slot<T>*>(b->d_slots), b->max_slots);
 }
 
-int stencil_grid(const oidt_batch* b) { return grid_for(b->ntiles, b->grid_override > 0 ? b->grid_override : 8192); }
+int stencil_grid(const oidt_batch* b) {
+  const int items = march_tiles(b) ? (b->ntiles + kBY - 1) / kBY : b->ntiles;
+  return grid_for(items, b->grid_override > 0 ? b->grid_override : 8192);
+}
 int dft_grid(const oidt_batch* b) { return grid_for(b->nwork, 8192); }
 
 template <typename T>
@@ -590,7 +599,7 @@ int profile_chunks(oidt_batch* b, int nchunks, double* out) {
       cnt[kind] += 1;
     }
   }
-  for (auto& e : ev) hipEventDestroy(e);
+  for (auto& e : ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 4; ++i) out[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
   b->chunks_enqueued += nchunks;
   b->chunks_done = b->chunks_enqueued;
@@ -722,7 +731,7 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
   sh.desc = *desc;
   rc = b->precision == OIDT_F64 ? build_slot<double>(b, sh, *desc, in) : build_slot<float>(b, sh, *desc, in);
   if (rc != OIDT_OK) {
-    if (sh.dmem) hipFreeAsync(sh.dmem, b->stream);
+    if (sh.dmem) (void)hipFreeAsync(sh.dmem, b->stream);
     sh = SlotHost();
     return rc;
   }
@@ -947,7 +956,7 @@ int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms) {
   float4* c = nullptr;
   HIP_TRY(hipMalloc(&a, n4 * 16));
   if (hipMalloc(&c, n4 * 16) != hipSuccess) {
-    hipFree(a);
+    (void)hipFree(a);
     return fail(OIDT_ENOMEM, "copy probe allocation failed");
   }
   HIP_TRY(hipMemset(a, 0, n4 * 16));
@@ -963,10 +972,10 @@ int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms) {
   float t = 0.f;
   HIP_TRY(hipEventElapsedTime(&t, e0, e1));
   *ms = t / reps;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipFree(a);
-  hipFree(c);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(c);
   return OIDT_OK;
 }
 
@@ -977,22 +986,22 @@ int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4) {
 
 int oidt_batch_destroy(oidt_batch* b) {
   if (!b) return OIDT_OK;
-  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto& sh : b->slots)
-    if (sh.dmem) hipFree(sh.dmem);
-  if (b->gexec) hipGraphExecDestroy(b->gexec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  if (b->d_tiles) hipFree(b->d_tiles);
-  if (b->d_work) hipFree(b->d_work);
-  if (b->d_counts) hipFree(b->d_counts);
-  if (b->d_slots) hipFree(b->d_slots);
-  for (auto& m : b->marks) hipEventDestroy(m.first);
-  for (auto e : b->free_events) hipEventDestroy(e);
+    if (sh.dmem) (void)hipFree(sh.dmem);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
+  if (b->d_work) (void)hipFree(b->d_work);
+  if (b->d_counts) (void)hipFree(b->d_counts);
+  if (b->d_slots) (void)hipFree(b->d_slots);
+  for (auto& m : b->marks) (void)hipEventDestroy(m.first);
+  for (auto e : b->free_events) (void)hipEventDestroy(e);
   b->staging.release();
-  if (b->aux) hipStreamSynchronize(b->aux);
-  if (b->d_pow) hipFree(b->d_pow);
-  if (b->aux) hipStreamDestroy(b->aux);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->aux) (void)hipStreamSynchronize(b->aux);
+  if (b->d_pow) (void)hipFree(b->d_pow);
+  if (b->aux) (void)hipStreamDestroy(b->aux);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return OIDT_OK;
 }

@@ -22,6 +22,14 @@
 
 namespace oidt {
 
+// Blocks are (64, 4): threadIdx.y is the wavefront index, the same for every
+// lane of a wave.  Reading it through readfirstlane lets the compiler keep
+// everything derived from it in scalar registers.  The marching fused step
+// gains from that (its per-plane values become scalars); the half-step
+// kernels measured slower with it (their CPML branches turn into scalar
+// branches between the loads), so they read threadIdx.y directly.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that the blocks
 // of one XCD take a contiguous range of tiles (shared L2 for x-neighbours).
 __device__ __forceinline__ int xcd_block() {
@@ -1142,73 +1150,200 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
 
 
 // ---------------------------------------------------------- fused Yee step ---
-// One launch per time step for TMz: E and H half-steps together, reading
+// One launch per time step for TMz: the E and H half-steps together, reading
 // field/psi set (n & 1) and writing the other set, so no tile races another.
-// A block owns 64 aligned rows (y) x 4*XC planes (x); wave w owns planes
-// i0 + w*XC .. +XC.  New Ez of the block goes to LDS; the H half-step needs it
-// at j-1 and i-1, so wave 0 also computes the plane below the block (i0-1) and
-// wave 1 the row below it (j0-1), neither stored.  Halo values come from the
-// same per-cell code as the owners', so they are bit-identical to them.  Per
-// cell and step each field is read once and written once (25 B in 2D FP32
-// instead of 37 B for the two half-step kernels).
+// Each wavefront owns a tile of 64 rows (y, one per lane) x XM planes (x) and
+// marches along x: the new Ez of the previous plane stays in a register (the
+// i-1 neighbour of Hy), the j-1 neighbour of Hx comes from the next-lower
+// lane, and every load of plane u+1 (fields, psi, coefficients) is issued
+// before the stores of plane u.  Before the march the wave computes, without
+// storing, the plane below its tile (i0-1) and the XM cells of the row below
+// it (j0-1, one per lane) with the same per-cell code as their owners, so
+// they are bit-identical to them.  Waves never wait for each other (no LDS,
+// no barriers); per cell and step each field is read once and written once.
+// The run's descriptor fields are copied to registers per tile (TmzView):
+// stores through the field pointers could otherwise alias them and force
+// reloads after every store.
 
-// New Ez of one cell (bulk curl, source, psi x, psi y), reading set A only.
 template <typename T>
-__device__ __forceinline__ T tmz_e_cell(const Slot<T>& s, int i, int j, int n, const T* __restrict__ EzA,
-                                        const T* __restrict__ HxA, const T* __restrict__ HyA, T* const* psiA,
-                                        T* const* psiB, bool store) {
-  const int nx = s.nx, ny = s.ny, N = s.N;
-  if (i < 0 || j < 0 || i >= nx || j >= ny) return T(0);
-  const unsigned idx = (unsigned)i * ny + j;
-  T v = EzA[idx];
-  const bool bulk = i >= s.ib && i < s.ie && j >= s.jb && j < s.je;
-  const bool src = s.src_field == F_EZ && i >= s.sx0 && i < s.sx1 && j >= s.sy0 && j < s.sy1;
-  const T jv = src ? s.jsrc[n] : T(0);
-  if (bulk) {
-    const T cb = cbsel(s, s.mat[idx], 2);
-    const T dHy = HyA[idx + ny] - HyA[idx];
-    const T dHx = HxA[idx + 1] - HxA[idx];
-    v = v + cb * (dHy * s.ike[0][i] - dHx * s.ike[1][j]);
-    if (src) v -= jv;
-    int dx = 0, cx = 0, dy = 0, cy = 0;
-    const int sx = slab_e(i, s.ib, s.ie, N, dx, cx);
-    const int sy = slab_e(j, s.jb, s.je, N, dy, cy);
-    if (sx >= 0) {
-      const T p = s.be[cx] * psiA[psi_id(F_EZ, 0, sx)][(size_t)dx * ny + j] + s.ceo[cx] * dHy;
-      if (store) psiB[psi_id(F_EZ, 0, sx)][(size_t)dx * ny + j] = p;
-      v += cb * p;
-    }
-    if (sy >= 0) {
-      const T p = s.be[cy] * psiA[psi_id(F_EZ, 1, sy)][(size_t)i * N + dy] + s.ceo[cy] * dHx;
-      if (store) psiB[psi_id(F_EZ, 1, sy)][(size_t)i * N + dy] = p;
-      v -= cb * p;
-    }
-  } else if (src) {
-    v -= jv;
-  }
+struct TmzView {
+  int nx, ny, N, ib, ie, jb, je, sx0, sx1, sy0, sy1, port0, port1, wy0, wy1, nplanes, pcells;
+  bool src_ez;
+  T db, cb[kMatMax];
+  const T *ike0, *ike1, *ikh0, *ikh1, *be, *ceo, *bh, *cho;
+  const uint8_t* mat;
+  T* ring;
+};
+
+template <typename T>
+__device__ __forceinline__ TmzView<T> tmz_view(const Slot<T>& s) {
+  TmzView<T> v;
+  v.nx = s.nx; v.ny = s.ny; v.N = s.N;
+  v.ib = s.ib; v.ie = s.ie; v.jb = s.jb; v.je = s.je;
+  v.sx0 = s.sx0; v.sx1 = s.sx1; v.sy0 = s.sy0; v.sy1 = s.sy1;
+  v.port0 = s.port[0]; v.port1 = s.port[1]; v.wy0 = s.wy0; v.wy1 = s.wy1;
+  v.nplanes = s.nplanes; v.pcells = s.pcells;
+  v.src_ez = s.src_field == F_EZ;
+  v.db = s.db;
+#pragma unroll
+  for (int q = 0; q < kMatMax; ++q) v.cb[q] = s.cbv[2][q];
+  v.ike0 = s.ike[0]; v.ike1 = s.ike[1]; v.ikh0 = s.ikh[0]; v.ikh1 = s.ikh[1];
+  v.be = s.be; v.ceo = s.ceo; v.bh = s.bh; v.cho = s.cho;
+  v.mat = s.mat;
+  v.ring = s.ring;
   return v;
 }
 
-template <typename T, int XC>
+// Where a TMz cell's Ez is updated (index ranges fdtd_funcs.py:44-80 and the
+// CPML slabs :82-100, FDTD_2D_TM).
+struct TmzWhere {
+  bool bulk, src;
+  int sx, sy, dx, dy, cx, cy;
+};
+
+template <typename T>
+__device__ __forceinline__ TmzWhere tmz_where(const TmzView<T>& v, int i, int j) {
+  TmzWhere f;
+  const bool in = i >= 0 && j >= 0 && i < v.nx && j < v.ny;
+  f.bulk = in && i >= v.ib && i < v.ie && j >= v.jb && j < v.je;
+  f.src = in && v.src_ez && i >= v.sx0 && i < v.sx1 && j >= v.sy0 && j < v.sy1;
+  f.dx = f.dy = f.cx = f.cy = 0;
+  f.sx = f.bulk ? slab_e(i, v.ib, v.ie, v.N, f.dx, f.cx) : -1;
+  f.sy = f.bulk ? slab_e(j, v.jb, v.je, v.N, f.dy, f.cy) : -1;
+  return f;
+}
+
+// Everything the new Ez of one cell reads from memory (indexes clamped into
+// the grid; coefficients only where the cell uses them).  Marching callers
+// skip hy: it is the previous plane's hyi.
+template <typename T>
+struct TmzIn {
+  T ez, hx, hxj, hy, hyi, pzx, pzy, ikx, bex, cex;
+  uint8_t m;
+};
+
+// Coefficients of the Ez update that depend on y only (fixed over a march).
+template <typename T>
+struct TmzJ {
+  T iky, bey, cey;
+};
+
+template <typename T>
+__device__ __forceinline__ TmzJ<T> tmz_jcoef(const TmzView<T>& v, const TmzWhere& f, int j) {
+  TmzJ<T> c;
+  const int jc = min(max(j, 0), v.ny - 1);
+  c.iky = f.bulk ? v.ike1[jc] : T(0);
+  c.bey = f.sy >= 0 ? v.be[f.cy] : T(0);
+  c.cey = f.sy >= 0 ? v.ceo[f.cy] : T(0);
+  return c;
+}
+
+template <typename T>
+__device__ __forceinline__ TmzIn<T> tmz_load(const TmzView<T>& v, int i, int j, const TmzWhere& f,
+                                             const T* __restrict__ EzA, const T* __restrict__ HxA,
+                                             const T* __restrict__ HyA, T* const* psiA, bool load_hy,
+                                             const T* pzy_lane = nullptr) {
+  TmzIn<T> a;
+  const int nx = v.nx, ny = v.ny;
+  const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);
+  const unsigned idx = (unsigned)ic * ny + jc;
+  a.ez = EzA[idx];
+  a.hx = HxA[idx];
+  a.hxj = HxA[jc + 1 < ny ? idx + 1 : idx];
+  a.hyi = HyA[ic + 1 < nx ? idx + ny : idx];
+  a.hy = load_hy ? HyA[idx] : T(0);
+  a.m = v.mat[idx];
+  a.ikx = f.bulk ? v.ike0[ic] : T(0);
+  a.pzx = f.sx >= 0 ? psiA[psi_id(F_EZ, 0, f.sx)][(size_t)f.dx * ny + jc] : T(0);
+  a.bex = f.sx >= 0 ? v.be[f.cx] : T(0);
+  a.cex = f.sx >= 0 ? v.ceo[f.cx] : T(0);
+  a.pzy = f.sy >= 0 ? (pzy_lane ? pzy_lane : psiA[psi_id(F_EZ, 1, f.sy)])[(size_t)ic * v.N + f.dy] : T(0);
+  return a;
+}
+
+template <typename T>
+struct TmzE {
+  T ez, pzx, pzy;
+};
+
+// New Ez in the reference order: bulk curl, source, psi x, psi y.
+template <typename T>
+__device__ __forceinline__ TmzE<T> tmz_e(const TmzView<T>& v, const TmzWhere& f, const TmzIn<T>& a,
+                                         const TmzJ<T>& c, T jv) {
+  TmzE<T> r;
+  r.ez = a.ez;
+  r.pzx = r.pzy = T(0);
+  if (f.bulk) {
+    const int q = (a.m >> 4) & 3;
+    const T cb = q == 0 ? v.cb[0] : (q == 1 ? v.cb[1] : (q == 2 ? v.cb[2] : v.cb[3]));
+    const T dHy = a.hyi - a.hy;
+    const T dHx = a.hxj - a.hx;
+    T e = a.ez + cb * (dHy * a.ikx - dHx * c.iky);
+    if (f.src) e -= jv;
+    if (f.sx >= 0) {
+      r.pzx = a.bex * a.pzx + a.cex * dHy;
+      e += cb * r.pzx;
+    }
+    if (f.sy >= 0) {
+      r.pzy = c.bey * a.pzy + c.cey * dHx;
+      e -= cb * r.pzy;
+    }
+    r.ez = e;
+  } else if (f.src) {
+    r.ez = a.ez - jv;
+  }
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T tmz_ez_cell(const TmzView<T>& v, int i, int j, T jv, const T* __restrict__ EzA,
+                                         const T* __restrict__ HxA, const T* __restrict__ HyA, T* const* psiA) {
+  const TmzWhere f = tmz_where(v, i, j);
+  return tmz_e(v, f, tmz_load(v, i, j, f, EzA, HxA, HyA, psiA, true), tmz_jcoef(v, f, j), jv).ez;
+}
+
+// What the march loads for one plane x = i.  The x-dependent parts are the
+// same for the whole wave (scalar registers); the per-lane part is the nine
+// values each cell reads.
+template <typename T>
+struct TmzPlane {
+  T ikx, bex, cex, ikhx, bhx, chx;            // x-only coefficients (wave-uniform)
+  T ez, hx, hxj, hyi, pzx, pzy, qxy, qyx;     // per lane
+  uint8_t m;
+};
+
+// x-only flags of plane i (recomputed where used rather than carried).
+struct TmzX {
+  bool ibulk, isrc, hbulk;
+  int sxe, dxe, cxe, sxh, dxh, cxh;
+};
+
+template <typename T>
+__device__ __forceinline__ TmzX tmz_x(const TmzView<T>& v, int i) {
+  TmzX x;
+  x.ibulk = i >= v.ib && i < v.ie;
+  x.isrc = i >= v.sx0 && i < v.sx1;
+  x.hbulk = i > v.ib && i < v.ie;
+  x.dxe = x.cxe = x.dxh = x.cxh = 0;
+  x.sxe = x.ibulk ? slab_e(i, v.ib, v.ie, v.N, x.dxe, x.cxe) : -1;
+  x.sxh = x.hbulk ? slab_h(i, v.ib, v.ie, v.N, x.dxh, x.cxh) : -1;
+  return x;
+}
+
+template <typename T, int XM>
 __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict__ slots,
                                                        const int4* __restrict__ tiles,
                                                        const int* __restrict__ counts, int tstep) {
-  // eL[r][c]: new Ez of plane i0 - 1 + r, row j0 - 1 + c
-  __shared__ T eL[kBY * XC + 1][kBX + 1];
+  static_assert(XM <= kBX, "the j0-1 halo row takes one cell per lane");
   const int ntiles = counts[0];
-  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+  const int lane = threadIdx.x;
+  const int stride = gridDim.x * kBY;
+  for (int tt = xcd_block() * kBY + wave_id(); tt < ntiles; tt += stride) {
     const int4 tl = tiles[tt];
     const Slot<T>& s = slots[tl.x];
     const int n = s.base + tstep;
     if (!s.active || n >= s.nt) continue;
-    const int nx = s.nx, ny = s.ny, N = s.N;
-    const int lane = threadIdx.x, w = threadIdx.y;
-    const int j = tl.z + lane;
-    const int jc = min(j, ny - 1);
-    const bool jok = j < ny;
-    const int i0 = tl.y;                 // first plane of the block
-    const int iw = i0 + w * XC;          // first plane of this wave
-    const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je;
+    const TmzView<T> v = tmz_view(s);
     const bool odd = (n & 1) != 0;
     const T* __restrict__ EzA = odd ? s.f2[F_EZ] : s.f[F_EZ];
     const T* __restrict__ HxA = odd ? s.f2[F_HX] : s.f[F_HX];
@@ -1216,127 +1351,149 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
     T* __restrict__ EzB = odd ? s.f[F_EZ] : s.f2[F_EZ];
     T* __restrict__ HxB = odd ? s.f[F_HX] : s.f2[F_HX];
     T* __restrict__ HyB = odd ? s.f[F_HY] : s.f2[F_HY];
-    T* const* psiA = odd ? s.psi2 : s.psi;
-    T* const* psiB = odd ? s.psi : s.psi2;
-    const T db = s.db;
-    const bool jin = jok && j >= jb && j < je, jin1 = jok && j > jb && j < je;
+    T* const* psiAll = odd ? s.psi2 : s.psi;
+    T* const* psiBll = odd ? s.psi : s.psi2;
+    const T jv = v.src_ez ? s.jsrc[n] : T(0);
+    const int nx = v.nx, ny = v.ny, N = v.N;
+    const int i0 = tl.y, j0 = tl.z;
+    const int j = j0 + lane;
+    const bool jok = j < ny;
+    const int jc = min(j, ny - 1);
+    // row (y-only) constants of this lane, fixed over the march
+    const bool jbulk = jok && j >= v.jb && j < v.je;          // Ez and Hy rows
+    const bool jin1 = jok && j > v.jb && j < v.je;            // Hx rows
+    const bool jsrc = jok && v.src_ez && j >= v.sy0 && j < v.sy1;
     int dye = 0, cye = 0, dyh = 0, cyh = 0;
-    const int sye = jin ? slab_e(j, jb, je, N, dye, cye) : -1;
-    const int syh = jin1 ? slab_h(j, jb, je, N, dyh, cyh) : -1;
-    const T ikye = jin ? s.ike[1][j] : T(0);
-    const T ikyh = jin1 ? s.ikh[1][j] : T(0);
-    const T bey = sye >= 0 ? s.be[cye] : T(0), cey = sye >= 0 ? s.ceo[cye] : T(0);
-    const T bhy = syh >= 0 ? s.bh[cyh] : T(0), chy = syh >= 0 ? s.cho[cyh] : T(0);
-    const bool jsrc = jok && s.src_field == F_EZ && j >= s.sy0 && j < s.sy1;
-    const T jv = jsrc ? s.jsrc[n] : T(0);
-    const bool jcap = jok && j >= s.wy0 && j <= s.wy1;
-    const unsigned offj = (jc + 1 < ny) ? 1u : 0u;
-    // ---- loads of the wave's XC planes (clamped, all before any store)
-    T ez[XC], hx[XC], hxj[XC], hy[XC + 1], pzx[XC], pzy[XC], pxy[XC], pyx[XC];
-    uint8_t m[XC];
-#pragma unroll
-    for (int u = 0; u <= XC; ++u) {
-      const int i = min(iw + u, nx - 1);
+    const int sye = jbulk ? slab_e(j, v.jb, v.je, N, dye, cye) : -1;
+    const int syh = jin1 ? slab_h(j, v.jb, v.je, N, dyh, cyh) : -1;
+    TmzJ<T> jcf;
+    jcf.iky = jbulk ? v.ike1[j] : T(0);
+    jcf.bey = sye >= 0 ? v.be[cye] : T(0);
+    jcf.cey = sye >= 0 ? v.ceo[cye] : T(0);
+    const T ikyh = jin1 ? v.ikh1[j] : T(0);
+    const T bhy = syh >= 0 ? v.bh[cyh] : T(0), chy = syh >= 0 ? v.cho[cyh] : T(0);
+    const bool jcap = jok && j >= v.wy0 && j <= v.wy1;
+    // y-slab psi arrays of this lane's row; the x-slab arrays are looked up
+    // in the few planes that need them
+    const T* pzyA = sye >= 0 ? psiAll[psi_id(F_EZ, 1, sye)] : nullptr;
+    T* pzyB = sye >= 0 ? psiBll[psi_id(F_EZ, 1, sye)] : nullptr;
+    const T* pxyA = syh >= 0 ? psiAll[psi_id(F_HX, 1, syh)] : nullptr;
+    T* pxyB = syh >= 0 ? psiBll[psi_id(F_HX, 1, syh)] : nullptr;
+    const int iend = min(i0 + XM, nx);
+    auto load_plane = [&](int i) {
+      TmzPlane<T> p;
+      const TmzX x = tmz_x(v, i);
       const unsigned idx = (unsigned)i * ny + jc;
-      hy[u] = HyA[idx];
-      if (u < XC) {
-        ez[u] = EzA[idx];
-        hx[u] = HxA[idx];
-        hxj[u] = HxA[idx + offj];
-        m[u] = s.mat[idx];
-        pzy[u] = sye >= 0 ? psiA[psi_id(F_EZ, 1, sye)][(size_t)i * N + dye] : T(0);
-        pxy[u] = syh >= 0 ? psiA[psi_id(F_HX, 1, syh)][(size_t)i * N + dyh] : T(0);
-        int dx = 0, cx = 0;
-        const int sxe = slab_e(i, ib, ie, N, dx, cx);
-        pzx[u] = sxe >= 0 ? psiA[psi_id(F_EZ, 0, sxe)][(size_t)dx * ny + jc] : T(0);
-        const int sxh = slab_h(i, ib, ie, N, dx, cx);
-        pyx[u] = sxh >= 0 ? psiA[psi_id(F_HY, 0, sxh)][(size_t)dx * ny + jc] : T(0);
-      }
-    }
-    // ---- E half-step of the owned cells
-#pragma unroll
-    for (int u = 0; u < XC; ++u) {
-      const int i = iw + u;
-      const bool iok = i < nx;
-      const unsigned idx = (unsigned)min(i, nx - 1) * ny + jc;
-      const bool bulk = iok && jin && i >= ib && i < ie;
-      const bool src = iok && jsrc && i >= s.sx0 && i < s.sx1;
-      T v = ez[u];
-      if (bulk) {
-        const T cb = cbsel(s, m[u], 2);
-        const T dHy = hy[u + 1] - hy[u];
-        const T dHx = hxj[u] - hx[u];
-        v = v + cb * (dHy * s.ike[0][i] - dHx * ikye);
-        if (src) v -= jv;
-        int dx = 0, cx = 0;
-        const int sx = slab_e(i, ib, ie, N, dx, cx);
-        if (sx >= 0) {
-          const T p = s.be[cx] * pzx[u] + s.ceo[cx] * dHy;
-          psiB[psi_id(F_EZ, 0, sx)][(size_t)dx * ny + j] = p;
-          v += cb * p;
-        }
-        if (sye >= 0) {
-          const T p = bey * pzy[u] + cey * dHx;
-          psiB[psi_id(F_EZ, 1, sye)][(size_t)i * N + dye] = p;
-          v -= cb * p;
-        }
-        EzB[idx] = v;
-      } else if (src) {
-        v -= jv;
-        EzB[idx] = v;
-      }
-      eL[1 + w * XC + u][1 + lane] = v;
-      if (jcap && iok) {
-        if (i == s.port[0]) capture(s, tstep, 0, j - s.wy0, v);
-        if (i == s.port[1]) capture(s, tstep, 3, j - s.wy0, v);
-      }
-    }
-    // ---- halo: plane i0-1 (wave 0) and row j0-1 (wave 1), not stored
-    if (w == 0) eL[0][1 + lane] = tmz_e_cell(s, i0 - 1, j, n, EzA, HxA, HyA, psiA, psiB, false);
-    if (w == 1)
-      for (int r = lane; r <= kBY * XC; r += kBX)
-        eL[r][0] = tmz_e_cell(s, i0 - 1 + r, tl.z - 1, n, EzA, HxA, HyA, psiA, psiB, false);
-    __syncthreads();
-    // ---- H half-step of the owned cells
-#pragma unroll
-    for (int u = 0; u < XC; ++u) {
-      const int i = iw + u;
-      if (!jok || i >= nx) continue;
-      const unsigned idx = (unsigned)i * ny + j;
-      const int r = 1 + w * XC + u;
-      const T ezc = eL[r][1 + lane];
-      if (i >= ib && i < ie && jin1) {
-        const T dEz = ezc - eL[r][lane];
-        T v = hx[u] + db * (-dEz) * ikyh;
-        if (syh >= 0) {
-          const T p = bhy * pxy[u] + chy * dEz;
-          psiB[psi_id(F_HX, 1, syh)][(size_t)i * N + dyh] = p;
-          v -= db * p;
-        }
-        HxB[idx] = v;
-      }
-      T v = hy[u];
-      if (i > ib && i < ie && jin) {
-        const T dEz = ezc - eL[r - 1][1 + lane];
-        v = v + db * dEz * s.ikh[0][i];
-        int dx = 0, cx = 0;
-        const int sx = slab_h(i, ib, ie, N, dx, cx);
-        if (sx >= 0) {
-          const T p = s.bh[cx] * pyx[u] + s.cho[cx] * dEz;
-          psiB[psi_id(F_HY, 0, sx)][(size_t)dx * ny + j] = p;
-          v += db * p;
-        }
-        HyB[idx] = v;
+      p.ikx = x.ibulk ? v.ike0[i] : T(0);
+      p.bex = x.sxe >= 0 ? v.be[x.cxe] : T(0);
+      p.cex = x.sxe >= 0 ? v.ceo[x.cxe] : T(0);
+      p.ikhx = v.ikh0[i];
+      p.bhx = x.sxh >= 0 ? v.bh[x.cxh] : T(0);
+      p.chx = x.sxh >= 0 ? v.cho[x.cxh] : T(0);
+      p.ez = EzA[idx];
+      p.hx = HxA[idx];
+      p.hxj = HxA[jc + 1 < ny ? idx + 1 : idx];
+      p.hyi = HyA[i + 1 < nx ? idx + ny : idx];
+      p.m = v.mat[idx];
+      const bool bulk = x.ibulk && jbulk;
+      p.pzx = bulk && x.sxe >= 0 ? psiAll[psi_id(F_EZ, 0, x.sxe)][(size_t)x.dxe * ny + jc] : T(0);
+      p.pzy = bulk && sye >= 0 ? pzyA[(size_t)i * N + dye] : T(0);
+      p.qxy = syh >= 0 ? pxyA[(size_t)i * N + dyh] : T(0);
+      p.qyx = jbulk && x.sxh >= 0 ? psiAll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + jc] : T(0);
+      return p;
+    };
+    // ---- halos from set A, not stored: row j0-1 (lane u holds plane i0+u) and plane i0-1
+    const T colE = lane < XM ? tmz_ez_cell(v, i0 + lane, j0 - 1, jv, EzA, HxA, HyA, psiAll) : T(0);
+    T prevE = tmz_ez_cell(v, i0 - 1, j, jv, EzA, HxA, HyA, psiAll);
+    // ---- march along x.  Loads run two planes ahead of the updates (planes
+    // i+1 and i+2 are in flight while plane i is updated); Hy(i) is the
+    // previous plane's Hy(i+1).
+    T hyC = HyA[(unsigned)i0 * ny + jc];
+    TmzPlane<T> p0 = load_plane(i0);
+    TmzPlane<T> p1 = i0 + 1 < iend ? load_plane(i0 + 1) : p0;
+#pragma unroll 1
+    for (int u = 0; u < XM; ++u) {
+      const int i = i0 + u;
+      if (i >= iend) break;
+      const TmzPlane<T> p2 = i + 2 < iend ? load_plane(i + 2) : p1;
+      // E half-step of (i, j), through the same update as the halo cells
+      const TmzX x = tmz_x(v, i);
+      TmzWhere f;
+      f.bulk = x.ibulk && jbulk;
+      f.src = x.isrc && jsrc;
+      f.sx = f.bulk ? x.sxe : -1;
+      f.dx = x.dxe;
+      f.cx = 0;
+      f.sy = f.bulk ? sye : -1;
+      f.dy = dye;
+      f.cy = cye;
+      TmzIn<T> a;
+      a.ez = p0.ez;
+      a.hx = p0.hx;
+      a.hxj = p0.hxj;
+      a.hy = hyC;
+      a.hyi = p0.hyi;
+      a.pzx = p0.pzx;
+      a.pzy = p0.pzy;
+      a.ikx = p0.ikx;
+      a.bex = p0.bex;
+      a.cex = p0.cex;
+      a.m = p0.m;
+      const TmzE<T> e = tmz_e(v, f, a, jcf, jv);
+      const unsigned idx = (unsigned)i * ny + jc;
+      if (f.bulk) {
+        EzB[idx] = e.ez;
+        if (f.sx >= 0) psiBll[psi_id(F_EZ, 0, f.sx)][(size_t)f.dx * ny + j] = e.pzx;
+        if (f.sy >= 0) pzyB[(size_t)i * N + f.dy] = e.pzy;
+      } else if (f.src) {
+        EzB[idx] = e.ez;
       }
       if (jcap) {
-        const int c = j - s.wy0;
-        if (i == s.port[0]) capture(s, tstep, 1, c, v);
-        if (i == s.port[0] + 1) capture(s, tstep, 2, c, v);
-        if (i == s.port[1]) capture(s, tstep, 4, c, v);
-        if (i == s.port[1] + 1) capture(s, tstep, 5, c, v);
+        const size_t c = (size_t)(j - v.wy0);
+        if (i == v.port0) v.ring[((size_t)tstep * v.nplanes + 0) * v.pcells + c] = e.ez;
+        if (i == v.port1) v.ring[((size_t)tstep * v.nplanes + 3) * v.pcells + c] = e.ez;
       }
+      // H half-step of (i, j): Ez(j-1) from the lower lane, lane 0 from the halo row
+      T ezj = __shfl_up(e.ez, 1, kBX);
+      const T ecol = __shfl(colE, u, kBX);
+      if (lane == 0) ezj = ecol;
+      if (jok) {
+        const T db = v.db;
+        if (x.ibulk && jin1) {
+          const T dEz = e.ez - ezj;
+          T hv = p0.hx + db * (-dEz) * ikyh;
+          if (syh >= 0) {
+            const T q = bhy * p0.qxy + chy * dEz;
+            pxyB[(size_t)i * N + dyh] = q;
+            hv -= db * q;
+          }
+          HxB[idx] = hv;
+        }
+        T hv = hyC;
+        if (x.hbulk && jbulk) {
+          const T dEz = e.ez - prevE;
+          hv = hv + db * dEz * p0.ikhx;
+          if (x.sxh >= 0) {
+            const T q = p0.bhx * p0.qyx + p0.chx * dEz;
+            psiBll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + j] = q;
+            hv += db * q;
+          }
+          HyB[idx] = hv;
+        }
+        if (jcap) {
+          const size_t c = (size_t)(j - v.wy0);
+          const size_t base = (size_t)tstep * v.nplanes;
+          if (i == v.port0) v.ring[(base + 1) * v.pcells + c] = hv;
+          if (i == v.port0 + 1) v.ring[(base + 2) * v.pcells + c] = hv;
+          if (i == v.port1) v.ring[(base + 4) * v.pcells + c] = hv;
+          if (i == v.port1 + 1) v.ring[(base + 5) * v.pcells + c] = hv;
+        }
+      }
+      prevE = e.ez;
+      hyC = p0.hyi;
+      p0 = p1;
+      p1 = p2;
     }
-    __syncthreads();   // eL is reused by the next tile
   }
 }
 
@@ -1497,7 +1654,7 @@ __global__ __launch_bounds__(kThreads) void k_3d_step(const Slot<T>* __restrict_
     const int n = s.base + tstep;
     if (!s.active || n >= s.nt) continue;
     const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
-    const int lane = threadIdx.x, w = threadIdx.y;
+    const int lane = threadIdx.x, w = wave_id();
     const int i0 = tl.y, j0 = tl.z, k0 = tl.w;
     const int j = j0 + w, k = k0 + lane;
     const bool odd = (n & 1) != 0;

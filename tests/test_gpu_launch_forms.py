@@ -10,6 +10,7 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step (TMz and 3D)
+FUSED_TM = [(3, 16), (3, 32)]                      # longer marches of the fused TMz step
 
 
 @pytest.mark.gpu
@@ -22,7 +23,7 @@ def test_all_launch_forms_match_oracle(scheme):
     ref_f = o.fields()
     od = o.dft()
     w = window(scheme, o.c)
-    for variant, xc in FORMS + (FUSED if scheme != 'te' else []):
+    for variant, xc in FORMS + (FUSED if scheme != 'te' else []) + (FUSED_TM if scheme == 'tm' else []):
         print('case', scheme, 'variant', variant, 'xchunk', xc, flush=True)
         b = Batch(scheme, np.float32, max_slots=1)
         b.set_launch(xc, 0, variant)
