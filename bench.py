@@ -82,15 +82,15 @@ def committed_traffic(kernel, workload_key, variant, xchunk):
 
 
 def choose_launch(args, runs):
-    """(variant, xchunk, autotune record) for the batch of ``runs``: the
+    """(variant, xchunk, grid, autotune record) for the batch of ``runs``: the
     command-line form, or the autotuner's pick when --variant is 0."""
     if args.variant:
-        return args.variant, args.xchunk, None
+        return args.variant, args.xchunk, args.grid, None
     from oidt_amd import autotune
     t0 = time.perf_counter()
-    (variant, xc), times = autotune.tune(runs, use_cache=False, cache_path=None)
-    return variant, xc, {'ms_per_step': {k: round(v, 5) for k, v in times.items()},
-                         'seconds': round(time.perf_counter() - t0, 2)}
+    (variant, xc, grid), times = autotune.tune(runs, use_cache=False, cache_path=None)
+    return variant, xc, grid, {'ms_per_step': {k: round(v, 5) for k, v in times.items()},
+                               'seconds': round(time.perf_counter() - t0, 2)}
 
 
 def workload(name):
@@ -122,9 +122,9 @@ def bench_serving(args):
         # autotune each scheme once on (up to) 16 of its requests
         for scheme in sorted({r.setup.scheme for r in reqs}):
             sub = [(r.setup, r.fg) for r in reqs if r.setup.scheme == scheme][:16]
-            variant, xchunk, rec = choose_launch(args, sub)
-            be.launch[scheme] = (variant, xchunk)
-            tuned[scheme] = dict(rec, choice=[variant, xchunk])
+            variant, xchunk, grid, rec = choose_launch(args, sub)
+            be.launch[scheme] = (variant, xchunk, grid)
+            tuned[scheme] = dict(rec, choice=[variant, xchunk, grid])
     # plain: each request alone, back to back, same kernel forms
     t0 = time.perf_counter()
     plain = serve.serve_plain(reqs, be.launch)
@@ -185,13 +185,13 @@ def bench_plain_ensemble(args, descs, label):
         p = dict(p, nt=K)
         s, up, lo = configs.realise(p)
         runs.append((s, materials.fg_mask(s, up, lo)))
-    variant, xchunk, tuned = choose_launch(args, runs[:1])
+    variant, xchunk, grid, tuned = choose_launch(args, runs[:1])
     # warm the device on the first run
-    Solver(*runs[0]).set_launch(xchunk, 0, variant).advance(64).sync().close()
+    Solver(*runs[0]).set_launch(xchunk, grid, variant).advance(64).sync().close()
     t0 = time.perf_counter()
     for s, fg in runs:
         sv = Solver(s, fg)
-        sv.set_launch(xchunk, 0, variant)
+        sv.set_launch(xchunk, grid, variant)
         sv.run()
         sv.close()
     wall = time.perf_counter() - t0
@@ -201,7 +201,8 @@ def bench_plain_ensemble(args, descs, label):
                       'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True, 'scaling': 'weak',
                       'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
                       'config': {'workload': label + ', plain: one run at a time incl. create/upload',
-                                 'runs': len(runs), 'variant': variant, 'xchunk': xchunk, 'autotune': tuned}}),
+                                 'runs': len(runs), 'variant': variant, 'xchunk': xchunk, 'grid_cap': grid,
+                                 'autotune': tuned}}),
           flush=True)
 
 
@@ -255,10 +256,10 @@ def main():
         s, up, lo = configs.realise(p)
         runs.append((p, s, materials.fg_mask(s, up, lo)))
     scheme, prec = runs[0][1].scheme, runs[0][1].precision
-    variant, xchunk, tuned = choose_launch(args, [(s, fg) for _, s, fg in runs])
+    variant, xchunk, grid, tuned = choose_launch(args, [(s, fg) for _, s, fg in runs])
     b = Batch(scheme, prec, max_slots=len(runs))
-    b.set_launch(xchunk, args.grid, variant)
-    variant, xchunk, _ = b.launch()
+    b.set_launch(xchunk, grid, variant)
+    variant, xchunk, grid = b.launch()
     for p, s, fg in runs:
         b.add(s, fg)
     b.advance(warm // chunk)
@@ -274,7 +275,7 @@ def main():
     # Per-kernel durations with HIP events on the batch stream (a second batch of
     # the same runs, so the timed run above is untouched).
     pb = Batch(scheme, prec, max_slots=len(runs))
-    pb.set_launch(xchunk, args.grid, variant)
+    pb.set_launch(xchunk, grid, variant)
     for p, s, fg in runs:
         pb.add(s, fg)
     import ctypes as C
@@ -303,7 +304,7 @@ def main():
         'steps': K, 'warmup': warm, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32' if es == 4 else 'f64', 'data': 'synthetic',
         'config': {'workload': label, 'runs': len(runs), 'cells_per_step': cells,
-                   'steps_per_run': K, 'xchunk': xchunk, 'variant': variant, 'grid_cap': args.grid or 8192,
+                   'steps_per_run': K, 'xchunk': xchunk, 'variant': variant, 'grid_cap': grid,
                    'parallelism': 'single GPU',
                    'autotune': tuned},
         'hbm_roofline_frac_step': round(value * 1e6 * (be + bh) / (HBM_PEAK_GBS * 1e9), 4),

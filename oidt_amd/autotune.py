@@ -20,10 +20,11 @@ import numpy as np
 from . import _lib
 from .solver import Batch
 
+# (kernel form, x-chunk depth, grid cap; 0 = library default)
 CANDIDATES = {
-    'tm': [(2, 2), (2, 4), (2, 8), (3, 1), (3, 2), (3, 4)],
-    'te': [(2, 2), (2, 4), (2, 8), (1, 4)],
-    '3d': [(2, 1), (2, 2), (2, 4), (1, 4), (3, 1), (3, 2), (3, 4)],
+    'tm': [(2, 4, 0), (2, 4, 65536), (2, 8, 0), (3, 1, 0), (3, 2, 0), (3, 8, 0), (3, 16, 0), (3, 32, 0)],
+    'te': [(2, 2, 0), (2, 4, 0), (2, 4, 65536), (2, 8, 0), (1, 4, 0)],
+    '3d': [(2, 1, 0), (2, 1, 65536), (2, 2, 0), (1, 4, 0), (3, 1, 0), (3, 2, 0)],
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'oidt_amd', 'autotune.json')
@@ -39,13 +40,13 @@ def workload_class(setups):
                               int(round(np.log2(max(cells, 1)))))
 
 
-def step_time(runs, variant, xchunk, chunks=2):
+def step_time(runs, variant, xchunk, grid=0, chunks=2):
     """Mean device time per time step (ms) of one candidate on ``runs``
     [(setup, fg), ...]: stencil launches of a chunk plus its DFT fold."""
     s0 = runs[0][0]
     b = Batch(s0.scheme, s0.precision, max_slots=len(runs))
     try:
-        b.set_launch(xchunk, 0, variant)
+        b.set_launch(xchunk, grid, variant)
         for s, fg in runs:
             b.add(s, fg)
         out = (C.c_double * 4)()
@@ -58,7 +59,7 @@ def step_time(runs, variant, xchunk, chunks=2):
 
 
 def tune(runs, candidates=None, cache_path=DEFAULT_CACHE, use_cache=True, verbose=False):
-    """Best (variant, xchunk) for ``runs``; returns (choice, {candidate: ms/step})."""
+    """Best (variant, xchunk, grid) for ``runs``; returns (choice, {candidate: ms/step})."""
     setups = [s for s, _ in runs]
     key = workload_class(setups)
     cache = {}
@@ -75,13 +76,13 @@ def tune(runs, candidates=None, cache_path=DEFAULT_CACHE, use_cache=True, verbos
     # short runs: tune on the first chunks of each run
     short = [(s, fg) for s, fg in runs]
     times = {}
-    for variant, xc in cands:
+    for variant, xc, grid in cands:
         t0 = time.perf_counter()
-        ms = step_time(short, variant, xc)
-        times['%d,%d' % (variant, xc)] = ms
+        ms = step_time(short, variant, xc, grid)
+        times['%d,%d,%d' % (variant, xc, grid)] = ms
         if verbose:
-            print('autotune %s variant %d xchunk %d: %.4f ms/step (%.1f s)' % (key, variant, xc, ms,
-                                                                               time.perf_counter() - t0), flush=True)
+            print('autotune %s variant %d xchunk %d grid %d: %.4f ms/step (%.1f s)'
+                  % (key, variant, xc, grid, ms, time.perf_counter() - t0), flush=True)
     best = min(times, key=times.get)
     choice = tuple(int(v) for v in best.split(','))
     if cache_path:

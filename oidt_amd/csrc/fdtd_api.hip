@@ -543,9 +543,14 @@ void launch_advance(oidt_batch* b) {
                      static_cast<Slot<T>*>(b->d_slots), b->max_slots);
 }
 
+// Grid of the stencil launches.  The marching step gives every wave exactly
+// one tile (no cap): a wave holding a second tile at the end of the launch
+// leaves a tail that measured 15-30 % of the step.  The other forms keep a
+// persistent grid of at most 8192 blocks unless overridden.
 int stencil_grid(const oidt_batch* b) {
-  const int items = march_tiles(b) ? (b->ntiles + kBY - 1) / kBY : b->ntiles;
-  return grid_for(items, b->grid_override > 0 ? b->grid_override : 8192);
+  const bool m = march_tiles(b);
+  const int items = m ? (b->ntiles + kBY - 1) / kBY : b->ntiles;
+  return grid_for(items, b->grid_override > 0 ? b->grid_override : (m ? (1 << 24) : 8192));
 }
 int dft_grid(const oidt_batch* b) { return grid_for(b->nwork, 8192); }
 

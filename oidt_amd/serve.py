@@ -62,8 +62,9 @@ class DeviceBackend:
     """The GPU: one oidt batch per (scheme, precision)."""
 
     def __init__(self, max_slots=64, xchunk=0, variant=2, launch=None):
-        """``launch`` maps a scheme ('tm', 'te', '3d') to its (variant, xchunk),
-        e.g. the autotuner's picks; other schemes use ``variant``/``xchunk``."""
+        """``launch`` maps a scheme ('tm', 'te', '3d') to its (variant, xchunk[,
+        grid]), e.g. the autotuner's picks; other schemes use ``variant`` and
+        ``xchunk``."""
         self.max_slots = max_slots
         self.xchunk, self.variant = xchunk, variant
         self.launch = dict(launch or {})
@@ -76,10 +77,11 @@ class DeviceBackend:
     def batch(self, key, setup):
         if key not in self.batches:
             b = S.Batch(setup.scheme, setup.precision, self.max_slots)
-            variant, xchunk = self.launch.get(setup.scheme, (self.variant, self.xchunk))
+            variant, xchunk, grid = (tuple(self.launch[setup.scheme]) + (0,))[:3] if setup.scheme in self.launch \
+                else (self.variant, self.xchunk, 0)
             if variant == 3 and setup.scheme == 'te':
                 variant = 2
-            b.set_launch(xchunk, 0, variant)
+            b.set_launch(xchunk, grid, variant)
             self.batches[key] = b
             self.resident[key] = 0
         return self.batches[key]
@@ -231,8 +233,8 @@ def serve_plain(requests, launch=None):
             prepare_request(r, DeviceBackend())
         sv = S.Solver(r.setup, r.fg)
         if launch and r.setup.scheme in launch:
-            variant, xchunk = launch[r.setup.scheme]
-            sv.set_launch(xchunk, 0, variant)
+            variant, xchunk, grid = (tuple(launch[r.setup.scheme]) + (0,))[:3]
+            sv.set_launch(xchunk, grid, variant)
         sv.run()
         r.result = sv.dft()
         sv.close()
