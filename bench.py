@@ -129,8 +129,8 @@ def bench_serving(args):
     t0 = time.perf_counter()
     plain = serve.serve_plain(reqs, be.launch)
     plain_wall = time.perf_counter() - t0
-    plain_rate = total_cu / plain_wall / 1e6
     total_cu = sum(r.cell_updates for r in reqs)
+    plain_rate = total_cu / plain_wall / 1e6
     # saturated continuous batching (everything queued at t = 0)
     hbm = int(0.8 * __import__('oidt_amd._lib', fromlist=['x']).device_info()['hbm_free_bytes'])
     for r in reqs:

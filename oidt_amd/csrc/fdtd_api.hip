@@ -429,7 +429,7 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
 }
 
 // Kernels whose tiles are per wavefront (four tiles per block).
-bool march_tiles(const oidt_batch* b) { return b->variant == 3 && b->scheme == SCHEME_TMZ; }
+bool march_tiles(const oidt_batch* b) { return b->variant == 3; }
 
 // Rebuild the tile and DFT work lists for the resident slots and upload them.
 int rebuild_lists(oidt_batch* b) {
@@ -761,7 +761,7 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
     return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   if (variant == 3 && b->scheme == SCHEME_3D && (xchunk ? xchunk : b->xchunk) > kFused3dMaxXc)
-    return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4 or 8");
+    return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4, 8 or 16");
   b->variant = variant;
   for (auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued == sh.admit_chunk) sh.fused = variant == 3;

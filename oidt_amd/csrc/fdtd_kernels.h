@@ -1498,33 +1498,62 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
 }
 
 // ------------------------------------------------------- fused 3D step ---
-// One launch per time step for 3D, as k_tmz_step: it reads field/psi set
-// (n & 1) and writes the other.  A block owns 64 z-cells (lanes) x 4 y-rows
-// (waves) x XC x-planes.  The H half-step at (i,j,k) needs new E at i-1
-// (Ey, Ez), j-1 (Ex, Ez) and k-1 (Ex, Ey).  Each wave keeps its new E per
-// plane in registers (the i-1 neighbour inside the tile) and computes the
-// plane below the tile for its own row; the row below the tile (j0-1) is
-// spread over the waves by plane; the column below the tile (k0-1, 4 x XC
-// cells) is one partial pass of one wave.  Ex, Ey, Ez of the tile and of
-// those halos go through LDS for the j-1 and k-1 reads.  Halo cells use the
-// same load and update code as their owners, so they are bit-identical.
+// One launch per time step for 3D, reading field/psi set (n & 1) and writing
+// the other, so no tile races another.  Each wavefront owns a tile of 64
+// z-cells (lanes) x 4 y-rows x XM x-planes and walks it as one sequence of
+// cell items: first the row below the tile (j0-1, all XM planes: new E only,
+// not stored), then for each of its 4 rows the plane below the tile (i0-1,
+// not stored) followed by the row's XM planes.  For the H update at (i, j, k)
+//   * the j-1 neighbour (new Ex, Ez) is the previous row's result at the same
+//     plane, kept per plane in the wave's own LDS row buffer,
+//   * the i-1 neighbour (new Ey, Ez) is the previous item of the same row,
+//   * the k-1 neighbour (new Ex, Ey) comes from the next-lower lane; lane 0
+//     takes it from the column below the tile (4 x XM cells, computed once
+//     per tile, one per lane).
+// The loads of the next item are issued before the stores of the current one,
+// and waves never wait for each other.  Halo cells go through the same load
+// and update code as their owners, so they are bit-identical to them.  The
+// run's descriptor fields are copied to registers per tile (View3): stores
+// through the field pointers could otherwise alias them and force reloads.
 
-// Old-set values one cell's new E reads (only the MASK components' inputs).
 template <typename T>
-struct E3In {
-  T ex, ey, ez, hx, hy, hz, hzj, hyk, hxk, hzi, hyi, hxj;
-  T qxy, qxz, qyx, qyz, qzx, qzy;
-  uint8_t m;
+struct View3 {
+  int nx, ny, nz, N, ib, ie, jb, je, kb, ke, src_field, sx0, sx1, sy0, sy1, sz0, sz1;
+  T db;
+  T cbv[3][kMatMax];
+  const T* ike[3];
+  const T* ikh[3];
+  const T *be, *ceo, *bh, *cho;
+  const uint8_t* mat;
 };
 
-// New E of one cell; hx, hy, hz are the cell's old H (for the H half-step).
 template <typename T>
-struct E3Out {
-  T ex, ey, ez, hx, hy, hz;
-  T pxy, pxz, pyx, pyz, pzx, pzy;
-};
+__device__ __forceinline__ View3<T> view3(const Slot<T>& s) {
+  View3<T> v;
+  v.nx = s.nx; v.ny = s.ny; v.nz = s.nz; v.N = s.N;
+  v.ib = s.ib; v.ie = s.ie; v.jb = s.jb; v.je = s.je; v.kb = s.kb; v.ke = s.ke;
+  v.src_field = s.src_field;
+  v.sx0 = s.sx0; v.sx1 = s.sx1; v.sy0 = s.sy0; v.sy1 = s.sy1; v.sz0 = s.sz0; v.sz1 = s.sz1;
+  v.db = s.db;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int q = 0; q < kMatMax; ++q) v.cbv[c][q] = s.cbv[c][q];
+    v.ike[c] = s.ike[c];
+    v.ikh[c] = s.ikh[c];
+  }
+  v.be = s.be; v.ceo = s.ceo; v.bh = s.bh; v.cho = s.cho;
+  v.mat = s.mat;
+  return v;
+}
 
-// Flags of one cell for the E half-step (3D index ranges of
+template <typename T>
+__device__ __forceinline__ T cbsel3(const View3<T>& v, uint8_t m, int c) {
+  const int q = (m >> (2 * c)) & 3;
+  return q == 0 ? v.cbv[c][0] : (q == 1 ? v.cbv[c][1] : (q == 2 ? v.cbv[c][2] : v.cbv[c][3]));
+}
+
+// Where one cell's E components are updated (3D index ranges of
 // fdtd_funcs.py:44-80, FDTD_3D; CPML slabs :82-126).
 struct E3Where {
   bool bx, by, bz, src;
@@ -1532,309 +1561,344 @@ struct E3Where {
 };
 
 template <typename T>
-__device__ __forceinline__ E3Where e3_where(const Slot<T>& s, int i, int j, int k) {
+__device__ __forceinline__ E3Where e3_where(const View3<T>& v, int i, int j, int k) {
   E3Where f;
-  const bool in = i >= 0 && j >= 0 && k >= 0 && i < s.nx && j < s.ny && k < s.nz;
-  const bool iin = in && i >= s.ib && i < s.ie, iin1 = in && i > s.ib && i < s.ie;
-  const bool jin = in && j >= s.jb && j < s.je, jin1 = in && j > s.jb && j < s.je;
-  const bool kin = in && k >= s.kb && k < s.ke, kin1 = in && k > s.kb && k < s.ke;
+  const bool in = i >= 0 && j >= 0 && k >= 0 && i < v.nx && j < v.ny && k < v.nz;
+  const bool iin = in && i >= v.ib && i < v.ie, iin1 = in && i > v.ib && i < v.ie;
+  const bool jin = in && j >= v.jb && j < v.je, jin1 = in && j > v.jb && j < v.je;
+  const bool kin = in && k >= v.kb && k < v.ke, kin1 = in && k > v.kb && k < v.ke;
   f.bx = iin1 && jin && kin;
   f.by = iin && jin1 && kin;
   f.bz = iin && jin && kin1;
-  f.src = in && s.src_field == F_EY && i >= s.sx0 && i < s.sx1 && j >= s.sy0 && j < s.sy1 && k >= s.sz0 &&
-          k < s.sz1;
+  f.src = in && v.src_field == F_EY && i >= v.sx0 && i < v.sx1 && j >= v.sy0 && j < v.sy1 && k >= v.sz0 &&
+          k < v.sz1;
   f.di = f.dj = f.dk = f.ci = f.cj = f.ck = 0;
-  f.si = in ? slab_e(i, s.ib, s.ie, s.N, f.di, f.ci) : -1;
-  f.sj = in ? slab_e(j, s.jb, s.je, s.N, f.dj, f.cj) : -1;
-  f.sk = in ? slab_e(k, s.kb, s.ke, s.N, f.dk, f.ck) : -1;
+  f.si = in ? slab_e(i, v.ib, v.ie, v.N, f.di, f.ci) : -1;
+  f.sj = in ? slab_e(j, v.jb, v.je, v.N, f.dj, f.cj) : -1;
+  f.sk = in ? slab_e(k, v.kb, v.ke, v.N, f.dk, f.ck) : -1;
   return f;
 }
 
-// Loads (indexes clamped into the grid; psi only inside its slab).
-template <typename T, int MASK>
-__device__ __forceinline__ E3In<T> e3_load(const Slot<T>& s, int i, int j, int k, const E3Where& f,
-                                           const T* const* A, T* const* psiA) {
-  E3In<T> a{};
-  const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+// Everything one cell's new E reads: old-set fields (indexes clamped into the
+// grid), psi only inside its slab, coefficients only where used.
+template <typename T>
+struct E3In {
+  T ex, ey, ez, hx, hy, hz, hzj, hyk, hxk, hzi, hyi, hxj;
+  T qxy, qxz, qyx, qyz, qzx, qzy;
+  T ikx, iky, ikz, bei, cei, bej, cej, bek, cek;
+  uint8_t m;
+};
+
+template <typename T>
+__device__ __forceinline__ E3In<T> e3_load(const View3<T>& v, int i, int j, int k, const E3Where& f,
+                                           const T* __restrict__ Ex, const T* __restrict__ Ey,
+                                           const T* __restrict__ Ez, const T* __restrict__ Hx,
+                                           const T* __restrict__ Hy, const T* __restrict__ Hz,
+                                           T* const* psiA) {
+  E3In<T> a;
+  const int nx = v.nx, ny = v.ny, nz = v.nz, N = v.N;
   const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
   const unsigned pl = (unsigned)ny * nz;
   const unsigned idx = (unsigned)ic * pl + (unsigned)jc * nz + kc;
   const unsigned oi = ic + 1 < nx ? pl : 0u, oj = jc + 1 < ny ? (unsigned)nz : 0u, ok = kc + 1 < nz ? 1u : 0u;
   const size_t iy = ((size_t)ic * N + f.dj) * nz + kc, iz = ((size_t)ic * ny + jc) * N + f.dk;
   const size_t ix = ((size_t)f.di * ny + jc) * nz + kc;
-  a.m = s.mat[idx];
-  if (MASK & 1) {
-    a.ex = A[F_EX][idx];
-    a.hzj = A[F_HZ][idx + oj];
-    a.hyk = A[F_HY][idx + ok];
-    if (f.bx && f.sj >= 0) a.qxy = psiA[psi_id(F_EX, 1, f.sj)][iy];
-    if (f.bx && f.sk >= 0) a.qxz = psiA[psi_id(F_EX, 2, f.sk)][iz];
-  }
-  if (MASK & 2) {
-    a.ey = A[F_EY][idx];
-    a.hxk = A[F_HX][idx + ok];
-    a.hzi = A[F_HZ][idx + oi];
-    if (f.by && f.si >= 0) a.qyx = psiA[psi_id(F_EY, 0, f.si)][ix];
-    if (f.by && f.sk >= 0) a.qyz = psiA[psi_id(F_EY, 2, f.sk)][iz];
-  }
-  if (MASK & 4) {
-    a.ez = A[F_EZ][idx];
-    a.hyi = A[F_HY][idx + oi];
-    a.hxj = A[F_HX][idx + oj];
-    if (f.bz && f.si >= 0) a.qzx = psiA[psi_id(F_EZ, 0, f.si)][ix];
-    if (f.bz && f.sj >= 0) a.qzy = psiA[psi_id(F_EZ, 1, f.sj)][iy];
-  }
-  if (MASK & 6) a.hx = A[F_HX][idx];
-  if (MASK & 5) a.hy = A[F_HY][idx];
-  if (MASK & 3) a.hz = A[F_HZ][idx];
+  a.ex = Ex[idx];
+  a.ey = Ey[idx];
+  a.ez = Ez[idx];
+  a.hx = Hx[idx];
+  a.hy = Hy[idx];
+  a.hz = Hz[idx];
+  a.hzj = Hz[idx + oj];
+  a.hyk = Hy[idx + ok];
+  a.hxk = Hx[idx + ok];
+  a.hzi = Hz[idx + oi];
+  a.hyi = Hy[idx + oi];
+  a.hxj = Hx[idx + oj];
+  a.m = v.mat[idx];
+  a.qxy = f.bx && f.sj >= 0 ? psiA[psi_id(F_EX, 1, f.sj)][iy] : T(0);
+  a.qxz = f.bx && f.sk >= 0 ? psiA[psi_id(F_EX, 2, f.sk)][iz] : T(0);
+  a.qyx = f.by && f.si >= 0 ? psiA[psi_id(F_EY, 0, f.si)][ix] : T(0);
+  a.qyz = f.by && f.sk >= 0 ? psiA[psi_id(F_EY, 2, f.sk)][iz] : T(0);
+  a.qzx = f.bz && f.si >= 0 ? psiA[psi_id(F_EZ, 0, f.si)][ix] : T(0);
+  a.qzy = f.bz && f.sj >= 0 ? psiA[psi_id(F_EZ, 1, f.sj)][iy] : T(0);
+  a.ikx = f.by || f.bz ? v.ike[0][ic] : T(0);
+  a.iky = f.bx || f.bz ? v.ike[1][jc] : T(0);
+  a.ikz = f.bx || f.by ? v.ike[2][kc] : T(0);
+  a.bei = f.si >= 0 ? v.be[f.ci] : T(0);
+  a.cei = f.si >= 0 ? v.ceo[f.ci] : T(0);
+  a.bej = f.sj >= 0 ? v.be[f.cj] : T(0);
+  a.cej = f.sj >= 0 ? v.ceo[f.cj] : T(0);
+  a.bek = f.sk >= 0 ? v.be[f.ck] : T(0);
+  a.cek = f.sk >= 0 ? v.ceo[f.ck] : T(0);
   return a;
 }
 
+template <typename T>
+struct E3Out {
+  T ex, ey, ez, pxy, pxz, pyx, pyz, pzx, pzy;
+};
+
 // The E update of one cell in the reference order: bulk curl, source, psi of
 // the first then the second transverse axis (fdtd_funcs.py:44-126, FDTD_3D).
-template <typename T, int MASK>
-__device__ __forceinline__ E3Out<T> e3_update(const Slot<T>& s, int i, int j, int k, int n, const E3Where& f,
-                                              const E3In<T>& a) {
+template <typename T>
+__device__ __forceinline__ E3Out<T> e3_update(const View3<T>& v, const E3Where& f, const E3In<T>& a, T jv) {
   E3Out<T> r;
   r.ex = a.ex;
   r.ey = a.ey;
   r.ez = a.ez;
-  r.hx = a.hx;
-  r.hy = a.hy;
-  r.hz = a.hz;
   r.pxy = r.pxz = r.pyx = r.pyz = r.pzx = r.pzy = T(0);
-  if ((MASK & 1) && f.bx) {
-    const T cb = cbsel(s, a.m, 0);
+  if (f.bx) {
+    const T cb = cbsel3(v, a.m, 0);
     const T dHz = a.hzj - a.hz;
     const T dHy = a.hyk - a.hy;
-    T v = a.ex + cb * (dHz * s.ike[1][j] - dHy * s.ike[2][k]);
-    if (f.sj >= 0) { r.pxy = s.be[f.cj] * a.qxy + s.ceo[f.cj] * dHz; v += cb * r.pxy; }
-    if (f.sk >= 0) { r.pxz = s.be[f.ck] * a.qxz + s.ceo[f.ck] * dHy; v -= cb * r.pxz; }
-    r.ex = v;
+    T e = a.ex + cb * (dHz * a.iky - dHy * a.ikz);
+    if (f.sj >= 0) { r.pxy = a.bej * a.qxy + a.cej * dHz; e += cb * r.pxy; }
+    if (f.sk >= 0) { r.pxz = a.bek * a.qxz + a.cek * dHy; e -= cb * r.pxz; }
+    r.ex = e;
   }
-  if (MASK & 2) {
-    if (f.by) {
-      const T cb = cbsel(s, a.m, 1);
-      const T dHx = a.hxk - a.hx;
-      const T dHz = a.hzi - a.hz;
-      T v = a.ey + cb * (dHx * s.ike[2][k] - dHz * s.ike[0][i]);
-      if (f.src) v -= s.jsrc[n];
-      if (f.si >= 0) { r.pyx = s.be[f.ci] * a.qyx + s.ceo[f.ci] * dHz; v -= cb * r.pyx; }
-      if (f.sk >= 0) { r.pyz = s.be[f.ck] * a.qyz + s.ceo[f.ck] * dHx; v += cb * r.pyz; }
-      r.ey = v;
-    } else if (f.src) {
-      r.ey = a.ey - s.jsrc[n];
-    }
+  if (f.by) {
+    const T cb = cbsel3(v, a.m, 1);
+    const T dHx = a.hxk - a.hx;
+    const T dHz = a.hzi - a.hz;
+    T e = a.ey + cb * (dHx * a.ikz - dHz * a.ikx);
+    if (f.src) e -= jv;
+    if (f.si >= 0) { r.pyx = a.bei * a.qyx + a.cei * dHz; e -= cb * r.pyx; }
+    if (f.sk >= 0) { r.pyz = a.bek * a.qyz + a.cek * dHx; e += cb * r.pyz; }
+    r.ey = e;
+  } else if (f.src) {
+    r.ey = a.ey - jv;
   }
-  if ((MASK & 4) && f.bz) {
-    const T cb = cbsel(s, a.m, 2);
+  if (f.bz) {
+    const T cb = cbsel3(v, a.m, 2);
     const T dHy = a.hyi - a.hy;
     const T dHx = a.hxj - a.hx;
-    T v = a.ez + cb * (dHy * s.ike[0][i] - dHx * s.ike[1][j]);
-    if (f.si >= 0) { r.pzx = s.be[f.ci] * a.qzx + s.ceo[f.ci] * dHy; v += cb * r.pzx; }
-    if (f.sj >= 0) { r.pzy = s.be[f.cj] * a.qzy + s.ceo[f.cj] * dHx; v -= cb * r.pzy; }
-    r.ez = v;
+    T e = a.ez + cb * (dHy * a.ikx - dHx * a.iky);
+    if (f.si >= 0) { r.pzx = a.bei * a.qzx + a.cei * dHy; e += cb * r.pzx; }
+    if (f.sj >= 0) { r.pzy = a.bej * a.qzy + a.cej * dHx; e -= cb * r.pzy; }
+    r.ez = e;
   }
   return r;
 }
 
-template <typename T, int XC>
+template <typename T, int XM>
 __global__ __launch_bounds__(kThreads) void k_3d_step(const Slot<T>* __restrict__ slots,
                                                       const int4* __restrict__ tiles,
                                                       const int* __restrict__ counts, int tstep) {
-  // sE?[u][q][p]: new E of plane i0 + u, row j0 - 1 + q, column k0 - 1 + p
-  __shared__ T sEx[XC][kBY + 1][kBX + 1];
-  __shared__ T sEy[XC][kBY + 1][kBX + 1];
-  __shared__ T sEz[XC][kBY + 1][kBX + 1];
-  constexpr int kColWave = XC >= kBY ? 0 : XC;   // the wave with the fewest row-halo planes
+  constexpr int R = kBY;                        // y-rows of a wave tile
+  constexpr int NITEM = XM + R * (XM + 1);      // halo row, then (plane halo + XM planes) per row
+  static_assert(R * XM <= kBX, "the k0-1 halo column takes one cell per lane");
+  __shared__ T rowbuf[kBY][2][XM][kBX];         // per wave: new Ex, Ez of the previous row, per plane
   const int ntiles = counts[0];
-  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+  const int lane = threadIdx.x;
+  const int w = wave_id();
+  const int stride = gridDim.x * kBY;
+  for (int tt = xcd_block() * kBY + w; tt < ntiles; tt += stride) {
     const int4 tl = tiles[tt];
     const Slot<T>& s = slots[tl.x];
     const int n = s.base + tstep;
     if (!s.active || n >= s.nt) continue;
-    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
-    const int lane = threadIdx.x, w = wave_id();
-    const int i0 = tl.y, j0 = tl.z, k0 = tl.w;
-    const int j = j0 + w, k = k0 + lane;
+    const View3<T> v = view3(s);
     const bool odd = (n & 1) != 0;
-    const T db = s.db;
-    const T* const* A = odd ? s.f2 : s.f;
-    T* const* B = odd ? s.f : s.f2;
+    T* const* FA = odd ? s.f2 : s.f;
+    T* const* FB = odd ? s.f : s.f2;
+    const T* __restrict__ Ex = FA[F_EX];
+    const T* __restrict__ Ey = FA[F_EY];
+    const T* __restrict__ Ez = FA[F_EZ];
+    const T* __restrict__ Hx = FA[F_HX];
+    const T* __restrict__ Hy = FA[F_HY];
+    const T* __restrict__ Hz = FA[F_HZ];
     T* const* psiA = odd ? s.psi2 : s.psi;
     T* const* psiB = odd ? s.psi : s.psi2;
-    // ---- loads of the owned cells, all issued before any update
-    E3Where fw[XC];
-    E3In<T> in[XC];
-#pragma unroll
-    for (int u = 0; u < XC; ++u) {
-      fw[u] = e3_where(s, i0 + u, j, k);
-      in[u] = e3_load<T, 7>(s, i0 + u, j, k, fw[u], A, psiA);
-    }
-    // ---- halos: plane i0-1 of this row (Ey, Ez), row j0-1 (Ex, Ez), column k0-1 (Ex, Ey)
-    E3Out<T> ph;
-    {
-      const E3Where f = e3_where(s, i0 - 1, j, k);
-      ph = e3_update<T, 6>(s, i0 - 1, j, k, n, f, e3_load<T, 6>(s, i0 - 1, j, k, f, A, psiA));
-    }
-    for (int u = w; u < XC; u += kBY) {
-      const E3Where f = e3_where(s, i0 + u, j0 - 1, k);
-      const E3Out<T> r = e3_update<T, 5>(s, i0 + u, j0 - 1, k, n, f, e3_load<T, 5>(s, i0 + u, j0 - 1, k, f, A, psiA));
-      sEx[u][0][1 + lane] = r.ex;
-      sEz[u][0][1 + lane] = r.ez;
-    }
-    if (w == kColWave && lane < kBY * XC) {
-      const int u = lane / kBY, q = lane % kBY;
-      const E3Where f = e3_where(s, i0 + u, j0 + q, k0 - 1);
-      const E3Out<T> r =
-          e3_update<T, 3>(s, i0 + u, j0 + q, k0 - 1, n, f, e3_load<T, 3>(s, i0 + u, j0 + q, k0 - 1, f, A, psiA));
-      sEx[u][1 + q][0] = r.ex;
-      sEy[u][1 + q][0] = r.ey;
-    }
-    // ---- E half-step of the owned cells
-    const bool jcap = j >= s.wy0 && j <= s.wy1;
-    const int cap = !jcap ? -1 : (k == s.port[0] ? 0 : (k == s.port[1] ? 1 : -1));
-    const bool cap2 = jcap && s.port[0] == s.port[1] && k == s.port[0];
+    const T jvE = v.src_field == F_EY ? s.jsrc[n] : T(0);
+    const T jvH = v.src_field == F_HY ? s.jsrc[n] : T(0);
+    const int nx = v.nx, ny = v.ny, nz = v.nz, N = v.N;
+    const int i0 = tl.y, j0 = tl.z, k0 = tl.w;
+    const int k = k0 + lane;
+    const bool kok = k < nz;
     const unsigned pl = (unsigned)ny * nz;
-    E3Out<T> e[XC];
-#pragma unroll
-    for (int u = 0; u < XC; ++u) {
-      const int i = i0 + u;
-      const E3Where& f = fw[u];
-      e[u] = e3_update<T, 7>(s, i, j, k, n, f, in[u]);
-      sEx[u][1 + w][1 + lane] = e[u].ex;
-      sEy[u][1 + w][1 + lane] = e[u].ey;
-      sEz[u][1 + w][1 + lane] = e[u].ez;
-      const unsigned idx = (unsigned)i * pl + (unsigned)j * nz + k;
-      if (f.bx) {
-        B[F_EX][idx] = e[u].ex;
-        if (f.sj >= 0) psiB[psi_id(F_EX, 1, f.sj)][((size_t)i * N + f.dj) * nz + k] = e[u].pxy;
-        if (f.sk >= 0) psiB[psi_id(F_EX, 2, f.sk)][((size_t)i * ny + j) * N + f.dk] = e[u].pxz;
-      }
-      if (f.by) {
-        B[F_EY][idx] = e[u].ey;
-        if (f.si >= 0) psiB[psi_id(F_EY, 0, f.si)][((size_t)f.di * ny + j) * nz + k] = e[u].pyx;
-        if (f.sk >= 0) psiB[psi_id(F_EY, 2, f.sk)][((size_t)i * ny + j) * N + f.dk] = e[u].pyz;
-      } else if (f.src) {
-        B[F_EY][idx] = e[u].ey;
-      }
-      if (f.bz) {
-        B[F_EZ][idx] = e[u].ez;
-        if (f.si >= 0) psiB[psi_id(F_EZ, 0, f.si)][((size_t)f.di * ny + j) * nz + k] = e[u].pzx;
-        if (f.sj >= 0) psiB[psi_id(F_EZ, 1, f.sj)][((size_t)i * N + f.dj) * nz + k] = e[u].pzy;
-      }
-      if (cap >= 0 && i < nx && i >= s.wx0 && i <= s.wx1) {
-        const int c = (i - s.wx0) * s.prow + (j - s.wy0);
-        capture(s, tstep, cap * 6 + 0, c, e[u].ex);
-        capture(s, tstep, cap * 6 + 1, c, e[u].ey);
-        if (cap2) {
-          capture(s, tstep, 6, c, e[u].ex);
-          capture(s, tstep, 7, c, e[u].ey);
-        }
-      }
+    // z-only constants of the H update (fdtd_funcs.py:128-266, FDTD_3D)
+    const bool kin = kok && k >= v.kb && k < v.ke, kin1 = kok && k > v.kb && k < v.ke;
+    int dkh = 0, ckh = 0;
+    const int skh = kin1 ? slab_h(k, v.kb, v.ke, N, dkh, ckh) : -1;
+    const T ikzh = kin1 ? v.ikh[2][k] : T(0);
+    const T bhk = skh >= 0 ? v.bh[ckh] : T(0), chk = skh >= 0 ? v.cho[ckh] : T(0);
+    const bool hksrc = kok && v.src_field == F_HY && k >= v.sz0 && k < v.sz1;
+    const int port0 = s.port[0], port1 = s.port[1];
+    const int wx0 = s.wx0, wx1 = s.wx1, wy0 = s.wy0, wy1 = s.wy1, prow = s.prow;
+    const bool kcapE = kok && (k == port0 || k == port1);
+    const bool kcapH = kok && (k == port0 || k == port0 + 1 || k == port1 || k == port1 + 1);
+    // ---- column k0-1 (halo, not stored): lane l holds cell (i0 + l / R, j0 + l % R, k0 - 1)
+    T colEx = T(0), colEy = T(0);
+    if (lane < R * XM) {
+      const int ih = i0 + lane / R, jh = j0 + lane % R;
+      const E3Where f = e3_where(v, ih, jh, k0 - 1);
+      const E3Out<T> o = e3_update(v, f, e3_load(v, ih, jh, k0 - 1, f, Ex, Ey, Ez, Hx, Hy, Hz, psiA), jvE);
+      colEx = o.ex;
+      colEy = o.ey;
     }
-    __syncthreads();
-    // ---- H half-step of the owned cells (index ranges fdtd_funcs.py:84-126, FDTD_3D)
-    const bool jok = j < ny, kok = k < nz;
-    const bool jin = jok && j >= s.jb && j < s.je, jin1 = jok && j > s.jb && j < s.je;
-    const bool kin = kok && k >= s.kb && k < s.ke, kin1 = kok && k > s.kb && k < s.ke;
-    int dj = 0, cj = 0, dk = 0, ck = 0;
-    const int sj = jin1 ? slab_h(j, s.jb, s.je, N, dj, cj) : -1;
-    const int sk = kin1 ? slab_h(k, s.kb, s.ke, N, dk, ck) : -1;
-    const T iky = jin1 ? s.ikh[1][j] : T(0);
-    const T ikz = kin1 ? s.ikh[2][k] : T(0);
-    const T bhj = sj >= 0 ? s.bh[cj] : T(0), chj = sj >= 0 ? s.cho[cj] : T(0);
-    const T bhk = sk >= 0 ? s.bh[ck] : T(0), chk = sk >= 0 ? s.cho[ck] : T(0);
-    const bool jksrc = s.src_field == F_HY && jok && kok && j >= s.sy0 && j < s.sy1 && k >= s.sz0 && k < s.sz1;
-    const T jv = jksrc ? s.jsrc[n] : T(0);
-    int capA = -1, capB = -1, capC = -1, capD = -1;
-    if (jcap && kok) {
-      if (k == s.port[0]) capA = 0;
-      if (k == s.port[0] + 1) capB = 0;
-      if (k == s.port[1]) capC = 1;
-      if (k == s.port[1] + 1) capD = 1;
+    // item t -> cell: t < XM: (i0 + t, j0 - 1), row below the tile;
+    // otherwise row r = (t - XM) / (XM + 1), position q: q = 0 the plane
+    // i0 - 1 of the row, q >= 1 the owned plane i0 + q - 1.
+    auto cell_of = [&](int t, int& i, int& j) {
+      if (t < XM) {
+        i = i0 + t;
+        j = j0 - 1;
+      } else {
+        const int r = (t - XM) / (XM + 1), q = (t - XM) - r * (XM + 1);
+        i = i0 - 1 + q;
+        j = j0 + r;
+      }
+    };
+    int ic, jc;
+    cell_of(0, ic, jc);
+    E3Where fc = e3_where(v, ic, jc, k);
+    E3In<T> ac = e3_load(v, ic, jc, k, fc, Ex, Ey, Ez, Hx, Hy, Hz, psiA);
+    T pey = T(0), pez = T(0);   // new Ey, Ez of the previous item of the row (i-1)
+#pragma unroll 1
+    for (int t = 0; t < NITEM; ++t) {
+      int i, j;
+      cell_of(t, i, j);
+      // prefetch the next item before any store of this one
+      E3Where fn = fc;
+      E3In<T> an = ac;
+      if (t + 1 < NITEM) {
+        int i2, j2;
+        cell_of(t + 1, i2, j2);
+        fn = e3_where(v, i2, j2, k);
+        an = e3_load(v, i2, j2, k, fn, Ex, Ey, Ez, Hx, Hy, Hz, psiA);
+      }
+      const E3Out<T> e = e3_update(v, fc, ac, jvE);
+      if (t < XM) {
+        const int u = t;
+        rowbuf[w][0][u][lane] = e.ex;
+        rowbuf[w][1][u][lane] = e.ez;
+      } else {
+        const int r = (t - XM) / (XM + 1), q = (t - XM) - r * (XM + 1);
+        if (q > 0 && i < nx && j < ny) {
+          const int u = q - 1;
+          const unsigned idx = (unsigned)i * pl + (unsigned)j * nz + min(k, nz - 1);
+          // ---- E stores of (i, j, k)
+          if (fc.bx) {
+            FB[F_EX][idx] = e.ex;
+            if (fc.sj >= 0) psiB[psi_id(F_EX, 1, fc.sj)][((size_t)i * N + fc.dj) * nz + k] = e.pxy;
+            if (fc.sk >= 0) psiB[psi_id(F_EX, 2, fc.sk)][((size_t)i * ny + j) * N + fc.dk] = e.pxz;
+          }
+          if (fc.by) {
+            FB[F_EY][idx] = e.ey;
+            if (fc.si >= 0) psiB[psi_id(F_EY, 0, fc.si)][((size_t)fc.di * ny + j) * nz + k] = e.pyx;
+            if (fc.sk >= 0) psiB[psi_id(F_EY, 2, fc.sk)][((size_t)i * ny + j) * N + fc.dk] = e.pyz;
+          } else if (fc.src) {
+            FB[F_EY][idx] = e.ey;
+          }
+          if (fc.bz) {
+            FB[F_EZ][idx] = e.ez;
+            if (fc.si >= 0) psiB[psi_id(F_EZ, 0, fc.si)][((size_t)fc.di * ny + j) * nz + k] = e.pzx;
+            if (fc.sj >= 0) psiB[psi_id(F_EZ, 1, fc.sj)][((size_t)i * N + fc.dj) * nz + k] = e.pzy;
+          }
+          const bool icap = i >= wx0 && i <= wx1, jcap = j >= wy0 && j <= wy1;
+          if (kcapE && jcap && icap) {
+            const int c = (i - wx0) * prow + (j - wy0);
+            const int cap = k == port0 ? 0 : 1;
+            capture(s, tstep, cap * 6 + 0, c, e.ex);
+            capture(s, tstep, cap * 6 + 1, c, e.ey);
+            if (port0 == port1) {
+              capture(s, tstep, 6, c, e.ex);
+              capture(s, tstep, 7, c, e.ey);
+            }
+          }
+          // ---- H update of (i, j, k): new E of the i-1, j-1 and k-1 neighbours
+          T exk = __shfl_up(e.ex, 1, kBX), eyk = __shfl_up(e.ey, 1, kBX);
+          const T cex = __shfl(colEx, u * R + r, kBX), cey = __shfl(colEy, u * R + r, kBX);
+          if (lane == 0) {
+            exk = cex;
+            eyk = cey;
+          }
+          const T exj = rowbuf[w][0][u][lane], ezj = rowbuf[w][1][u][lane];
+          const T eyi = pey, ezi = pez;
+          if (kok) {
+            const T db = v.db;
+            const bool iin = i >= v.ib && i < v.ie, iin1 = i > v.ib && i < v.ie;
+            const bool jin = j >= v.jb && j < v.je, jin1 = j > v.jb && j < v.je;
+            int dih = 0, cih = 0, djh = 0, cjh = 0;
+            const int sih = slab_h(i, v.ib, v.ie, N, dih, cih);
+            const int sjh = jin1 ? slab_h(j, v.jb, v.je, N, djh, cjh) : -1;
+            const T ikxh = v.ikh[0][i];
+            const T ikyh = jin1 ? v.ikh[1][j] : T(0);
+            const bool bx = iin && jin1 && kin1;
+            const bool by = iin1 && jin && kin1;
+            const bool bz = iin1 && jin1 && kin;
+            const bool src = hksrc && i >= v.sx0 && i < v.sx1 && j >= v.sy0 && j < v.sy1;
+            const size_t iy = ((size_t)i * N + djh) * nz + k, iz = ((size_t)i * ny + j) * N + dkh;
+            const size_t ix = ((size_t)dih * ny + j) * nz + k;
+            T vx = ac.hx;
+            if (bx) {
+              const T dEy = e.ey - eyk;
+              const T dEz = e.ez - ezj;
+              vx = vx + db * (dEy * ikzh - dEz * ikyh);
+              if (sjh >= 0) {
+                const T p = v.bh[cjh] * psiA[psi_id(F_HX, 1, sjh)][iy] + v.cho[cjh] * dEz;
+                psiB[psi_id(F_HX, 1, sjh)][iy] = p;
+                vx -= db * p;
+              }
+              if (skh >= 0) {
+                const T p = bhk * psiA[psi_id(F_HX, 2, skh)][iz] + chk * dEy;
+                psiB[psi_id(F_HX, 2, skh)][iz] = p;
+                vx += db * p;
+              }
+              FB[F_HX][idx] = vx;
+            }
+            T vy = ac.hy;
+            if (src) vy -= jvH;
+            if (by) {
+              const T dEz = e.ez - ezi;
+              const T dEx = e.ex - exk;
+              vy = vy + db * (dEz * ikxh - dEx * ikzh);
+              if (sih >= 0) {
+                const T p = v.bh[cih] * psiA[psi_id(F_HY, 0, sih)][ix] + v.cho[cih] * dEz;
+                psiB[psi_id(F_HY, 0, sih)][ix] = p;
+                vy += db * p;
+              }
+              if (skh >= 0) {
+                const T p = bhk * psiA[psi_id(F_HY, 2, skh)][iz] + chk * dEx;
+                psiB[psi_id(F_HY, 2, skh)][iz] = p;
+                vy -= db * p;
+              }
+            }
+            if (by || src) FB[F_HY][idx] = vy;
+            if (bz) {
+              const T dEx = e.ex - exj;
+              const T dEy = e.ey - eyi;
+              T vz = ac.hz + db * (dEx * ikyh - dEy * ikxh);
+              if (sih >= 0) {
+                const T p = v.bh[cih] * psiA[psi_id(F_HZ, 0, sih)][ix] + v.cho[cih] * dEy;
+                psiB[psi_id(F_HZ, 0, sih)][ix] = p;
+                vz -= db * p;
+              }
+              if (sjh >= 0) {
+                const T p = v.bh[cjh] * psiA[psi_id(F_HZ, 1, sjh)][iy] + v.cho[cjh] * dEx;
+                psiB[psi_id(F_HZ, 1, sjh)][iy] = p;
+                vz += db * p;
+              }
+              FB[F_HZ][idx] = vz;
+            }
+            if (kcapH && jcap && icap) {
+              const int c = (i - wx0) * prow + (j - wy0);
+              if (k == port0) { capture(s, tstep, 2, c, vx); capture(s, tstep, 4, c, vy); }
+              if (k == port0 + 1) { capture(s, tstep, 3, c, vx); capture(s, tstep, 5, c, vy); }
+              if (k == port1) { capture(s, tstep, 8, c, vx); capture(s, tstep, 10, c, vy); }
+              if (k == port1 + 1) { capture(s, tstep, 9, c, vx); capture(s, tstep, 11, c, vy); }
+            }
+          }
+          rowbuf[w][0][u][lane] = e.ex;
+          rowbuf[w][1][u][lane] = e.ez;
+        }
+        pey = e.ey;
+        pez = e.ez;
+      }
+      fc = fn;
+      ac = an;
     }
-    const bool anycap = capA >= 0 || capB >= 0 || capC >= 0 || capD >= 0;
-#pragma unroll
-    for (int u = 0; u < XC; ++u) {
-      const int i = i0 + u;
-      if (!jok || !kok || i >= nx) continue;
-      const bool iin = i >= s.ib && i < s.ie, iin1 = i > s.ib && i < s.ie;
-      const bool bx = iin && jin1 && kin1;
-      const bool by = iin1 && jin && kin1;
-      const bool bz = iin1 && jin1 && kin;
-      const bool src = jksrc && i >= s.sx0 && i < s.sx1;
-      const bool icap = anycap && i >= s.wx0 && i <= s.wx1;
-      if (!(bx || by || bz || src || icap)) continue;
-      int di = 0, ci = 0;
-      const int si = slab_h(i, s.ib, s.ie, N, di, ci);
-      const size_t iy = ((size_t)i * N + dj) * nz + k, iz = ((size_t)i * ny + j) * N + dk;
-      const size_t ix = ((size_t)di * ny + j) * nz + k;
-      const unsigned idx = (unsigned)i * pl + (unsigned)j * nz + k;
-      const T ex = e[u].ex, ey = e[u].ey, ez = e[u].ez;
-      const T exk = sEx[u][1 + w][lane], eyk = sEy[u][1 + w][lane];     // k-1
-      const T exj = sEx[u][w][1 + lane], ezj = sEz[u][w][1 + lane];     // j-1
-      const T eyi = u ? e[u - 1].ey : ph.ey, ezi = u ? e[u - 1].ez : ph.ez;   // i-1
-      T vx = e[u].hx;
-      if (bx) {
-        const T dEy = ey - eyk;
-        const T dEz = ez - ezj;
-        vx = vx + db * (dEy * ikz - dEz * iky);
-        if (sj >= 0) {
-          const T p = bhj * psiA[psi_id(F_HX, 1, sj)][iy] + chj * dEz;
-          psiB[psi_id(F_HX, 1, sj)][iy] = p;
-          vx -= db * p;
-        }
-        if (sk >= 0) {
-          const T p = bhk * psiA[psi_id(F_HX, 2, sk)][iz] + chk * dEy;
-          psiB[psi_id(F_HX, 2, sk)][iz] = p;
-          vx += db * p;
-        }
-        B[F_HX][idx] = vx;
-      }
-      T vy = e[u].hy;
-      if (src) vy -= jv;
-      if (by) {
-        const T dEz = ez - ezi;
-        const T dEx = ex - exk;
-        vy = vy + db * (dEz * s.ikh[0][i] - dEx * ikz);
-        if (si >= 0) {
-          const T p = s.bh[ci] * psiA[psi_id(F_HY, 0, si)][ix] + s.cho[ci] * dEz;
-          psiB[psi_id(F_HY, 0, si)][ix] = p;
-          vy += db * p;
-        }
-        if (sk >= 0) {
-          const T p = bhk * psiA[psi_id(F_HY, 2, sk)][iz] + chk * dEx;
-          psiB[psi_id(F_HY, 2, sk)][iz] = p;
-          vy -= db * p;
-        }
-      }
-      if (by || src) B[F_HY][idx] = vy;
-      if (bz) {
-        const T dEx = ex - exj;
-        const T dEy = ey - eyi;
-        T v = e[u].hz + db * (dEx * iky - dEy * s.ikh[0][i]);
-        if (si >= 0) {
-          const T p = s.bh[ci] * psiA[psi_id(F_HZ, 0, si)][ix] + s.cho[ci] * dEy;
-          psiB[psi_id(F_HZ, 0, si)][ix] = p;
-          v -= db * p;
-        }
-        if (sj >= 0) {
-          const T p = bhj * psiA[psi_id(F_HZ, 1, sj)][iy] + chj * dEx;
-          psiB[psi_id(F_HZ, 1, sj)][iy] = p;
-          v += db * p;
-        }
-        B[F_HZ][idx] = v;
-      }
-      if (icap) {
-        const int c = (i - s.wx0) * s.prow + (j - s.wy0);
-        if (capA >= 0) { capture(s, tstep, 2, c, vx); capture(s, tstep, 4, c, vy); }
-        if (capB >= 0) { capture(s, tstep, 3, c, vx); capture(s, tstep, 5, c, vy); }
-        if (capC >= 0) { capture(s, tstep, 8, c, vx); capture(s, tstep, 10, c, vy); }
-        if (capD >= 0) { capture(s, tstep, 9, c, vx); capture(s, tstep, 11, c, vy); }
-      }
-    }
-    __syncthreads();   // the LDS planes are reused by the next tile
   }
 }
 

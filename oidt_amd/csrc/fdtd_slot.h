@@ -20,7 +20,7 @@ constexpr int kFB = 16;
 // Distinct cb values per E component (2-bit index per component in the
 // material byte).
 constexpr int kMatMax = 4;
-constexpr int kFused3dMaxXc = 8;   // x-planes per block of the fused 3D step (LDS size)
+constexpr int kFused3dMaxXc = 16;  // x-planes of a fused 3D step tile (4 rows x 16 planes: one halo cell per lane)
 
 enum : int { SCHEME_TMZ = 0, SCHEME_TEZ = 1, SCHEME_3D = 2 };
 enum : int { F_EX = 0, F_EY = 1, F_EZ = 2, F_HX = 3, F_HY = 4, F_HZ = 5 };

@@ -10,7 +10,7 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step (TMz and 3D)
-FUSED_TM = [(3, 16), (3, 32)]                      # longer marches of the fused TMz step
+FUSED_LONG = {'tm': [(3, 16), (3, 32)], '3d': [(3, 16)], 'te': []}   # longer marches of the fused steps
 
 
 @pytest.mark.gpu
@@ -23,7 +23,7 @@ def test_all_launch_forms_match_oracle(scheme):
     ref_f = o.fields()
     od = o.dft()
     w = window(scheme, o.c)
-    for variant, xc in FORMS + (FUSED if scheme != 'te' else []) + (FUSED_TM if scheme == 'tm' else []):
+    for variant, xc in FORMS + (FUSED if scheme != 'te' else []) + FUSED_LONG[scheme]:
         print('case', scheme, 'variant', variant, 'xchunk', xc, flush=True)
         b = Batch(scheme, np.float32, max_slots=1)
         b.set_launch(xc, 0, variant)
@@ -44,6 +44,7 @@ def test_all_launch_forms_match_oracle(scheme):
 @pytest.mark.parametrize('scheme,xc,direction,extra', [
     ('tm', 1, 'b', {}), ('tm', 4, 'f', {'nt': 101}),
     ('3d', 1, 'f', {}), ('3d', 2, 'b', {}), ('3d', 4, 'f', {'nt': 101}), ('3d', 8, 'f', {'source_condition': 'TM'}),
+    ('3d', 16, 'b', {'source_condition': 'TM'}),
 ])
 def test_fused_step_fp64(scheme, xc, direction, extra):
     """The fused step in FP64 equals the oracle to 1e-10 (same precision, only
