@@ -158,6 +158,25 @@ int validate(const oidt_run_desc* d) {
   return OIDT_OK;
 }
 
+// The device layout pads the unit-stride axis (y in 2D, z in 3D) to a
+// multiple of 128 bytes, so every 64-lane row segment of every plane starts
+// on a cache line (rows of 240 FP32 cells otherwise straddle lines and the
+// kernels fetch and write partial lines).  Kernels index with the padded
+// extent; every update range, CPML slab, source box, port and DFT window is
+// taken from the real grid, so padding cells are never updated, and no
+// updated cell reads one (the last updated cell along the axis is offset
+// cells inside the real grid).
+template <typename T>
+oidt_run_desc padded(const oidt_run_desc& d) {
+  oidt_run_desc p = d;
+  constexpr int q = 128 / (int)sizeof(T);
+  if (d.scheme == SCHEME_3D)
+    p.nz = (d.nz + q - 1) / q * q;
+  else
+    p.ny = (d.ny + q - 1) / q * q;
+  return p;
+}
+
 template <typename T>
 struct Layout {
   size_t off_f[6], off_psi[36], off_f2[6], off_psi2[36], off_mat, off_ike[3], off_ikh[3], off_be, off_ceo, off_bh, off_cho,
@@ -169,7 +188,8 @@ template <typename T>
 Layout<T> layout(const oidt_run_desc& d) {
   Layout<T> L{};
   const bool is3d = d.scheme == SCHEME_3D;
-  const size_t cells = (size_t)d.nx * d.ny * d.nz;
+  const oidt_run_desc pd = padded<T>(d);   // allocation extents
+  const size_t cells = (size_t)pd.nx * pd.ny * pd.nz;
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -184,12 +204,12 @@ Layout<T> layout(const oidt_run_desc& d) {
   for (int i = 0; i < 36; ++i) L.off_psi[i] = L.off_psi2[i] = (size_t)-1;
   for (auto pr : psi_pairs(d.scheme))
     for (int side = 0; side < 2; ++side) {
-      L.off_psi[psi_id(pr.first, pr.second, side)] = take(psi_elems(d, pr.second) * sizeof(T));
-      if (two) L.off_psi2[psi_id(pr.first, pr.second, side)] = take(psi_elems(d, pr.second) * sizeof(T));
+      L.off_psi[psi_id(pr.first, pr.second, side)] = take(psi_elems(pd, pr.second) * sizeof(T));
+      if (two) L.off_psi2[psi_id(pr.first, pr.second, side)] = take(psi_elems(pd, pr.second) * sizeof(T));
     }
   L.off_mat = take(cells);
   for (int a = 0; a < 3; ++a) {
-    const int n = a == 0 ? d.nx : (a == 1 ? d.ny : d.nz);
+    const int n = a == 0 ? pd.nx : (a == 1 ? pd.ny : pd.nz);
     L.off_ike[a] = take(n * sizeof(T));
     L.off_ikh[a] = take(n * sizeof(T));
   }
@@ -282,7 +302,11 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
 
   // Material compression: per E component, the distinct cb values (at most
   // kMatMax) and a 2-bit index per cell, packed as bits 2c..2c+1 of one byte.
-  std::vector<uint8_t> mat(cells, 0);
+  const oidt_run_desc pd = padded<T>(d);
+  const size_t row = d.scheme == SCHEME_3D ? (size_t)d.nz : (size_t)d.ny;            // real cells per row
+  const size_t prow = d.scheme == SCHEME_3D ? (size_t)pd.nz : (size_t)pd.ny;         // padded
+  const size_t pcells = (size_t)pd.nx * pd.ny * pd.nz;
+  std::vector<uint8_t> mat(pcells, 0);
   T cbv[3][kMatMax] = {};
   for (int c = 0; c < ncb; ++c) {
     const int comp = cb_comp[d.scheme][c];
@@ -303,7 +327,7 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
         id = nv++;
       }
       last = id;
-      mat[e] |= (uint8_t)(id << (2 * comp));
+      mat[e / row * prow + e % row] |= (uint8_t)(id << (2 * comp));
     }
     for (int q = 0; q < kMatMax; ++q) cbv[comp][q] = vals[q];
   }
@@ -343,7 +367,7 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   // Stage the constant arrays in pinned memory and upload.
   struct Piece { size_t off; const void* src; size_t bytes; };
   std::vector<Piece> pieces = {
-      {L.off_mat, mat.data(), cells},
+      {L.off_mat, mat.data(), pcells},
       {L.off_be, be.data(), N * sizeof(T)},
       {L.off_ceo, ceo.data(), N * sizeof(T)},
       {L.off_bh, bh.data(), N * sizeof(T)},
@@ -370,7 +394,7 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
 
   // Descriptor.
   Slot<T> s{};
-  s.nx = d.nx; s.ny = d.ny; s.nz = d.nz; s.N = N;
+  s.nx = pd.nx; s.ny = pd.ny; s.nz = pd.nz; s.N = N;   // strides: padded extents
   s.ib = s.jb = d.offset; s.kb = d.scheme == SCHEME_3D ? d.offset : 0;
   s.ie = d.nx - d.offset; s.je = d.ny - d.offset; s.ke = d.scheme == SCHEME_3D ? d.nz - d.offset : 1;
   s.nt = d.nt; s.nf = d.nf; s.base = 0; s.active = 1; s.scheme = d.scheme;
@@ -922,7 +946,12 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
   const bool in_b = sh.fused && (steps & 1);
   const size_t off = d.precision == OIDT_F64 ? (in_b ? layout<double>(d).off_f2[field] : layout<double>(d).off_f[field])
                                              : (in_b ? layout<float>(d).off_f2[field] : layout<float>(d).off_f[field]);
-  HIP_TRY(hipMemcpy(host, static_cast<char*>(sh.dmem) + off, bytes, hipMemcpyDeviceToHost));
+  const bool is3d = d.scheme == SCHEME_3D;
+  const size_t row = (is3d ? (size_t)d.nz : (size_t)d.ny) * es;
+  const size_t prow = (is3d ? (size_t)(d.precision == OIDT_F64 ? padded<double>(d).nz : padded<float>(d).nz)
+                            : (size_t)(d.precision == OIDT_F64 ? padded<double>(d).ny : padded<float>(d).ny)) * es;
+  const size_t rows = bytes / row;
+  HIP_TRY(hipMemcpy2D(host, row, static_cast<char*>(sh.dmem) + off, prow, row, rows, hipMemcpyDeviceToHost));
   return OIDT_OK;
 }
 

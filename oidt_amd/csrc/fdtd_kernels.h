@@ -30,6 +30,27 @@ namespace oidt {
 // branches between the loads), so they read threadIdx.y directly.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
 
+// Cross-lane moves that stay in registers (no LDS, so no wait on the
+// counter that scalar memory loads also use): the value of lane l-1 (DPP
+// wave shift right by one; lane 0 gets `lo`) and the value of a uniform lane.
+__device__ __forceinline__ float lane_up1(float v, float lo) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lo), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_up1(double v, double lo) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(lo);
+  const int blo = __builtin_amdgcn_update_dpp((int)o, (int)b, 0x138, 0xf, 0xf, false);
+  const int bhi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)bhi << 32) | (unsigned)blo);
+}
+__device__ __forceinline__ float lane_get(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_get(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that the blocks
 // of one XCD take a contiguous range of tiles (shared L2 for x-neighbours).
 __device__ __forceinline__ int xcd_block() {
@@ -1302,13 +1323,12 @@ __device__ __forceinline__ T tmz_ez_cell(const TmzView<T>& v, int i, int j, T jv
   return tmz_e(v, f, tmz_load(v, i, j, f, EzA, HxA, HyA, psiA, true), tmz_jcoef(v, f, j), jv).ez;
 }
 
-// What the march loads for one plane x = i.  The x-dependent parts are the
-// same for the whole wave (scalar registers); the per-lane part is the nine
-// values each cell reads.
+// What the march loads for one plane x = i: the nine values each cell reads.
+// The plane's x-only coefficients are loaded once per tile, one plane per
+// lane, and broadcast with readlane when the plane is updated.
 template <typename T>
 struct TmzPlane {
-  T ikx, bex, cex, ikhx, bhx, chx;            // x-only coefficients (wave-uniform)
-  T ez, hx, hxj, hyi, pzx, pzy, qxy, qyx;     // per lane
+  T ez, hx, hxj, hyi, pzx, pzy, qxy, qyx;
   uint8_t m;
 };
 
@@ -1384,12 +1404,6 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
       TmzPlane<T> p;
       const TmzX x = tmz_x(v, i);
       const unsigned idx = (unsigned)i * ny + jc;
-      p.ikx = x.ibulk ? v.ike0[i] : T(0);
-      p.bex = x.sxe >= 0 ? v.be[x.cxe] : T(0);
-      p.cex = x.sxe >= 0 ? v.ceo[x.cxe] : T(0);
-      p.ikhx = v.ikh0[i];
-      p.bhx = x.sxh >= 0 ? v.bh[x.cxh] : T(0);
-      p.chx = x.sxh >= 0 ? v.cho[x.cxh] : T(0);
       p.ez = EzA[idx];
       p.hx = HxA[idx];
       p.hxj = HxA[jc + 1 < ny ? idx + 1 : idx];
@@ -1402,6 +1416,17 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
       p.qyx = jbulk && x.sxh >= 0 ? psiAll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + jc] : T(0);
       return p;
     };
+    // ---- x-only coefficients of the tile's planes: lane u holds plane i0+u
+    T lk_ikx = T(0), lk_bex = T(0), lk_cex = T(0), lk_ikhx = T(0), lk_bhx = T(0), lk_chx = T(0);
+    if (lane < XM && i0 + lane < nx) {
+      const TmzX x = tmz_x(v, i0 + lane);
+      lk_ikx = x.ibulk ? v.ike0[i0 + lane] : T(0);
+      lk_bex = x.sxe >= 0 ? v.be[x.cxe] : T(0);
+      lk_cex = x.sxe >= 0 ? v.ceo[x.cxe] : T(0);
+      lk_ikhx = v.ikh0[i0 + lane];
+      lk_bhx = x.sxh >= 0 ? v.bh[x.cxh] : T(0);
+      lk_chx = x.sxh >= 0 ? v.cho[x.cxh] : T(0);
+    }
     // ---- halos from set A, not stored: row j0-1 (lane u holds plane i0+u) and plane i0-1
     const T colE = lane < XM ? tmz_ez_cell(v, i0 + lane, j0 - 1, jv, EzA, HxA, HyA, psiAll) : T(0);
     T prevE = tmz_ez_cell(v, i0 - 1, j, jv, EzA, HxA, HyA, psiAll);
@@ -1435,9 +1460,9 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
       a.hyi = p0.hyi;
       a.pzx = p0.pzx;
       a.pzy = p0.pzy;
-      a.ikx = p0.ikx;
-      a.bex = p0.bex;
-      a.cex = p0.cex;
+      a.ikx = lane_get(lk_ikx, u);
+      a.bex = lane_get(lk_bex, u);
+      a.cex = lane_get(lk_cex, u);
       a.m = p0.m;
       const TmzE<T> e = tmz_e(v, f, a, jcf, jv);
       const unsigned idx = (unsigned)i * ny + jc;
@@ -1454,9 +1479,7 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
         if (i == v.port1) v.ring[((size_t)tstep * v.nplanes + 3) * v.pcells + c] = e.ez;
       }
       // H half-step of (i, j): Ez(j-1) from the lower lane, lane 0 from the halo row
-      T ezj = __shfl_up(e.ez, 1, kBX);
-      const T ecol = __shfl(colE, u, kBX);
-      if (lane == 0) ezj = ecol;
+      const T ezj = lane_up1(e.ez, lane_get(colE, u));
       if (jok) {
         const T db = v.db;
         if (x.ibulk && jin1) {
@@ -1472,9 +1495,9 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
         T hv = hyC;
         if (x.hbulk && jbulk) {
           const T dEz = e.ez - prevE;
-          hv = hv + db * dEz * p0.ikhx;
+          hv = hv + db * dEz * lane_get(lk_ikhx, u);
           if (x.sxh >= 0) {
-            const T q = p0.bhx * p0.qyx + p0.chx * dEz;
+            const T q = lane_get(lk_bhx, u) * p0.qyx + lane_get(lk_chx, u) * dEz;
             psiBll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + j] = q;
             hv += db * q;
           }
