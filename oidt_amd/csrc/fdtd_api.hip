@@ -502,15 +502,6 @@ void launch_half(oidt_batch* b, int gs, int tstep, bool e_half) {
   const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
   const dim3 blk(kBX, kBY), grd(gs);
   hipStream_t st = b->stream;
-  if (b->variant == 3) {   // fused step: the whole step in the "E" launch
-    if (!e_half) return;
-    if (b->scheme == SCHEME_TMZ) {
-      hipLaunchKernelGGL((k_tmz_step<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
-    } else if constexpr (XC <= kFused3dMaxXc) {
-      hipLaunchKernelGGL((k_3d_step<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
-    }
-    return;
-  }
   const bool v1 = b->variant == 1;
   switch (b->scheme) {
     case SCHEME_TMZ:
@@ -543,8 +534,36 @@ void launch_half(oidt_batch* b, int gs, int tstep, bool e_half) {
   }
 }
 
+// The fused step: the whole time step in the "E" launch; its tile depth
+// (planes per wave) has more choices than the half-step forms' x-chunk.
+template <typename T, int XM>
+void launch_fused(oidt_batch* b, int gs, int tstep) {
+  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
+  const dim3 blk(kBX, kBY), grd(gs);
+  if (b->scheme == SCHEME_TMZ) {
+    hipLaunchKernelGGL((k_tmz_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+  } else if constexpr (XM <= kFused3dMaxXc) {
+    hipLaunchKernelGGL((k_3d_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+  }
+}
+
 template <typename T>
 void launch_stencil(oidt_batch* b, int gs, int tstep, bool e_half) {
+  if (b->variant == 3) {
+    if (!e_half) return;
+    switch (b->xchunk) {
+      case 1: launch_fused<T, 1>(b, gs, tstep); break;
+      case 2: launch_fused<T, 2>(b, gs, tstep); break;
+      case 4: launch_fused<T, 4>(b, gs, tstep); break;
+      case 8: launch_fused<T, 8>(b, gs, tstep); break;
+      case 12: launch_fused<T, 12>(b, gs, tstep); break;
+      case 16: launch_fused<T, 16>(b, gs, tstep); break;
+      case 20: launch_fused<T, 20>(b, gs, tstep); break;
+      case 24: launch_fused<T, 24>(b, gs, tstep); break;
+      default: launch_fused<T, 32>(b, gs, tstep); break;
+    }
+    return;
+  }
   switch (b->xchunk) {
     case 1: launch_half<T, 1>(b, gs, tstep, e_half); break;
     case 2: launch_half<T, 2>(b, gs, tstep, e_half); break;
@@ -781,8 +800,11 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   for (const auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.fused != (variant == 3))
       return fail(OIDT_ESTATE, "cannot switch between fused and two-kernel forms while a run is in progress");
-  if (xchunk != 0 && xchunk != 1 && xchunk != 2 && xchunk != 4 && xchunk != 8 && xchunk != 16 && xchunk != 32)
-    return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32");
+  const int xc = xchunk ? xchunk : b->xchunk;   // the depth this launch form will use
+  const bool pow2 = xc == 1 || xc == 2 || xc == 4 || xc == 8 || xc == 16 || xc == 32;
+  const bool fused_only = xc == 12 || xc == 20 || xc == 24;
+  if (!pow2 && !(fused_only && variant == 3))
+    return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32 (also 12, 20 or 24 for the fused step)");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   if (variant == 3 && b->scheme == SCHEME_3D && (xchunk ? xchunk : b->xchunk) > kFused3dMaxXc)
     return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4, 8 or 16");

@@ -10,7 +10,7 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step (TMz and 3D)
-FUSED_LONG = {'tm': [(3, 16), (3, 32)], '3d': [(3, 16)], 'te': []}   # longer marches of the fused steps
+FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32)], '3d': [(3, 12), (3, 16)], 'te': []}
 
 
 @pytest.mark.gpu
@@ -70,3 +70,36 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
         err = rel_l2(z[w], od[key][w])
         assert err < 1e-10, (key, err)
     b.close()
+
+
+@pytest.mark.gpu
+def test_launch_form_validation():
+    """Tile depths outside a form's set are rejected, also when kept from an
+    earlier call, and the fused form does not exist for TEz."""
+    from oidt_amd._lib import OidtError
+    b = Batch('tm', np.float32, max_slots=1)
+    try:
+        b.set_launch(12, 0, 3)
+        assert b.launch()[:2] == (3, 12)
+        with pytest.raises(OidtError):
+            b.set_launch(0, 0, 2)       # keeps xchunk 12, which the two-kernel forms do not take
+        with pytest.raises(OidtError):
+            b.set_launch(12, 0, 1)
+        b.set_launch(4, 0, 2)
+        assert b.launch()[:2] == (2, 4)
+    finally:
+        b.close()
+    t = Batch('te', np.float32, max_slots=1)
+    try:
+        with pytest.raises(OidtError):
+            t.set_launch(4, 0, 3)
+    finally:
+        t.close()
+    d = Batch('3d', np.float32, max_slots=1)
+    try:
+        with pytest.raises(OidtError):
+            d.set_launch(20, 0, 3)
+        d.set_launch(12, 0, 3)
+    finally:
+        d.close()
+
