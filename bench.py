@@ -101,6 +101,8 @@ def workload(name):
         return [configs.config1()], '3D FP32 strip (x,y,z)=(128,96,320), 64 freqs (config 1)'
     if name == 'c2':
         return configs.config2(), '64 x 2D TMz FP32 slabs nx~U{800..4000} x 240, one batch (config 2)'
+    if name == 'c2te':
+        return configs.config2(scheme='te'), '64 x 2D TEz FP32 slabs nx~U{800..4000} x 240, one batch (config 2 in TEz)'
     if name == 'c3':
         return configs.config3(), '16 x 3D FP32 strips (96,64,nz~U{192..512}), one batch (config 3)'
     raise SystemExit('unknown workload ' + name)
@@ -211,7 +213,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=6016)
     ap.add_argument('--warmup', type=int, default=256)
-    ap.add_argument('--workload', default='c1')
+    ap.add_argument('--workload', default='c1', help='c0, c1, c2, c2te, c3 or c4 (serving)')
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=0,
                     help='kernel form: 1 march, 2 batched loads, 3 fused step; 0: autotuner picks form and xchunk')

@@ -7,7 +7,7 @@ with the smallest time per step.  Results are cached per workload class in a
 JSON file so a serving process tunes each class once.
 
 Kernel forms: 1 = per-plane march, 2 = batched loads (two launches per step),
-3 = fused Yee step (one launch per step; TMz and 3D).  The x-chunk depth is the number
+3 = fused Yee step (one launch per step).  The x-chunk depth is the number
 of cells each thread owns along x (the z-block depth of the tiles in 3D).
 """
 import ctypes as C
@@ -24,7 +24,7 @@ from .solver import Batch
 CANDIDATES = {
     'tm': [(2, 4, 0), (2, 4, 65536), (2, 8, 0), (3, 1, 0), (3, 2, 0), (3, 8, 0), (3, 12, 0), (3, 16, 0),
            (3, 20, 0), (3, 24, 0), (3, 32, 0)],
-    'te': [(2, 2, 0), (2, 4, 0), (2, 4, 65536), (2, 8, 0), (1, 4, 0)],
+    'te': [(2, 2, 0), (2, 4, 0), (2, 4, 65536), (2, 8, 0), (1, 4, 0), (3, 1, 0), (3, 2, 0), (3, 8, 0), (3, 16, 0)],
     '3d': [(2, 1, 0), (2, 1, 65536), (2, 2, 0), (1, 4, 0), (3, 4, 0), (3, 8, 0), (3, 16, 0)],
 }
 

@@ -62,13 +62,14 @@ def config1():
     return dict(strip3d(320), rough_std=5e-9, rough_acl=50e-9, seed=1)
 
 
-def config2(count=64, nt=5000):
-    """Ensemble of 2D FP32 realisations: nx ~ U{800..4000}, ny = 240, sigma ~ U(2,10) nm, Lc ~ U(20,100) nm."""
+def config2(count=64, nt=5000, scheme='tm'):
+    """Ensemble of 2D FP32 realisations: nx ~ U{800..4000}, ny = 240, sigma ~ U(2,10) nm, Lc ~ U(20,100) nm
+    (TMz; the same draws in TEz with ``scheme='te'``)."""
     rng = np.random.default_rng(2)
     out = []
     for r in range(count):
         nx = int(rng.integers(800, 4001))
-        out.append(dict(slab2d(nx, precision='float32', nt=nt), rough_std=float(rng.uniform(2e-9, 10e-9)),
+        out.append(dict(slab2d(nx, precision='float32', nt=nt, scheme=scheme), rough_std=float(rng.uniform(2e-9, 10e-9)),
                         rough_acl=float(rng.uniform(20e-9, 100e-9)), seed=1000 + r))
     return out
 

@@ -196,9 +196,9 @@ Layout<T> layout(const oidt_run_desc& d) {
     off = align_up(off + bytes, 256);
     return o;
   };
-  // TMz and 3D runs hold a second set of fields and psi: the fused step form
-  // reads one set and writes the other.  TEz updates in place.
-  const bool two = d.scheme != SCHEME_TEZ;
+  // Every run holds a second set of fields and psi: the fused step form
+  // reads one set and writes the other.
+  const bool two = true;
   for (int f = 0; f < 6; ++f) L.off_f[f] = field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
   for (int f = 0; f < 6; ++f) L.off_f2[f] = two && field_used(d.scheme, f) ? take(cells * sizeof(T)) : (size_t)-1;
   for (int i = 0; i < 36; ++i) L.off_psi[i] = L.off_psi2[i] = (size_t)-1;
@@ -468,7 +468,7 @@ int rebuild_lists(oidt_batch* b) {
         for (int y0 = 0; y0 < d.ny; y0 += kBY)
           for (int z0 = 0; z0 < d.nz; z0 += kBX) tiles.push_back(make_int4(si, x0, y0, z0));
     } else {
-      // the fused TMz step gives each wavefront its own tile of xc planes;
+      // the fused 2D steps give each wavefront its own tile of xc planes;
       // the other 2D forms give a block 4 waves x xc planes
       const int xstep = march_tiles(b) ? xc : xc * kBY;
       for (int x0 = 0; x0 < d.nx; x0 += xstep)
@@ -542,6 +542,8 @@ void launch_fused(oidt_batch* b, int gs, int tstep) {
   const dim3 blk(kBX, kBY), grd(gs);
   if (b->scheme == SCHEME_TMZ) {
     hipLaunchKernelGGL((k_tmz_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+  } else if (b->scheme == SCHEME_TEZ) {
+    hipLaunchKernelGGL((k_tez_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
   } else if constexpr (XM <= kFused3dMaxXc) {
     hipLaunchKernelGGL((k_3d_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
   }
@@ -796,7 +798,6 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
   if (variant < 1 || variant > 3) return fail(OIDT_EINVAL, "variant must be 1, 2 or 3");
-  if (variant == 3 && b->scheme == SCHEME_TEZ) return fail(OIDT_EINVAL, "the fused step form exists for TMz and 3D only");
   for (const auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.fused != (variant == 3))
       return fail(OIDT_ESTATE, "cannot switch between fused and two-kernel forms while a run is in progress");

@@ -1522,6 +1522,334 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
   }
 }
 
+// ---------------------------------------------------------- fused TEz step ---
+// The TEz counterpart of k_tmz_step: one launch per time step, set (n & 1)
+// read and the other written, one wavefront per tile of 64 rows x XM planes
+// marching along x.  The E half-step reads Hz at j+1 and i+1 (old set); the
+// Hz update reads the new Ex at j-1 (next-lower lane, lane 0 from the row
+// below the tile) and the new Ey at i-1 (the previous plane, in a register;
+// the plane below the tile computed first).  Same register rules as TMz: the
+// run's fields copied per tile, x-coefficients as per-tile lane vectors, DPP
+// and readlane for cross-lane values, loads two planes ahead, non-temporal
+// field stores.
+
+template <typename T>
+struct TezView {
+  int nx, ny, N, ib, ie, jb, je, sx0, sx1, sy0, sy1, port0, port1, wy0, wy1, nplanes, pcells;
+  bool src_hz;
+  T db, cb0[kMatMax], cb1[kMatMax];
+  const T *ike0, *ike1, *ikh0, *ikh1, *be, *ceo, *bh, *cho;
+  const uint8_t* mat;
+  T* ring;
+};
+
+template <typename T>
+__device__ __forceinline__ TezView<T> tez_view(const Slot<T>& s) {
+  TezView<T> v;
+  v.nx = s.nx; v.ny = s.ny; v.N = s.N;
+  v.ib = s.ib; v.ie = s.ie; v.jb = s.jb; v.je = s.je;
+  v.sx0 = s.sx0; v.sx1 = s.sx1; v.sy0 = s.sy0; v.sy1 = s.sy1;
+  v.port0 = s.port[0]; v.port1 = s.port[1]; v.wy0 = s.wy0; v.wy1 = s.wy1;
+  v.nplanes = s.nplanes; v.pcells = s.pcells;
+  v.src_hz = s.src_field == F_HZ;
+  v.db = s.db;
+#pragma unroll
+  for (int q = 0; q < kMatMax; ++q) {
+    v.cb0[q] = s.cbv[0][q];
+    v.cb1[q] = s.cbv[1][q];
+  }
+  v.ike0 = s.ike[0]; v.ike1 = s.ike[1]; v.ikh0 = s.ikh[0]; v.ikh1 = s.ikh[1];
+  v.be = s.be; v.ceo = s.ceo; v.bh = s.bh; v.cho = s.cho;
+  v.mat = s.mat;
+  v.ring = s.ring;
+  return v;
+}
+
+// Where a TEz cell's E components are updated (index ranges of
+// fdtd_funcs.py:44-80 and the CPML slabs :82-100, FDTD_2D_TE).
+struct TezWhere {
+  bool bx, by;
+  int sx, sy, dx, dy, cx, cy;   // Ey x-slab, Ex y-slab
+};
+
+template <typename T>
+__device__ __forceinline__ TezWhere tez_where(const TezView<T>& v, int i, int j) {
+  TezWhere f;
+  const bool in = i >= 0 && j >= 0 && i < v.nx && j < v.ny;
+  const bool iin = in && i >= v.ib && i < v.ie, iin1 = in && i > v.ib && i < v.ie;
+  const bool jin = in && j >= v.jb && j < v.je, jin1 = in && j > v.jb && j < v.je;
+  f.bx = iin1 && jin;
+  f.by = iin && jin1;
+  f.dx = f.dy = f.cx = f.cy = 0;
+  f.sy = f.bx ? slab_e(j, v.jb, v.je, v.N, f.dy, f.cy) : -1;
+  f.sx = f.by ? slab_e(i, v.ib, v.ie, v.N, f.dx, f.cx) : -1;
+  return f;
+}
+
+template <typename T>
+struct TezIn {
+  T ex, ey, hz, hzj, hzi, pxy, pyx, ikx, iky, bex, cex, bey, cey;
+  uint8_t m;
+};
+
+template <typename T>
+__device__ __forceinline__ TezIn<T> tez_load(const TezView<T>& v, int i, int j, const TezWhere& f,
+                                             const T* __restrict__ ExA, const T* __restrict__ EyA,
+                                             const T* __restrict__ HzA, T* const* psiA) {
+  TezIn<T> a;
+  const int nx = v.nx, ny = v.ny;
+  const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);
+  const unsigned idx = (unsigned)ic * ny + jc;
+  a.ex = ExA[idx];
+  a.ey = EyA[idx];
+  a.hz = HzA[idx];
+  a.hzj = HzA[jc + 1 < ny ? idx + 1 : idx];
+  a.hzi = HzA[ic + 1 < nx ? idx + ny : idx];
+  a.m = v.mat[idx];
+  a.iky = f.bx ? v.ike1[jc] : T(0);
+  a.ikx = f.by ? v.ike0[ic] : T(0);
+  a.pxy = f.sy >= 0 ? psiA[psi_id(F_EX, 1, f.sy)][(size_t)ic * v.N + f.dy] : T(0);
+  a.bey = f.sy >= 0 ? v.be[f.cy] : T(0);
+  a.cey = f.sy >= 0 ? v.ceo[f.cy] : T(0);
+  a.pyx = f.sx >= 0 ? psiA[psi_id(F_EY, 0, f.sx)][(size_t)f.dx * ny + jc] : T(0);
+  a.bex = f.sx >= 0 ? v.be[f.cx] : T(0);
+  a.cex = f.sx >= 0 ? v.ceo[f.cx] : T(0);
+  return a;
+}
+
+template <typename T>
+struct TezE {
+  T ex, ey, pxy, pyx;
+};
+
+// New Ex and Ey in the reference order: bulk curl, then the CPML term.
+template <typename T>
+__device__ __forceinline__ TezE<T> tez_e(const TezView<T>& v, const TezWhere& f, const TezIn<T>& a) {
+  TezE<T> r;
+  r.ex = a.ex;
+  r.ey = a.ey;
+  r.pxy = r.pyx = T(0);
+  if (f.bx) {
+    const int q = a.m & 3;
+    const T cb = q == 0 ? v.cb0[0] : (q == 1 ? v.cb0[1] : (q == 2 ? v.cb0[2] : v.cb0[3]));
+    const T dHz = a.hzj - a.hz;
+    T e = a.ex + cb * dHz * a.iky;
+    if (f.sy >= 0) {
+      r.pxy = a.bey * a.pxy + a.cey * dHz;
+      e += cb * r.pxy;
+    }
+    r.ex = e;
+  }
+  if (f.by) {
+    const int q = (a.m >> 2) & 3;
+    const T cb = q == 0 ? v.cb1[0] : (q == 1 ? v.cb1[1] : (q == 2 ? v.cb1[2] : v.cb1[3]));
+    const T dHz = a.hzi - a.hz;
+    T e = a.ey + cb * (-dHz) * a.ikx;
+    if (f.sx >= 0) {
+      r.pyx = a.bex * a.pyx + a.cex * dHz;
+      e -= cb * r.pyx;
+    }
+    r.ey = e;
+  }
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ TezE<T> tez_e_cell(const TezView<T>& v, int i, int j, const T* __restrict__ ExA,
+                                              const T* __restrict__ EyA, const T* __restrict__ HzA,
+                                              T* const* psiA) {
+  const TezWhere f = tez_where(v, i, j);
+  return tez_e(v, f, tez_load(v, i, j, f, ExA, EyA, HzA, psiA));
+}
+
+template <typename T>
+struct TezPlane {
+  T ex, ey, hzj, hzi, pxy, pyx, qzx, qzy;
+  uint8_t m;
+};
+
+template <typename T, int XM>
+__global__ __launch_bounds__(kThreads) void k_tez_step(const Slot<T>* __restrict__ slots,
+                                                       const int4* __restrict__ tiles,
+                                                       const int* __restrict__ counts, int tstep) {
+  static_assert(XM <= kBX, "the j0-1 halo row takes one cell per lane");
+  const int ntiles = counts[0];
+  const int lane = threadIdx.x;
+  const int stride = gridDim.x * kBY;
+  for (int tt = xcd_block() * kBY + wave_id(); tt < ntiles; tt += stride) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const TezView<T> v = tez_view(s);
+    const bool odd = (n & 1) != 0;
+    const T* __restrict__ ExA = odd ? s.f2[F_EX] : s.f[F_EX];
+    const T* __restrict__ EyA = odd ? s.f2[F_EY] : s.f[F_EY];
+    const T* __restrict__ HzA = odd ? s.f2[F_HZ] : s.f[F_HZ];
+    T* __restrict__ ExB = odd ? s.f[F_EX] : s.f2[F_EX];
+    T* __restrict__ EyB = odd ? s.f[F_EY] : s.f2[F_EY];
+    T* __restrict__ HzB = odd ? s.f[F_HZ] : s.f2[F_HZ];
+    T* const* psiAll = odd ? s.psi2 : s.psi;
+    T* const* psiBll = odd ? s.psi : s.psi2;
+    const T jv = v.src_hz ? s.jsrc[n] : T(0);
+    const int nx = v.nx, ny = v.ny, N = v.N;
+    const int i0 = tl.y, j0 = tl.z;
+    const int j = j0 + lane;
+    const bool jok = j < ny;
+    const int jc = min(j, ny - 1);
+    // row (y-only) constants of this lane, fixed over the march
+    const bool jin = jok && j >= v.jb && j < v.je;     // Ex rows
+    const bool jin1 = jok && j > v.jb && j < v.je;     // Ey and Hz rows
+    const bool jsrc = jok && v.src_hz && j >= v.sy0 && j < v.sy1;
+    int dye = 0, cye = 0, dyh = 0, cyh = 0;
+    const int sye = jin ? slab_e(j, v.jb, v.je, N, dye, cye) : -1;
+    const int syh = jin1 ? slab_h(j, v.jb, v.je, N, dyh, cyh) : -1;
+    const T ikye = jin ? v.ike1[j] : T(0);
+    const T bey = sye >= 0 ? v.be[cye] : T(0), cey = sye >= 0 ? v.ceo[cye] : T(0);
+    const T ikyh = jin1 ? v.ikh1[j] : T(0);
+    const T bhy = syh >= 0 ? v.bh[cyh] : T(0), chy = syh >= 0 ? v.cho[cyh] : T(0);
+    const bool jcap = jok && j >= v.wy0 && j <= v.wy1;
+    const T* pxyA = sye >= 0 ? psiAll[psi_id(F_EX, 1, sye)] : nullptr;
+    T* pxyB = sye >= 0 ? psiBll[psi_id(F_EX, 1, sye)] : nullptr;
+    const T* pzyA = syh >= 0 ? psiAll[psi_id(F_HZ, 1, syh)] : nullptr;
+    T* pzyB = syh >= 0 ? psiBll[psi_id(F_HZ, 1, syh)] : nullptr;
+    const int iend = min(i0 + XM, nx);
+    auto load_plane = [&](int i) {
+      TezPlane<T> p;
+      const bool iin = i >= v.ib && i < v.ie, iin1 = i > v.ib && i < v.ie;
+      int dx = 0, cx = 0, dxh = 0, cxh = 0;
+      const int sxe = iin ? slab_e(i, v.ib, v.ie, N, dx, cx) : -1;
+      const int sxh = iin1 ? slab_h(i, v.ib, v.ie, N, dxh, cxh) : -1;
+      const unsigned idx = (unsigned)i * ny + jc;
+      p.ex = ExA[idx];
+      p.ey = EyA[idx];
+      p.hzj = HzA[jc + 1 < ny ? idx + 1 : idx];
+      p.hzi = HzA[i + 1 < nx ? idx + ny : idx];
+      p.m = v.mat[idx];
+      p.pxy = iin1 && sye >= 0 ? pxyA[(size_t)i * N + dye] : T(0);
+      p.pyx = jin1 && sxe >= 0 ? psiAll[psi_id(F_EY, 0, sxe)][(size_t)dx * ny + jc] : T(0);
+      p.qzx = jin1 && sxh >= 0 ? psiAll[psi_id(F_HZ, 0, sxh)][(size_t)dxh * ny + jc] : T(0);
+      p.qzy = iin1 && syh >= 0 ? pzyA[(size_t)i * N + dyh] : T(0);
+      return p;
+    };
+    // ---- x-only coefficients of the tile's planes: lane u holds plane i0+u
+    T lk_ikx = T(0), lk_bex = T(0), lk_cex = T(0), lk_ikhx = T(0), lk_bhx = T(0), lk_chx = T(0);
+    if (lane < XM && i0 + lane < nx) {
+      const int i = i0 + lane;
+      const bool iin = i >= v.ib && i < v.ie, iin1 = i > v.ib && i < v.ie;
+      int dx = 0, cx = 0;
+      lk_ikx = iin ? v.ike0[i] : T(0);
+      if (iin && slab_e(i, v.ib, v.ie, N, dx, cx) >= 0) {
+        lk_bex = v.be[cx];
+        lk_cex = v.ceo[cx];
+      }
+      lk_ikhx = v.ikh0[i];
+      if (iin1 && slab_h(i, v.ib, v.ie, N, dx, cx) >= 0) {
+        lk_bhx = v.bh[cx];
+        lk_chx = v.cho[cx];
+      }
+    }
+    // ---- halos from set A, not stored: new Ex of row j0-1 (lane u holds
+    // plane i0+u) and new Ey of plane i0-1
+    const T colEx = lane < XM ? tez_e_cell(v, i0 + lane, j0 - 1, ExA, EyA, HzA, psiAll).ex : T(0);
+    T prevEy = tez_e_cell(v, i0 - 1, j, ExA, EyA, HzA, psiAll).ey;
+    // ---- march along x; Hz(i) is the previous plane's Hz(i+1)
+    T hzC = HzA[(unsigned)i0 * ny + jc];
+    TezPlane<T> p0 = load_plane(i0);
+    TezPlane<T> p1 = i0 + 1 < iend ? load_plane(i0 + 1) : p0;
+#pragma unroll 1
+    for (int u = 0; u < XM; ++u) {
+      const int i = i0 + u;
+      if (i >= iend) break;
+      const TezPlane<T> p2 = i + 2 < iend ? load_plane(i + 2) : p1;
+      const bool iin = i >= v.ib && i < v.ie, iin1 = i > v.ib && i < v.ie;
+      int dxe = 0, cxe = 0, dxh = 0, cxh = 0;
+      const int sxe = iin ? slab_e(i, v.ib, v.ie, N, dxe, cxe) : -1;
+      const int sxh = iin1 ? slab_h(i, v.ib, v.ie, N, dxh, cxh) : -1;
+      // E half-step of (i, j), through the same update as the halo cells
+      TezWhere f;
+      f.bx = iin1 && jin;
+      f.by = iin && jin1;
+      f.sy = f.bx ? sye : -1;
+      f.dy = dye;
+      f.cy = cye;
+      f.sx = f.by ? sxe : -1;
+      f.dx = dxe;
+      f.cx = cxe;
+      TezIn<T> a;
+      a.ex = p0.ex;
+      a.ey = p0.ey;
+      a.hz = hzC;
+      a.hzj = p0.hzj;
+      a.hzi = p0.hzi;
+      a.m = p0.m;
+      a.iky = ikye;
+      a.ikx = lane_get(lk_ikx, u);
+      a.pxy = p0.pxy;
+      a.bey = bey;
+      a.cey = cey;
+      a.pyx = p0.pyx;
+      a.bex = lane_get(lk_bex, u);
+      a.cex = lane_get(lk_cex, u);
+      const TezE<T> e = tez_e(v, f, a);
+      const unsigned idx = (unsigned)i * ny + jc;
+      if (f.bx) {
+        __builtin_nontemporal_store(e.ex, ExB + idx);
+        if (f.sy >= 0) pxyB[(size_t)i * N + f.dy] = e.pxy;
+      }
+      if (f.by) {
+        __builtin_nontemporal_store(e.ey, EyB + idx);
+        if (f.sx >= 0) psiBll[psi_id(F_EY, 0, f.sx)][(size_t)f.dx * ny + j] = e.pyx;
+      }
+      if (jcap) {
+        const size_t c = (size_t)(j - v.wy0);
+        if (i == v.port0) v.ring[((size_t)tstep * v.nplanes + 0) * v.pcells + c] = e.ey;
+        if (i == v.port1) v.ring[((size_t)tstep * v.nplanes + 3) * v.pcells + c] = e.ey;
+      }
+      // Hz of (i, j): Ex(j-1) from the lower lane, lane 0 from the halo row
+      const T exj = lane_up1(e.ex, lane_get(colEx, u));
+      if (jok) {
+        const T db = v.db;
+        const bool bulk = jin1 && iin1;
+        const bool src = jsrc && i >= v.sx0 && i < v.sx1;
+        T hv = hzC;
+        if (bulk) {
+          const T dEx = e.ex - exj;
+          const T dEy = e.ey - prevEy;
+          hv = hv + db * (dEx * ikyh - dEy * lane_get(lk_ikhx, u));
+          if (src) hv -= jv;
+          if (sxh >= 0) {
+            const T q = lane_get(lk_bhx, u) * p0.qzx + lane_get(lk_chx, u) * dEy;
+            psiBll[psi_id(F_HZ, 0, sxh)][(size_t)dxh * ny + j] = q;
+            hv -= db * q;
+          }
+          if (syh >= 0) {
+            const T q = bhy * p0.qzy + chy * dEx;
+            pzyB[(size_t)i * N + dyh] = q;
+            hv += db * q;
+          }
+          __builtin_nontemporal_store(hv, HzB + idx);
+        } else if (src) {
+          hv -= jv;
+          __builtin_nontemporal_store(hv, HzB + idx);
+        }
+        if (jcap) {
+          const size_t c = (size_t)(j - v.wy0);
+          const size_t base = (size_t)tstep * v.nplanes;
+          if (i == v.port0) v.ring[(base + 1) * v.pcells + c] = hv;
+          if (i == v.port0 + 1) v.ring[(base + 2) * v.pcells + c] = hv;
+          if (i == v.port1) v.ring[(base + 4) * v.pcells + c] = hv;
+          if (i == v.port1 + 1) v.ring[(base + 5) * v.pcells + c] = hv;
+        }
+      }
+      prevEy = e.ey;
+      hzC = p0.hzi;
+      p0 = p1;
+      p1 = p2;
+    }
+  }
+}
+
 // ------------------------------------------------------- fused 3D step ---
 // One launch per time step for 3D, reading field/psi set (n & 1) and writing
 // the other, so no tile races another.  Each wavefront owns a tile of 64

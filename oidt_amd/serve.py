@@ -79,8 +79,6 @@ class DeviceBackend:
             b = S.Batch(setup.scheme, setup.precision, self.max_slots)
             variant, xchunk, grid = (tuple(self.launch[setup.scheme]) + (0,))[:3] if setup.scheme in self.launch \
                 else (self.variant, self.xchunk, 0)
-            if variant == 3 and setup.scheme == 'te':
-                variant = 2
             b.set_launch(xchunk, grid, variant)
             self.batches[key] = b
             self.resident[key] = 0

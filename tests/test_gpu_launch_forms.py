@@ -9,8 +9,8 @@ from oidt_amd.solver import Batch
 from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
-FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step (TMz and 3D)
-FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32)], '3d': [(3, 12), (3, 16)], 'te': []}
+FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
+FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32)], '3d': [(3, 12), (3, 16)], 'te': [(3, 16), (3, 32)]}
 
 
 @pytest.mark.gpu
@@ -23,7 +23,7 @@ def test_all_launch_forms_match_oracle(scheme):
     ref_f = o.fields()
     od = o.dft()
     w = window(scheme, o.c)
-    for variant, xc in FORMS + (FUSED if scheme != 'te' else []) + FUSED_LONG[scheme]:
+    for variant, xc in FORMS + FUSED + FUSED_LONG[scheme]:
         print('case', scheme, 'variant', variant, 'xchunk', xc, flush=True)
         b = Batch(scheme, np.float32, max_slots=1)
         b.set_launch(xc, 0, variant)
@@ -42,7 +42,7 @@ def test_all_launch_forms_match_oracle(scheme):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('scheme,xc,direction,extra', [
-    ('tm', 1, 'b', {}), ('tm', 4, 'f', {'nt': 101}),
+    ('tm', 1, 'b', {}), ('tm', 4, 'f', {'nt': 101}), ('te', 2, 'f', {}), ('te', 16, 'b', {'nt': 101}),
     ('3d', 1, 'f', {}), ('3d', 2, 'b', {}), ('3d', 4, 'f', {'nt': 101}), ('3d', 8, 'f', {'source_condition': 'TM'}),
     ('3d', 16, 'b', {'source_condition': 'TM'}),
 ])
@@ -75,7 +75,7 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
 @pytest.mark.gpu
 def test_launch_form_validation():
     """Tile depths outside a form's set are rejected, also when kept from an
-    earlier call, and the fused form does not exist for TEz."""
+    earlier call."""
     from oidt_amd._lib import OidtError
     b = Batch('tm', np.float32, max_slots=1)
     try:
@@ -91,8 +91,9 @@ def test_launch_form_validation():
         b.close()
     t = Batch('te', np.float32, max_slots=1)
     try:
+        t.set_launch(20, 0, 3)
         with pytest.raises(OidtError):
-            t.set_launch(4, 0, 3)
+            t.set_launch(0, 0, 1)
     finally:
         t.close()
     d = Batch('3d', np.float32, max_slots=1)
