@@ -130,7 +130,13 @@ int validate(const oidt_run_desc* d) {
     // both CPML slabs must fit inside the update range without overlapping
     if (dims[a] - 2 * o < 2 * N + 1) return fail(OIDT_EINVAL, "grid too small for the CPML thickness");
   }
-  if ((double)d->nx * d->ny * d->nz > 4.0e9) return fail(OIDT_EINVAL, "grid larger than 2^32 cells");
+  // Kernels index cells with 32-bit offsets into the padded layout (the
+  // unit-stride axis rounded up to 128 bytes).
+  const int q = d->precision == OIDT_F64 ? 16 : 32;
+  const double fast = is3d ? (double)((d->nz + q - 1) / q * q) : (double)((d->ny + q - 1) / q * q);
+  const double slow = is3d ? (double)d->nx * d->ny : (double)d->nx;
+  if ((double)d->nx * d->ny * d->nz > 4.0e9 || slow * fast > 4.0e9)
+    return fail(OIDT_EINVAL, "grid larger than 2^32 cells (after padding the unit-stride axis to 128 bytes)");
   if (d->nt < 0 || d->nf < 0) return fail(OIDT_EINVAL, "negative nt or nf");
   const int plim = is3d ? d->nz : d->nx;
   for (int p = 0; p < 2; ++p)

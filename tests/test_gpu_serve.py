@@ -23,12 +23,18 @@ def _requests():
     return descs
 
 
+FUSED = {'tm': (3, 16, 0), '3d': (3, 4, 0)}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize('budget_runs', [100, 2])
-def test_results_identical_alone_and_in_mix(budget_runs):
+@pytest.mark.parametrize('budget_runs,launch', [(100, None), (2, None), (100, FUSED)])
+def test_results_identical_alone_and_in_mix(budget_runs, launch):
+    """Also with the fused step forms, whose runs alternate between two field
+    sets and share per-wave tile lists: a run admitted while others are half
+    way through still gives the phasors it gives alone."""
     descs = _requests()
     reqs = serve.make_requests(descs, [0.0, 0.002, 0.004, 0.006, 0.02, 0.03])
-    be = serve.DeviceBackend(max_slots=8)
+    be = serve.DeviceBackend(max_slots=8, launch=launch)
     fp = max(be.footprint(configs.realise(d)[0]) for d in descs)
     srv = serve.Server(be, hbm_budget=budget_runs * fp, chunk_group=2)
     done = srv.run(reqs)
@@ -41,7 +47,11 @@ def test_results_identical_alone_and_in_mix(budget_runs):
         assert srv.peak_resident >= 3            # requests overlapped on the device
     for r in done:
         print('request', r.rid, r.setup.scheme, r.setup.shape, r.setup.nt, 'latency %.3f' % r.latency, flush=True)
-        alone = Solver(r.setup, r.fg).run()
+        alone = Solver(r.setup, r.fg)
+        if launch:
+            variant, xchunk, grid = launch[r.setup.scheme]
+            alone.set_launch(xchunk, grid, variant)
+        alone.run()
         ref = alone.dft()
         alone.close()
         for k, z in ref.items():
