@@ -87,6 +87,37 @@ typedef struct oidt_run_inputs {
   const double* freqs;     /* sparam_freqs[nf], Hz */
 } oidt_run_inputs;
 
+/* Field monitor of the loss and impedance study drivers: at every cell of a
+ * box, the sum over 1-4 stencil points of one field component (points outside
+ * the grid read 0), times `scale`, taken once per step right after the E or
+ * the H half-step, and either
+ *   mode OIDT_MON_DFT:     re += v * cs[r][0], im += v * cs[r][1]   (single-
+ *                          frequency running DFT; the product and sum are
+ *                          formed in double and rounded to the run precision,
+ *                          as the reference's float32 accumulators are), or
+ *   mode OIDT_MON_CAPTURE: out[r][cell] = v                          (time record),
+ * where r = step + shift, and steps with r outside [0, nrec) are skipped.
+ * Cells are ordered x-major, z fastest, over the box.
+ * Replaces simul_fft (FDTD_2D_TM/scattering_loss/fdtd_funcs.py:138-144, TE
+ * :113-120), simul_fft_efield / simul_fft_hfield (FDTD_3D/scattering_loss/
+ * fdtd_funcs.py:217-242) and map_efield_zwave / map_hfield_zwave
+ * (FDTD_2D_TM/wave_impedance/fdtd_funcs.py:155-168, FDTD_2D_TE/wave_impedance/
+ * fdtd_funcs.py:169-181, FDTD_3D/wave_impedance/fdtd_funcs.py:321-335). */
+typedef enum { OIDT_MON_DFT = 0, OIDT_MON_CAPTURE = 1 } oidt_monitor_mode;
+
+typedef struct oidt_monitor_desc {
+  int32_t field;        /* oidt_field */
+  int32_t after_h;      /* 0: taken after the E half-step, 1: after the H half-step */
+  int32_t mode;         /* oidt_monitor_mode */
+  int32_t npts;         /* stencil points, 1..4 */
+  int32_t off[4][3];    /* stencil offsets (x, y, z) of each point, each in -1..1 */
+  double scale;         /* factor on the stencil sum: 1, 0.5, 0.25 in the reference */
+  int32_t box[6];       /* x0,x1,y0,y1,z0,z1: half-open box of monitored cells */
+  int32_t shift;        /* record index r = step + shift */
+  int32_t nrec;         /* records kept, r in [0, nrec) */
+  const double* cs;     /* OIDT_MON_DFT: [nrec][2] cos and sin weight of each record */
+} oidt_monitor_desc;
+
 typedef struct oidt_device_info {
   char name[64];
   char arch[32];
@@ -150,6 +181,11 @@ int oidt_solver_read_dft(oidt_solver* s, int32_t port, int32_t comp, double* re,
 /* Copy one field component ([nx][ny][nz], run precision) to the host. */
 int oidt_solver_read_field(oidt_solver* s, int32_t field, void* host, uint64_t nbytes);
 
+/* Field monitors of a single run; as oidt_batch_add_monitor /
+ * oidt_batch_read_monitor below. */
+int oidt_solver_add_monitor(oidt_solver* s, const oidt_monitor_desc* m, int32_t* monitor);
+int oidt_solver_read_monitor(oidt_solver* s, int32_t monitor, void* re, void* im, uint64_t nbytes);
+
 /* Free the run's device memory.  Replaces cuda.close(), fdtd_main.py:388. */
 int oidt_solver_destroy(oidt_solver* s);
 
@@ -191,6 +227,22 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
  * FDTD_3D/.../extract_sparams.py:114-131.  re/im receive nf values. */
 int oidt_batch_port_power(oidt_batch* b, int32_t slot, int32_t port, int32_t sub_slot, int32_t sub_port,
                           int32_t row0, int32_t nrows, double* re, double* im);
+
+/* Attach a field monitor (oidt_monitor_desc) to a run that has not started;
+ * *monitor receives its index within the run.  Monitors need a two-kernel
+ * form (set_launch variant 1 or 2): the fused step has no point between the
+ * half-steps (OIDT_ESTATE).  Replaces the monitor arrays' allocation and
+ * cuda.to_device of FDTD_2D_TM/scattering_loss/fdtd_main.py:51-56 and their
+ * launches in the time loop (:216-218; TE :222-223; 3D :341-344; wave
+ * impedance 2D TM :210-211, 2D TE :220-221, 3D :298-299). */
+int oidt_batch_add_monitor(oidt_batch* b, int32_t slot, const oidt_monitor_desc* m, int32_t* monitor);
+
+/* Copy a monitor to the host (waits for the batch stream): OIDT_MON_DFT fills
+ * re and im with [cells] values, OIDT_MON_CAPTURE fills re with [nrec][cells]
+ * (im unused); run precision; nbytes is the size of each buffer.  Replaces
+ * the copy_to_host calls of FDTD_2D_TM/scattering_loss/fdtd_main.py:260-262
+ * (3D :386-409; wave impedance 2D TM :253-254, 3D :342-346). */
+int oidt_batch_read_monitor(oidt_batch* b, int32_t slot, int32_t monitor, void* re, void* im, uint64_t nbytes);
 
 /* Release a slot (its device memory is returned in stream order). */
 int oidt_batch_remove(oidt_batch* b, int32_t slot);

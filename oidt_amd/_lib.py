@@ -34,6 +34,15 @@ class RunInputs(C.Structure):
                 ('jsrc', C.c_void_p), ('freqs', C.c_void_p)]
 
 
+MON_DFT, MON_CAPTURE = 0, 1
+
+
+class MonitorDesc(C.Structure):
+    _fields_ = [('field', C.c_int32), ('after_h', C.c_int32), ('mode', C.c_int32), ('npts', C.c_int32),
+                ('off', (C.c_int32 * 3) * 4), ('scale', C.c_double), ('box', C.c_int32 * 6),
+                ('shift', C.c_int32), ('nrec', C.c_int32), ('cs', C.c_void_p)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [('name', C.c_char * 64), ('arch', C.c_char * 32), ('compute_units', C.c_int32),
                 ('device_id', C.c_int32), ('hbm_total_bytes', C.c_uint64), ('hbm_free_bytes', C.c_uint64)]
@@ -53,6 +62,8 @@ SIGNATURES = {
     'oidt_solver_steps': (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     'oidt_solver_read_dft': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     'oidt_solver_read_field': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]),
+    'oidt_solver_add_monitor': (C.c_int, [C.c_void_p, C.POINTER(MonitorDesc), C.POINTER(C.c_int32)]),
+    'oidt_solver_read_monitor': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64]),
     'oidt_solver_destroy': (C.c_int, [C.c_void_p]),
     'oidt_batch_create': (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     'oidt_batch_add': (C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(RunInputs), C.POINTER(C.c_int32)]),
@@ -62,6 +73,9 @@ SIGNATURES = {
     'oidt_batch_progress': (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     'oidt_batch_read_dft': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]),
     'oidt_batch_read_field': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_uint64]),
+    'oidt_batch_add_monitor': (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(MonitorDesc), C.POINTER(C.c_int32)]),
+    'oidt_batch_read_monitor': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                          C.c_uint64]),
     'oidt_batch_remove': (C.c_int, [C.c_void_p, C.c_int32]),
     'oidt_batch_set_launch': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     'oidt_batch_port_power': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,

@@ -82,6 +82,15 @@ struct Staging {
   }
 };
 
+// A field monitor of one run: its description and device buffers.
+struct MonHost {
+  oidt_monitor_desc d{};
+  void* re = nullptr;
+  void* im = nullptr;
+  double* cs = nullptr;
+  size_t cells = 0;
+};
+
 struct SlotHost {
   bool used = false;
   oidt_run_desc desc{};
@@ -91,6 +100,7 @@ struct SlotHost {
   int64_t admit_chunk = 0;          // batch chunk counter when admitted
   bool fused = false;               // advanced by the fused step kernel (alternating field sets)
   int nplanes = 0, nq = 0, wcells = 0, pcells = 0, wyn = 0;
+  std::vector<MonHost> mons;
 };
 
 int field_count(int scheme) { return scheme == SCHEME_3D ? 6 : 3; }
@@ -280,6 +290,16 @@ struct oidt_batch {
   hipStream_t aux = nullptr;   // reductions over finished runs, off the stepping stream
   double* d_pow = nullptr;     // [nf][2] port power output
   size_t pow_cap = 0;
+  void* d_mons = nullptr;      // Monitor<T> table of all resident runs
+  size_t mons_cap = 0;         // bytes
+  int4* d_mwork[2] = {nullptr, nullptr};   // monitor work after the E / H half-step
+  size_t mwork_cap[2] = {0, 0};
+  int nmwork[2] = {0, 0};
+  bool has_monitors() const {
+    for (const auto& sh : slots)
+      if (sh.used && !sh.mons.empty()) return true;
+    return false;
+  }
   size_t slot_bytes() const { return precision == OIDT_F64 ? sizeof(Slot<double>) : sizeof(Slot<float>); }
 };
 
@@ -461,6 +481,53 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
 // Kernels whose tiles are per wavefront (four tiles per block).
 bool march_tiles(const oidt_batch* b) { return b->variant == 3; }
 
+// Monitor table and the two work lists (after the E and after the H
+// half-step): one item per kThreads consecutive box cells.
+template <typename T>
+int rebuild_monitors(oidt_batch* b) {
+  std::vector<Monitor<T>> table;
+  std::vector<int4> work[2];
+  for (int si = 0; si < (int)b->slots.size(); ++si) {
+    const SlotHost& sh = b->slots[si];
+    if (!sh.used) continue;
+    for (const MonHost& mh : sh.mons) {
+      const oidt_monitor_desc& d = mh.d;
+      Monitor<T> m{};
+      m.re = static_cast<T*>(mh.re);
+      m.im = static_cast<T*>(mh.im);
+      m.cs = mh.cs;
+      m.scale = d.scale;
+      m.slot = si; m.field = d.field; m.mode = d.mode; m.npts = d.npts;
+      for (int q = 0; q < 4; ++q) { m.ox[q] = d.off[q][0]; m.oy[q] = d.off[q][1]; m.oz[q] = d.off[q][2]; }
+      m.x0 = d.box[0]; m.y0 = d.box[2]; m.z0 = d.box[4];
+      m.ex = d.box[1] - d.box[0]; m.ey = d.box[3] - d.box[2]; m.ez = d.box[5] - d.box[4];
+      m.gx = sh.desc.nx; m.gy = sh.desc.ny; m.gz = sh.desc.nz;
+      m.shift = d.shift; m.nrec = d.nrec;
+      const int id = (int)table.size();
+      table.push_back(m);
+      for (size_t c0 = 0; c0 < mh.cells; c0 += kThreads) work[d.after_h ? 1 : 0].push_back(make_int4(id, (int)c0, 0, 0));
+    }
+  }
+  int rc;
+  const size_t tbytes = std::max<size_t>(table.size(), 1) * sizeof(Monitor<T>);
+  if (tbytes > b->mons_cap) {
+    void* nb = nullptr;
+    if (hipMallocAsync(&nb, tbytes, b->stream) != hipSuccess) return fail(OIDT_ENOMEM, "monitor table allocation failed");
+    if (b->d_mons) HIP_TRY(hipFreeAsync(b->d_mons, b->stream));
+    b->d_mons = nb;
+    b->mons_cap = tbytes;
+  }
+  if (!table.empty() && (rc = upload_bytes(b, b->d_mons, table.data(), table.size() * sizeof(Monitor<T>))) != OIDT_OK)
+    return rc;
+  for (int h = 0; h < 2; ++h) {
+    if ((rc = grow(b, &b->d_mwork[h], &b->mwork_cap[h], std::max<size_t>(work[h].size(), 1))) != OIDT_OK) return rc;
+    if (!work[h].empty() && (rc = upload_bytes(b, b->d_mwork[h], work[h].data(), work[h].size() * sizeof(int4))) != OIDT_OK)
+      return rc;
+    b->nmwork[h] = (int)work[h].size();
+  }
+  return OIDT_OK;
+}
+
 // Rebuild the tile and DFT work lists for the resident slots and upload them.
 int rebuild_lists(oidt_batch* b) {
   std::vector<int4> tiles, work;
@@ -495,7 +562,7 @@ int rebuild_lists(oidt_batch* b) {
   if ((rc = upload_bytes(b, b->d_counts, counts, sizeof(counts))) != OIDT_OK) return rc;
   b->ntiles = counts[0];
   b->nwork = counts[1];
-  return OIDT_OK;
+  return b->precision == OIDT_F64 ? rebuild_monitors<double>(b) : rebuild_monitors<float>(b);
 }
 
 int grid_for(int items, int cap) {
@@ -606,11 +673,21 @@ int stencil_grid(const oidt_batch* b) {
 int dft_grid(const oidt_batch* b) { return grid_for(b->nwork, 8192); }
 
 template <typename T>
+void launch_monitors(oidt_batch* b, int t, int h) {
+  if (b->nmwork[h] == 0) return;
+  hipLaunchKernelGGL((k_monitor<T>), dim3(grid_for(b->nmwork[h], 8192)), dim3(kThreads), 0, b->stream,
+                     static_cast<const Slot<T>*>(b->d_slots), static_cast<const Monitor<T>*>(b->d_mons),
+                     b->d_mwork[h], b->nmwork[h], t);
+}
+
+template <typename T>
 int enqueue_chunk(oidt_batch* b) {
   const int gs = stencil_grid(b), gd = dft_grid(b);
   for (int t = 0; t < kChunk; ++t) {
     launch_stencil<T>(b, gs, t, true);
+    launch_monitors<T>(b, t, 0);
     if (b->variant != 3) launch_stencil<T>(b, gs, t, false);
+    launch_monitors<T>(b, t, 1);
   }
   launch_dft<T>(b, gd);
   launch_advance<T>(b);
@@ -631,6 +708,7 @@ int profile_chunks(oidt_batch* b, int nchunks, double* out) {
         HIP_TRY(hipEventRecord(ev[k++], b->stream));
         launch_stencil<T>(b, gs, t, h == 0);
         HIP_TRY(hipEventRecord(ev[k++], b->stream));
+        launch_monitors<T>(b, t, h);   // untimed; keeps the runs' records whole
       }
     HIP_TRY(hipEventRecord(ev[k++], b->stream));
     launch_dft<T>(b, gd);
@@ -815,6 +893,8 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   if (variant == 3 && b->scheme == SCHEME_3D && (xchunk ? xchunk : b->xchunk) > kFused3dMaxXc)
     return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4, 8 or 16");
+  if (variant == 3 && b->has_monitors())
+    return fail(OIDT_ESTATE, "field monitors need a two-kernel form (variant 1 or 2)");
   b->variant = variant;
   for (auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued == sh.admit_chunk) sh.fused = variant == 3;
@@ -993,8 +1073,93 @@ int oidt_batch_remove(oidt_batch* b, int32_t slot) {
   rc = upload_bytes(b, static_cast<char*>(b->d_slots) + (size_t)slot * b->slot_bytes(), zero.data(), zero.size());
   if (rc != OIDT_OK) return rc;
   if (sh.dmem) HIP_TRY(hipFreeAsync(sh.dmem, b->stream));
+  for (auto& mh : sh.mons)
+    for (void* p : {mh.re, mh.im, static_cast<void*>(mh.cs)})
+      if (p) HIP_TRY(hipFreeAsync(p, b->stream));
   sh = SlotHost();
   b->dirty = true;
+  return OIDT_OK;
+}
+
+int oidt_batch_add_monitor(oidt_batch* b, int32_t slot, const oidt_monitor_desc* m, int32_t* monitor) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  if (!m || !monitor) return fail(OIDT_EINVAL, "null argument");
+  SlotHost& sh = b->slots[slot];
+  const oidt_run_desc& d = sh.desc;
+  if (b->chunks_enqueued != sh.admit_chunk) return fail(OIDT_ESTATE, "monitors are attached before the run starts");
+  if (b->variant == 3) return fail(OIDT_ESTATE, "field monitors need a two-kernel form (variant 1 or 2)");
+  if (m->field < 0 || m->field > 5 || !field_used(d.scheme, m->field))
+    return fail(OIDT_EINVAL, "monitor field not present in this scheme");
+  if ((m->after_h != 0 && m->after_h != 1) || (m->mode != OIDT_MON_DFT && m->mode != OIDT_MON_CAPTURE))
+    return fail(OIDT_EINVAL, "monitor after_h must be 0 or 1 and mode OIDT_MON_DFT or OIDT_MON_CAPTURE");
+  if (m->npts < 1 || m->npts > 4) return fail(OIDT_EINVAL, "monitor stencil takes 1 to 4 points");
+  for (int q = 0; q < m->npts; ++q)
+    for (int a = 0; a < 3; ++a)
+      if (m->off[q][a] < -1 || m->off[q][a] > 1) return fail(OIDT_EINVAL, "monitor stencil offsets must be -1, 0 or 1");
+  const int dims[3] = {d.nx, d.ny, d.nz};
+  for (int a = 0; a < 3; ++a)
+    if (m->box[2 * a] < 0 || m->box[2 * a] >= m->box[2 * a + 1] || m->box[2 * a + 1] > dims[a])
+      return fail(OIDT_EINVAL, "monitor box empty or outside the grid");
+  if (m->nrec < 1) return fail(OIDT_EINVAL, "monitor needs nrec >= 1");
+  if (!std::isfinite(m->scale)) return fail(OIDT_EINVAL, "monitor scale not finite");
+  if (m->mode == OIDT_MON_DFT && !m->cs) return fail(OIDT_EINVAL, "DFT monitor needs its cos/sin table");
+  const size_t cells = (size_t)(m->box[1] - m->box[0]) * (m->box[3] - m->box[2]) * (m->box[5] - m->box[4]);
+  if (cells > (size_t)1 << 30) return fail(OIDT_EINVAL, "monitor box larger than 2^30 cells");
+  const size_t esz = b->precision == OIDT_F64 ? sizeof(double) : sizeof(float);
+  MonHost mh;
+  mh.d = *m;
+  mh.d.cs = nullptr;
+  mh.cells = cells;
+  const size_t rbytes = (m->mode == OIDT_MON_DFT ? cells : cells * (size_t)m->nrec) * esz;
+  auto undo = [&]() {
+    for (void* p : {mh.re, mh.im, static_cast<void*>(mh.cs)})
+      if (p) (void)hipFreeAsync(p, b->stream);
+  };
+  if (hipMallocAsync(&mh.re, rbytes, b->stream) != hipSuccess) {
+    undo();
+    return fail(OIDT_ENOMEM, "monitor allocation of " + std::to_string(rbytes) + " bytes failed");
+  }
+  if (m->mode == OIDT_MON_DFT) {
+    if (hipMallocAsync(&mh.im, cells * esz, b->stream) != hipSuccess ||
+        hipMallocAsync(reinterpret_cast<void**>(&mh.cs), (size_t)m->nrec * 2 * sizeof(double), b->stream) != hipSuccess) {
+      undo();
+      return fail(OIDT_ENOMEM, "monitor allocation failed");
+    }
+    if ((rc = upload_bytes(b, mh.cs, m->cs, (size_t)m->nrec * 2 * sizeof(double))) != OIDT_OK) {
+      undo();
+      return rc;
+    }
+    if (hipMemsetAsync(mh.im, 0, cells * esz, b->stream) != hipSuccess) {
+      undo();
+      return fail(OIDT_EHIP, "monitor clear failed");
+    }
+  }
+  if (hipMemsetAsync(mh.re, 0, rbytes, b->stream) != hipSuccess) {
+    undo();
+    return fail(OIDT_EHIP, "monitor clear failed");
+  }
+  sh.mons.push_back(mh);
+  b->dirty = true;
+  *monitor = (int32_t)sh.mons.size() - 1;
+  return OIDT_OK;
+}
+
+int oidt_batch_read_monitor(oidt_batch* b, int32_t slot, int32_t monitor, void* re, void* im, uint64_t nbytes) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  const SlotHost& sh = b->slots[slot];
+  if (monitor < 0 || monitor >= (int)sh.mons.size()) return fail(OIDT_EINVAL, "no such monitor");
+  const MonHost& mh = sh.mons[monitor];
+  const size_t esz = b->precision == OIDT_F64 ? sizeof(double) : sizeof(float);
+  const bool dft = mh.d.mode == OIDT_MON_DFT;
+  const size_t want = (dft ? mh.cells : mh.cells * (size_t)mh.d.nrec) * esz;
+  if (nbytes != want) return fail(OIDT_EINVAL, "buffer size must be " + std::to_string(want) + " bytes");
+  if (!re || (dft && !im)) return fail(OIDT_EINVAL, "null output");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  reap_marks(b);
+  HIP_TRY(hipMemcpy(re, mh.re, want, hipMemcpyDeviceToHost));
+  if (dft) HIP_TRY(hipMemcpy(im, mh.im, want, hipMemcpyDeviceToHost));
   return OIDT_OK;
 }
 
@@ -1050,8 +1215,15 @@ int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4) {
 int oidt_batch_destroy(oidt_batch* b) {
   if (!b) return OIDT_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (auto& sh : b->slots)
+  for (auto& sh : b->slots) {
     if (sh.dmem) (void)hipFree(sh.dmem);
+    for (auto& mh : sh.mons)
+      for (void* p : {mh.re, mh.im, static_cast<void*>(mh.cs)})
+        if (p) (void)hipFree(p);
+  }
+  if (b->d_mons) (void)hipFree(b->d_mons);
+  for (int h = 0; h < 2; ++h)
+    if (b->d_mwork[h]) (void)hipFree(b->d_mwork[h]);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->d_tiles) (void)hipFree(b->d_tiles);
@@ -1124,6 +1296,16 @@ int oidt_solver_read_dft(oidt_solver* s, int32_t port, int32_t comp, double* re,
 int oidt_solver_read_field(oidt_solver* s, int32_t field, void* host, uint64_t nbytes) {
   if (!s) return fail(OIDT_EINVAL, "null solver");
   return oidt_batch_read_field(s->b, s->slot, field, host, nbytes);
+}
+
+int oidt_solver_add_monitor(oidt_solver* s, const oidt_monitor_desc* m, int32_t* monitor) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_add_monitor(s->b, s->slot, m, monitor);
+}
+
+int oidt_solver_read_monitor(oidt_solver* s, int32_t monitor, void* re, void* im, uint64_t nbytes) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_read_monitor(s->b, s->slot, monitor, re, im, nbytes);
 }
 
 int oidt_solver_destroy(oidt_solver* s) {

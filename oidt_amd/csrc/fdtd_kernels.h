@@ -2526,6 +2526,49 @@ __global__ void k_advance(Slot<T>* __restrict__ slots, int nslots) {
   if (i < nslots && slots[i].active) slots[i].base += kChunk;
 }
 
+// Field monitors taken after one half-step: one work item = (monitor, first
+// of kThreads consecutive box cells, -, -).  Reads the current field set (the
+// two-kernel forms keep the state in set A).  DFT: the stencil sum is formed
+// in T point by point, scaled and weighted in double, and the running sum is
+// rounded back to T each step, as the reference's T accumulators are.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_monitor(const Slot<T>* __restrict__ slots,
+                                                      const Monitor<T>* __restrict__ mons,
+                                                      const int4* __restrict__ work, int nwork, int tstep) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const int4 it = work[w];
+    const Monitor<T>& m = mons[it.x];
+    const Slot<T>& s = slots[m.slot];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const int r = n + m.shift;
+    if (r < 0 || r >= m.nrec) continue;
+    const int cells = m.ex * m.ey * m.ez;
+    const int c = it.y + (int)threadIdx.x;
+    if (c >= cells) continue;
+    const int cz = c % m.ez, cxy = c / m.ez;
+    const int x = m.x0 + cxy / m.ey, y = m.y0 + cxy % m.ey, z = m.z0 + cz;
+    const T* __restrict__ f = s.f[m.field];
+    const bool is3d = s.scheme == SCHEME_3D;
+    T sum = T(0);
+    for (int q = 0; q < m.npts; ++q) {
+      const int xx = x + m.ox[q], yy = y + m.oy[q], zz = z + m.oz[q];
+      T v = T(0);
+      if (xx >= 0 && xx < m.gx && yy >= 0 && yy < m.gy && zz >= 0 && zz < m.gz)
+        v = f[is3d ? ((size_t)xx * s.ny + yy) * s.nz + zz : (size_t)xx * s.ny + yy];
+      sum = q == 0 ? v : sum + v;
+    }
+    const double v = m.scale * (double)sum;
+    if (m.mode == 0) {
+      const double cr = m.cs[2 * r], ci = m.cs[2 * r + 1];
+      m.re[c] = (T)((double)m.re[c] + v * cr);
+      m.im[c] = (T)((double)m.im[c] + v * ci);
+    } else {
+      m.re[(size_t)r * cells + c] = (T)v;
+    }
+  }
+}
+
 // Plain streaming copy (16 B per lane): calibrates the HBM traffic counters.
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ a, float4* __restrict__ c, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) c[i] = a[i];

@@ -70,4 +70,20 @@ struct Slot {
   const double* w;      // [nf] angular frequency 2*pi*f
 };
 
+// Field monitor (loss and impedance study drivers): the scaled stencil sum of
+// one field component at every cell of a box, folded into a single-frequency
+// running DFT (mode 0) or stored per step (mode 1).
+template <typename T>
+struct Monitor {
+  T* re;              // mode 0: real part [cells]; mode 1: records [nrec][cells]
+  T* im;              // mode 0: imaginary part [cells]
+  const double* cs;   // mode 0: (cos, sin) weight of each record [nrec][2]
+  double scale;
+  int slot, field, mode, npts;
+  int ox[4], oy[4], oz[4];
+  int x0, y0, z0, ex, ey, ez;   // box origin and extent
+  int gx, gy, gz;               // real grid extents: stencil points outside read 0
+  int shift, nrec;
+};
+
 }  // namespace oidt

@@ -111,8 +111,11 @@ class Setup:
         return self.nx * self.ny * (self.nz if self.scheme == '3d' else 1)
 
 
-def source_waveform(precision, nt, delta_t, source_type, source_amp, f0, wave_packet_bw, gauss_pulse_deg=-6):
-    """J_SRC and nt after the ramp extension (fdtd_auto_setup.py:121-159)."""
+def source_waveform(precision, nt, delta_t, source_type, source_amp, f0, wave_packet_bw, gauss_pulse_deg=-6,
+                    extend=True):
+    """J_SRC and nt after the ramp extension (fdtd_auto_setup.py:121-159).
+    ``extend=False``: no extension (the 2D loss and impedance study configs,
+    FDTD_2D_TE/scattering_loss/fdtd_config.py:115, 148-167)."""
     dt = float(delta_t)
     gp_tsp = np.sqrt(-2 * np.log(10 ** (gauss_pulse_deg / 20))) / (2 * np.pi * f0)
     gp_tpk = 9 * gp_tsp
@@ -121,7 +124,8 @@ def source_waveform(precision, nt, delta_t, source_type, source_amp, f0, wave_pa
     ext = {1: lambda: int(gp_tpk / nt), 2: lambda: int(wp_tpk / nt), 3: lambda: int((gp_tpk + gp_tsp) / dt)}
     if source_type not in ext:
         raise Exception('Pick a valid source type')
-    nt = nt + ext[source_type]()
+    if extend:
+        nt = nt + ext[source_type]()
     t = np.arange(nt, dtype=np.float64) * dt
     if source_type == 1:
         shape = np.exp(-0.5 * ((t - gp_tpk) / gp_tsp) ** 2).astype(precision)

@@ -19,7 +19,7 @@ from ._lib import check, lib, ptr
 class RunArrays:
     """Host arrays of one run, kept alive while the library reads them."""
 
-    def __init__(self, s, fg, direction='f', line=0):
+    def __init__(self, s, fg, direction='f', line=0, cb=None):
         if direction not in ('f', 'b'):
             raise ValueError("direction must be 'f' or 'b'")
         P = s.precision
@@ -32,7 +32,7 @@ class RunArrays:
         o = s.offset
         dims = (s.nx, s.ny, s.nz) if s.scheme == '3d' else (s.nx, s.ny)
         self.kappa = [M.kappa_axes(n, s.num_cpml, o, g['KE'], g['KH'], s.delta_x, P) for n in dims]
-        self.cb = [np.ascontiguousarray(a, dtype=P) for a in M.cb_maps(s, fg)]
+        self.cb = [np.ascontiguousarray(a, dtype=P) for a in (M.cb_maps(s, fg) if cb is None else cb)]
         self.jsrc = np.ascontiguousarray(s.J_SRC, dtype=P)
         self.freqs = np.ascontiguousarray(s.sparam_freqs, dtype=np.float64)
         self.db = P(float(s.delta_t) / s.mu)   # DB = delta_t / mu (fdtd_main.py:86), narrowed at launch
@@ -132,8 +132,8 @@ def _window_cells(s):
 class Solver:
     """One run on the device (a batch of one inside the library)."""
 
-    def __init__(self, setup, fg, direction='f', line=0):
-        self.arrays = RunArrays(setup, fg, direction, line)
+    def __init__(self, setup, fg, direction='f', line=0, cb=None):
+        self.arrays = RunArrays(setup, fg, direction, line, cb)
         self.setup = setup
         self._desc = self.arrays.desc()
         self._in = self.arrays.inputs()
@@ -180,6 +180,22 @@ class Solver:
         a = np.zeros(s.shape, dtype=s.precision)
         check(lib().oidt_solver_read_field(self.h, _lib.FIELD_ID[name], ptr(a), a.nbytes))
         return a
+
+    def add_monitor(self, mon):
+        """Attach a ``Monitor`` before the first advance; returns its index."""
+        d = mon.desc()
+        idx = C.c_int32()
+        check(lib().oidt_solver_add_monitor(self.h, C.byref(d), C.byref(idx)))
+        self.monitors = getattr(self, 'monitors', []) + [mon]
+        return idx.value
+
+    def read_monitor(self, idx):
+        """DFT monitor: complex re + 1j*im over the box (run precision parts);
+        capture monitor: [nrec, *box shape] records."""
+        mon = self.monitors[idx]
+        re, im = mon.buffers(self.setup.precision)
+        check(lib().oidt_solver_read_monitor(self.h, idx, ptr(re), ptr(im), re.nbytes))
+        return mon.result(re, im)
 
     def fields(self):
         return {k: self.field(k) for k in FIELDS[self.setup.scheme]}
@@ -288,6 +304,60 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Monitor:
+    """A field monitor (include/oidt_fdtd.h, oidt_monitor_desc): the stencil
+    sum of ``field`` over ``points`` (cell offsets), times ``scale``, at every
+    cell of ``box`` ((x0, x1), (y0, y1)[, (z0, z1)]), taken after the E
+    (``after_h`` False) or H half-step of every step n, as record r = n + shift.
+    ``cs`` ([nrec, 2] cos/sin weights) makes it a single-frequency DFT;
+    without it every record is kept."""
+
+    def __init__(self, field, points, scale, box, nrec, after_h=True, shift=0, cs=None):
+        self.field = field
+        self.points = [tuple(p) + (0,) * (3 - len(p)) for p in points]
+        self.scale = float(scale)
+        self.box = [tuple(b) for b in box] + [(0, 1)] * (3 - len(box))
+        self.nrec = int(nrec)
+        self.after_h = bool(after_h)
+        self.shift = int(shift)
+        self.cs = None if cs is None else np.ascontiguousarray(cs, dtype=np.float64).reshape(-1, 2)
+        if self.cs is not None and len(self.cs) != self.nrec:
+            raise ValueError('cs needs one (cos, sin) row per record')
+
+    @property
+    def extent(self):
+        return tuple(b1 - b0 for b0, b1 in self.box)
+
+    def desc(self):
+        d = _lib.MonitorDesc()
+        d.field = _lib.FIELD_ID[self.field]
+        d.after_h = int(self.after_h)
+        d.mode = _lib.MON_CAPTURE if self.cs is None else _lib.MON_DFT
+        d.npts = len(self.points)
+        for q, p in enumerate(self.points):
+            for a in range(3):
+                d.off[q][a] = p[a]
+        d.scale = self.scale
+        for a, (b0, b1) in enumerate(self.box):
+            d.box[2 * a], d.box[2 * a + 1] = b0, b1
+        d.shift = self.shift
+        d.nrec = self.nrec
+        d.cs = None if self.cs is None else self.cs.ctypes.data_as(C.c_void_p)
+        return d
+
+    def buffers(self, precision):
+        cells = int(np.prod(self.extent))
+        if self.cs is None:
+            return np.zeros(self.nrec * cells, dtype=precision), None
+        return np.zeros(cells, dtype=precision), np.zeros(cells, dtype=precision)
+
+    def result(self, re, im):
+        shape = tuple(e for e in self.extent)
+        if im is None:
+            return re.reshape((self.nrec,) + shape)
+        return re.reshape(shape), im.reshape(shape)
 
 
 def footprint(setup, direction='f', line=0):

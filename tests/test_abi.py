@@ -94,3 +94,30 @@ def test_padded_cell_count_limit():
     d = _desc(ny=64, nx=40_000_000)           # 2.6e9 cells, no padding needed
     d.port[0], d.port[1] = 20, 40
     assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.OK
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of the ABI structs have the C compiler's sizes and field
+    offsets (compiled against include/oidt_fdtd.h with the host gcc)."""
+    import shutil
+    import subprocess
+    if shutil.which('gcc') is None:
+        pytest.skip('no host C compiler')
+    structs = {'oidt_run_desc': _lib.RunDesc, 'oidt_run_inputs': _lib.RunInputs,
+               'oidt_monitor_desc': _lib.MonitorDesc, 'oidt_device_info': _lib.DeviceInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "oidt_fdtd.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f[0], cname, f[0]))
+    lines.append('return 0; }')
+    src = tmp_path / 'layout.c'
+    src.write_text('\n'.join(lines))
+    exe = tmp_path / 'layout'
+    subprocess.run(['gcc', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split('\n')
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, 'size')] == C.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])

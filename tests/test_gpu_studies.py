@@ -1,0 +1,89 @@
+"""Loss and wave-impedance study drivers on the device against the FP64 study
+oracle (oracle/studies.py), through the C ABI's field monitors.
+
+Tolerance: the study drivers run in float32 (their kernels are float32 only);
+device outputs are within relative L2 1e-4 of the FP64 oracle, the FP32 field
+bar.  The single-frequency DFTs also keep float32 running sums, as the
+reference's accumulators do."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oidt_amd import _lib, variants
+from oidt_amd.solver import Monitor, Solver
+from oracle import studies as ost
+
+from helpers import rel_l2
+from test_studies import SMALL_3D, SMALL_TE
+
+CASES = sorted(variants.DEFAULTS)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('scheme,study', CASES)
+def test_study_outputs_match_oracle(scheme, study):
+    c = variants.config(scheme, study, **(SMALL_TE if scheme == 'te' else SMALL_3D))
+    print('case', scheme, study, c.nx, c.ny, getattr(c, 'nz', 1), c.nt, flush=True)
+    dev = variants.run(c)
+    length = (c.p2x - c.p1x) if scheme == 'te' else (c.p2z - c.p1z)
+    up, lo = variants._profiles(c, length)
+    ref = ost.make(c, up, lo).run()
+    for k, v in dev.items():
+        r = ref[k]
+        assert v.shape == r.shape, (k, v.shape, r.shape)
+        assert v.dtype == (np.complex64 if study == 'scattering_loss' else np.float32), (k, v.dtype)
+        assert np.abs(r).max() > 0, k
+        err = rel_l2(v, r)
+        print(k, 'rel L2', err, flush=True)
+        assert err < 1e-4, (k, err)
+
+
+@pytest.mark.gpu
+def test_study_record_edges():
+    """Records outside the reference's loops stay zero: 3D lines end at
+    i = nx - 3 (i < ie), 3D plane DFTs cover [N, nx-N) x [N, ny-N) only."""
+    c = variants.config('3d', 'scattering_loss', **SMALL_3D)
+    c.nt = 96
+    c.cosE, c.sinE, c.cosH, c.sinH = (a[:96] for a in (c.cosE, c.sinE, c.cosH, c.sinH))
+    out = variants.run(c)
+    N = c.num_cpml
+    for v in out.values():
+        inner = np.zeros(v.shape, bool)
+        inner[N:c.nx - N, N:c.ny - N] = True
+        assert not np.any(v[~inner])
+    c = variants.config('3d', 'wave_impedance', **SMALL_3D)
+    c.nt = 96
+    for v in variants.run(c).values():
+        assert v.shape == (96, c.nx) and not np.any(v[:, c.nx - 2:])
+
+
+@pytest.mark.gpu
+def test_monitor_argument_checks():
+    c = variants.config('te', 'wave_impedance', **SMALL_TE)
+    s, fg, cb = variants.build(c)
+    sol = Solver(s, fg, cb=cb)
+    try:
+        bad_box = Monitor('ex', [(0, 0)], 1.0, [(0, c.nx + 1), (0, c.ny)], 4)
+        with pytest.raises(_lib.OidtError) as e:
+            sol.add_monitor(bad_box)
+        assert e.value.code == _lib.EINVAL
+        with pytest.raises(_lib.OidtError) as e:
+            sol.add_monitor(Monitor('ez', [(0, 0)], 1.0, [(0, 4), (0, 4)], 4))   # no Ez in TEz
+        assert e.value.code == _lib.EINVAL
+        with pytest.raises(_lib.OidtError) as e:
+            sol.add_monitor(Monitor('ex', [(0, 2)], 1.0, [(0, 4), (0, 4)], 4))   # offset beyond +-1
+        assert e.value.code == _lib.EINVAL
+        sol.add_monitor(Monitor('ex', [(0, 0)], 1.0, [(0, 4), (0, 4)], 4))
+        with pytest.raises(_lib.OidtError) as e:
+            sol.set_launch(8, 0, 3)                 # fused step has no point between the half-steps
+        assert e.value.code == _lib.ESTATE
+        sol.advance(32).sync()
+        with pytest.raises(_lib.OidtError) as e:
+            sol.add_monitor(Monitor('ex', [(0, 0)], 1.0, [(0, 4), (0, 4)], 4))   # run already started
+        assert e.value.code == _lib.ESTATE
+        re = np.zeros(4 * 16 + 1, np.float32)
+        rc = _lib.lib().oidt_solver_read_monitor(sol.h, 0, re.ctypes.data_as(C.c_void_p), None, re.nbytes)
+        assert rc == _lib.EINVAL                    # wrong buffer size
+    finally:
+        sol.close()
