@@ -41,8 +41,14 @@ typedef enum {
   OIDT_EFULL = -5     /* batch has no free slot */
 } oidt_status;
 
-/* Update schemes: FDTD_2D_TM, FDTD_2D_TE, FDTD_3D of the reference. */
-typedef enum { OIDT_TMZ = 0, OIDT_TEZ = 1, OIDT_3D = 2 } oidt_scheme;
+/* Update schemes: FDTD_2D_TM, FDTD_2D_TE, FDTD_3D of the reference, and the
+ * lossy 2D TMz form of the TM loss and wave-impedance study drivers
+ * (FDTD_2D_TM/scattering_loss/fdtd_funcs.py:77-152): offset 1, num_cpml =
+ * cpml_range, nf = 0, cb = {CURL_H, MOD_E} (at most 4 distinct values each),
+ * kappa_e / kappa_h = {DEN_EX, DEN_EY} / {DEN_HX, DEN_HY} (no dx factor),
+ * db = CURL_E; the H CPML scale is dt / mu with mu = 4e-7 pi; two-kernel forms
+ * only. */
+typedef enum { OIDT_TMZ = 0, OIDT_TEZ = 1, OIDT_3D = 2, OIDT_TMZ_LOSSY = 3 } oidt_scheme;
 
 /* Precision of fields, psi and coefficients ("precision" of user_config.py:65).
  * DFT accumulators are always double. */

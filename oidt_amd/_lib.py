@@ -13,10 +13,10 @@ LIB_PATH = os.path.join(HERE, 'liboidt_fdtd.so')
 
 ABI_VERSION = 1
 OK, EINVAL, EHIP, ENOMEM, ESTATE, EFULL = 0, -1, -2, -3, -4, -5
-TMZ, TEZ, YEE3D = 0, 1, 2
+TMZ, TEZ, YEE3D, TMZ_LOSSY = 0, 1, 2, 3
 F32, F64 = 0, 1
 EX, EY, EZ, HX, HY, HZ = range(6)
-SCHEME_ID = {'tm': TMZ, 'te': TEZ, '3d': YEE3D}
+SCHEME_ID = {'tm': TMZ, 'te': TEZ, '3d': YEE3D, 'tm_lossy': TMZ_LOSSY}
 FIELD_ID = {'ex': EX, 'ey': EY, 'ez': EZ, 'hx': HX, 'hy': HY, 'hz': HZ}
 
 

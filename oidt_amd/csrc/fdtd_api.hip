@@ -107,13 +107,13 @@ int field_count(int scheme) { return scheme == SCHEME_3D ? 6 : 3; }
 
 bool field_used(int scheme, int f) {
   if (scheme == SCHEME_3D) return true;
-  if (scheme == SCHEME_TMZ) return f == F_EZ || f == F_HX || f == F_HY;
+  if (scheme == SCHEME_TMZ || scheme == SCHEME_TMZ_LOSSY) return f == F_EZ || f == F_HX || f == F_HY;
   return f == F_EX || f == F_EY || f == F_HZ;
 }
 
 // psi (component, axis) pairs present per scheme, from the reference allocations.
 std::vector<std::pair<int, int>> psi_pairs(int scheme) {
-  if (scheme == SCHEME_TMZ) return {{F_EZ, 0}, {F_EZ, 1}, {F_HX, 1}, {F_HY, 0}};
+  if (scheme == SCHEME_TMZ || scheme == SCHEME_TMZ_LOSSY) return {{F_EZ, 0}, {F_EZ, 1}, {F_HX, 1}, {F_HY, 0}};
   if (scheme == SCHEME_TEZ) return {{F_EX, 1}, {F_EY, 0}, {F_HZ, 0}, {F_HZ, 1}};
   return {{F_EX, 1}, {F_EX, 2}, {F_EY, 0}, {F_EY, 2}, {F_EZ, 0}, {F_EZ, 1},
           {F_HX, 1}, {F_HX, 2}, {F_HY, 0}, {F_HY, 2}, {F_HZ, 0}, {F_HZ, 1}};
@@ -129,10 +129,15 @@ size_t psi_elems(const oidt_run_desc& d, int axis) {
 int validate(const oidt_run_desc* d) {
   if (!d) return fail(OIDT_EINVAL, "null run description");
   if (d->abi_version != OIDT_ABI_VERSION) return fail(OIDT_EINVAL, "abi_version mismatch");
-  if (d->scheme < 0 || d->scheme > 2) return fail(OIDT_EINVAL, "unknown scheme");
+  if (d->scheme < 0 || d->scheme > 3) return fail(OIDT_EINVAL, "unknown scheme");
   if (d->precision != OIDT_F32 && d->precision != OIDT_F64) return fail(OIDT_EINVAL, "unknown precision");
   const bool is3d = d->scheme == SCHEME_3D;
   if (!is3d && d->nz != 1) return fail(OIDT_EINVAL, "2D runs need nz == 1");
+  if (d->scheme == SCHEME_TMZ_LOSSY && (d->offset != 1 || d->nf != 0))
+    return fail(OIDT_EINVAL, "the lossy TMz form updates from offset 1 and has no port DFT (nf = 0)");
+  if (d->scheme == SCHEME_TMZ_LOSSY && d->src_field >= 0 &&
+      (d->src_box[0] < 1 || d->src_box[1] > d->nx - 1 || d->src_box[2] < 1 || d->src_box[3] > d->ny - 1))
+    return fail(OIDT_EINVAL, "lossy TMz source box must lie inside the update range");
   const int N = d->num_cpml, o = d->offset;
   if (N < 2 || o < 1 || N < o) return fail(OIDT_EINVAL, "need num_cpml >= offset >= 1 and num_cpml >= 2");
   const int dims[3] = {d->nx, d->ny, d->nz};
@@ -153,7 +158,8 @@ int validate(const oidt_run_desc* d) {
     if (d->port[p] < 0 || d->port[p] + 1 >= plim) return fail(OIDT_EINVAL, "port plane outside the grid");
   if (d->src_field < -1 || d->src_field > 5) return fail(OIDT_EINVAL, "bad src_field");
   if (d->src_field >= 0) {
-    const int allowed = d->scheme == SCHEME_TMZ ? F_EZ : (d->scheme == SCHEME_TEZ ? F_HZ : -2);
+    const int allowed = (d->scheme == SCHEME_TMZ || d->scheme == SCHEME_TMZ_LOSSY) ? F_EZ
+                        : (d->scheme == SCHEME_TEZ ? F_HZ : -2);
     if (is3d ? (d->src_field != F_EY && d->src_field != F_HY) : d->src_field != allowed)
       return fail(OIDT_EINVAL, "src_field not driven by this scheme in the reference");
     for (int a = 0; a < 3; ++a)
@@ -315,8 +321,10 @@ template <typename T>
 int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_run_inputs* in) {
   const Layout<T> L = layout<T>(d);
   const size_t cells = (size_t)d.nx * d.ny * d.nz;
-  const int ncb = d.scheme == SCHEME_TMZ ? 1 : (d.scheme == SCHEME_TEZ ? 2 : 3);
-  const int cb_comp[3][3] = {{2, -1, -1}, {0, 1, -1}, {0, 1, 2}};
+  // maps per scheme and the material-byte component each is indexed under;
+  // lossy TMz: {CURL_H, MOD_E}
+  const int ncb = d.scheme == SCHEME_TMZ ? 1 : (d.scheme == SCHEME_3D ? 3 : 2);
+  const int cb_comp[4][3] = {{2, -1, -1}, {0, 1, -1}, {0, 1, 2}, {2, 0, -1}};
   for (int c = 0; c < ncb; ++c)
     if (!in->cb[c]) return fail(OIDT_EINVAL, "missing cb map");
   const int naxes = d.scheme == SCHEME_3D ? 3 : 2;
@@ -431,6 +439,8 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   s.wx0 = L.wx0; s.wx1 = L.wx1; s.wy0 = L.wy0; s.wy1 = L.wy1;
   s.prow = L.prow; s.pcells = L.pcells; s.wcells = L.wcells; s.nplanes = L.nplanes; s.nq = L.nq;
   s.db = (T)d.db;
+  s.dxT = (T)d.dx;
+  s.dtmu = (T)d.dt / (T)kMu0;
   s.dt = d.dt;
   for (int f = 0; f < 6; ++f) s.f[f] = L.off_f[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f[f]);
   for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi[i]);
@@ -585,6 +595,10 @@ void launch_half(oidt_batch* b, int gs, int tstep, bool e_half) {
         if (v1) hipLaunchKernelGGL((k_tmz_h<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
         else hipLaunchKernelGGL((k_tmz_h2<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
       }
+      break;
+    case SCHEME_TMZ_LOSSY:
+      if (e_half) hipLaunchKernelGGL((k_tmzl_e<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
+      else hipLaunchKernelGGL((k_tmzl_h<T, XC>), grd, blk, 0, st, sl, b->d_tiles, b->d_counts, tstep);
       break;
     case SCHEME_TEZ:
       if (e_half) {
@@ -823,7 +837,7 @@ int oidt_run_footprint(const oidt_run_desc* desc, uint64_t* bytes) {
 int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt_batch** out) {
   if (!out) return fail(OIDT_EINVAL, "null output");
   *out = nullptr;
-  if (scheme < 0 || scheme > 2) return fail(OIDT_EINVAL, "unknown scheme");
+  if (scheme < 0 || scheme > 3) return fail(OIDT_EINVAL, "unknown scheme");
   if (precision != OIDT_F32 && precision != OIDT_F64) return fail(OIDT_EINVAL, "unknown precision");
   if (max_slots < 1 || max_slots > 4096) return fail(OIDT_EINVAL, "max_slots out of range");
   oidt_batch* b = new (std::nothrow) oidt_batch();
@@ -893,6 +907,8 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   if (variant == 3 && b->scheme == SCHEME_3D && (xchunk ? xchunk : b->xchunk) > kFused3dMaxXc)
     return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4, 8 or 16");
+  if (variant == 3 && b->scheme == SCHEME_TMZ_LOSSY)
+    return fail(OIDT_EINVAL, "the lossy TMz form has no fused step (variant 1 or 2)");
   if (variant == 3 && b->has_monitors())
     return fail(OIDT_ESTATE, "field monitors need a two-kernel form (variant 1 or 2)");
   b->variant = variant;
@@ -994,6 +1010,7 @@ int oidt_batch_port_power(oidt_batch* b, int32_t slot, int32_t port, int32_t sub
   const oidt_run_desc& d = sh.desc;
   if (port < 0 || port > 1) return fail(OIDT_EINVAL, "bad port");
   if (!re || !im) return fail(OIDT_EINVAL, "null output");
+  if (d.scheme == SCHEME_TMZ_LOSSY) return fail(OIDT_EINVAL, "the lossy TMz form has no port DFT");
   const SlotHost* ss = nullptr;
   if (sub_slot >= 0) {
     if ((rc = slot_ok(b, sub_slot)) != OIDT_OK) return rc;

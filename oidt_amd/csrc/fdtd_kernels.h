@@ -2526,6 +2526,102 @@ __global__ void k_advance(Slot<T>* __restrict__ slots, int nslots) {
   if (i < nslots && slots[i].active) slots[i].base += kChunk;
 }
 
+// ------------------------------------------------ 2D TMz, lossy study form ---
+// E half-step of the TM loss and wave-impedance study drivers
+// (FDTD/GPU/FDTD_2D_TM/scattering_loss/fdtd_funcs.py:77-83, 94-114, source
+// :147-152), on 0 < i < nx-1, 0 < j < ny-1:
+//   Ez = MOD_E Ez + CURL_H ((Hy[i]-Hy[i-1]) / DEN_EX[i] + (Hx[j-1]-Hx[j]) / DEN_EY[j]),
+// then the source, then on the CPML cells
+//   psi = BE[q] psi + CE[q]/dx (same difference),  Ez += CURL_H dx psi,
+// q = distance from the grid edge, 1 <= q < R (R = cpml_range = s.N), with
+// its own psi array on each side.  CURL_H / MOD_E are the material values of
+// components 2 / 0; DEN_* arrive as reciprocals in ike / ikh; ie = nx-1, je = ny-1.
+template <typename T, int XC>
+__global__ __launch_bounds__(kThreads) void k_tmzl_e(const Slot<T>* __restrict__ slots,
+                                                     const int4* __restrict__ tiles,
+                                                     const int* __restrict__ counts, int tstep) {
+  const int ntiles = counts[0];
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const int ny = s.ny, R = s.N, ie = s.ie, je = s.je;
+    const int j = tl.z + threadIdx.x;
+    if (j < 1 || j >= je) continue;
+    const int i0 = max(tl.y + (int)threadIdx.y * XC, 1);
+    const int i1 = min(tl.y + (int)threadIdx.y * XC + XC, ie);
+    T* __restrict__ Ez = s.f[F_EZ];
+    const T* __restrict__ Hx = s.f[F_HX];
+    const T* __restrict__ Hy = s.f[F_HY];
+    const T rdy = s.ike[1][j];
+    const int qy = j < R ? j : (je - j < R ? je - j : -1);
+    T* __restrict__ py = qy >= 0 ? s.psi[psi_id(F_EZ, 1, j < R ? 0 : 1)] : nullptr;
+    const bool jsrc = s.src_field == F_EZ && j >= s.sy0 && j < s.sy1;
+    const T jv = jsrc ? s.jsrc[n] : T(0);
+    for (int i = i0; i < i1; ++i) {
+      const size_t idx = (size_t)i * ny + j;
+      const uint8_t m = s.mat[idx];
+      const T ch = cbsel(s, m, 2);
+      const T me = cbsel(s, m, 0);
+      const T dHy = Hy[idx] - Hy[idx - ny];
+      const T dHx = Hx[idx - 1] - Hx[idx];
+      T v = me * Ez[idx] + ch * (dHy * s.ike[0][i] + dHx * rdy);
+      if (jsrc && i >= s.sx0 && i < s.sx1) v -= jv;
+      const T chdx = ch * s.dxT;
+      const int qx = i < R ? i : (ie - i < R ? ie - i : -1);
+      if (qx >= 0)
+        v += chdx * psi_upd(s.psi[psi_id(F_EZ, 0, i < R ? 0 : 1)] + (size_t)qx * ny + j, s.be[qx], s.ceo[qx], dHy);
+      if (qy >= 0) v += chdx * psi_upd(py + (size_t)i * R + qy, s.be[qy], s.ceo[qy], dHx);
+      Ez[idx] = v;
+    }
+  }
+}
+
+// H half-step of the same form (fdtd_funcs.py:85-92, 116-136), on i < nx-1,
+// j < ny-1:
+//   Hx += CURL_E (Ez[j]-Ez[j+1]) / DEN_HY[j],  Hy += CURL_E (Ez[i+1]-Ez[i]) / DEN_HX[i],
+// then psi = BH[q] psi + CH[q]/dx (same difference), H += (dt/mu) psi on the
+// CPML cells: low q = i, high q = nx-2-i (likewise in y), 1 <= q < R.
+template <typename T, int XC>
+__global__ __launch_bounds__(kThreads) void k_tmzl_h(const Slot<T>* __restrict__ slots,
+                                                     const int4* __restrict__ tiles,
+                                                     const int* __restrict__ counts, int tstep) {
+  const int ntiles = counts[0];
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const int ny = s.ny, R = s.N, ie = s.ie, je = s.je;
+    const int j = tl.z + threadIdx.x;
+    if (j >= je) continue;
+    const int i0 = tl.y + (int)threadIdx.y * XC;
+    const int i1 = min(i0 + XC, ie);
+    const T* __restrict__ Ez = s.f[F_EZ];
+    T* __restrict__ Hx = s.f[F_HX];
+    T* __restrict__ Hy = s.f[F_HY];
+    const T ce = s.db, dtmu = s.dtmu;
+    const T rdy = s.ikh[1][j];
+    const int qy = (j >= 1 && j < R) ? j : (je - 1 - j >= 1 && je - 1 - j < R ? je - 1 - j : -1);
+    T* __restrict__ py = qy >= 0 ? s.psi[psi_id(F_HX, 1, j < R ? 0 : 1)] : nullptr;
+    for (int i = i0; i < i1; ++i) {
+      const size_t idx = (size_t)i * ny + j;
+      const T ez = Ez[idx];
+      const T dEy = ez - Ez[idx + 1];
+      const T dEx = Ez[idx + ny] - ez;
+      T hx = Hx[idx] + ce * dEy * rdy;
+      T hy = Hy[idx] + ce * dEx * s.ikh[0][i];
+      const int qx = (i >= 1 && i < R) ? i : (ie - 1 - i >= 1 && ie - 1 - i < R ? ie - 1 - i : -1);
+      if (qx >= 0)
+        hy += dtmu * psi_upd(s.psi[psi_id(F_HY, 0, i < R ? 0 : 1)] + (size_t)qx * ny + j, s.bh[qx], s.cho[qx], dEx);
+      if (qy >= 0) hx += dtmu * psi_upd(py + (size_t)i * R + qy, s.bh[qy], s.cho[qy], dEy);
+      Hx[idx] = hx;
+      Hy[idx] = hy;
+    }
+  }
+}
+
 // Field monitors taken after one half-step: one work item = (monitor, first
 // of kThreads consecutive box cells, -, -).  Reads the current field set (the
 // two-kernel forms keep the state in set A).  DFT: the stencil sum is formed

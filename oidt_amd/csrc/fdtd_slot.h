@@ -20,9 +20,13 @@ constexpr int kFB = 16;
 // Distinct cb values per E component (2-bit index per component in the
 // material byte).
 constexpr int kMatMax = 4;
+// Vacuum permeability of the reference configurations (mu = 4e-7 pi).
+constexpr double kMu0 = 4.0e-7 * 3.14159265358979323846;
 constexpr int kFused3dMaxXc = 16;  // x-planes of a fused 3D step tile (4 rows x 16 planes: one halo cell per lane)
 
-enum : int { SCHEME_TMZ = 0, SCHEME_TEZ = 1, SCHEME_3D = 2 };
+// SCHEME_TMZ_LOSSY: the 2D TMz update form of the loss and wave-impedance
+// study drivers (lossy E update, CPML over the whole border).
+enum : int { SCHEME_TMZ = 0, SCHEME_TEZ = 1, SCHEME_3D = 2, SCHEME_TMZ_LOSSY = 3 };
 enum : int { F_EX = 0, F_EY = 1, F_EZ = 2, F_HX = 3, F_HY = 4, F_HZ = 5 };
 
 // psi arrays are addressed as psi[(component * 3 + axis) * 2 + side]
@@ -51,6 +55,8 @@ struct Slot {
   int nplanes;             // raw planes per step
   int nq;                  // DFT sample components
   T db;
+  T dxT;    // delta_x in the run precision (lossy TMz: curl_h * dx)
+  T dtmu;   // delta_t / mu in the run precision (lossy TMz H CPML)
   double dt;
   T* f[6];       // field set A (current state of the two-kernel forms)
   T* psi[36];

@@ -8,6 +8,7 @@ port phasors in the reference's layout (``EZF1 = EZ1r + 1j*EZ1i``,
 fdtd_main.py:359-363; 3D [nf, nx, ny], FDTD_3D fdtd_main.py:462-470).
 """
 import ctypes as C
+from types import SimpleNamespace
 
 import numpy as np
 
@@ -140,6 +141,24 @@ class Solver:
         h = C.c_void_p()
         check(lib().oidt_solver_create(C.byref(self._desc), C.byref(self._in), C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_arrays(cls, arrays, nt, precision=np.float32, shape=None):
+        """A run from any object with ``desc()`` / ``inputs()`` (the study
+        drivers' own array sets); ``setup`` then only carries nt, precision
+        and the grid shape."""
+        self = cls.__new__(cls)
+        self.arrays = arrays
+        self._desc = arrays.desc()
+        self._in = arrays.inputs()
+        d = self._desc
+        self.setup = SimpleNamespace(nt=nt, precision=precision,
+                                     shape=shape or ((d.nx, d.ny, d.nz) if d.nz > 1 else (d.nx, d.ny)),
+                                     scheme=None, sparam_freqs=np.zeros(0))
+        h = C.c_void_p()
+        check(lib().oidt_solver_create(C.byref(self._desc), C.byref(self._in), C.byref(h)))
+        self.h = h
+        return self
 
     def set_launch(self, xchunk=0, grid_blocks=0, variant=2):
         """Kernel form and x-chunk depth (before the first advance)."""

@@ -30,18 +30,25 @@ E half-step with record index step - 1.  The study's Ey x-CPML kernel
 (fdtd_funcs.py:135-144) negates both the difference and the correction of
 the S-parameter kernel; psi changes sign and E is unchanged.
 
-The 2D TM study drivers use a different discretisation (lossy update form,
-CPML over the whole border, FDTD/GPU/FDTD_2D_TM/scattering_loss/fdtd_funcs.py:
-77-136) and are not covered here: 2D TMz studies stay on the reference.
+TMz.  The 2D TM study drivers use their own discretisation (lossy update
+form, CPML graded over the whole border with a kappa divisor,
+FDTD/GPU/FDTD_2D_TM/scattering_loss/fdtd_funcs.py:77-152): the library's
+lossy TMz scheme (``oidt_scheme`` OIDT_TMZ_LOSSY) with the arrays of
+fdtd_main.py:38-119, built by ``LossyTmArrays``.  Its config:
+FDTD/GPU/FDTD_2D_TM/scattering_loss/fdtd_config.py:95-177.
 """
 import os
 from types import SimpleNamespace
 
 import numpy as np
 
+import ctypes as C
+
+from . import _lib
 from . import materials as M
 from . import params
 from . import roughness
+from ._lib import ptr
 from .solver import Monitor, Solver
 
 STUDIES = ('scattering_loss', 'wave_impedance')
@@ -62,7 +69,17 @@ _3D = dict(eps_rel_bg=1.5 ** 2, eps_rel_fg=3.5 ** 2, sgl_wg_length=24.0e-6, sgl_
            f0=194.8e12, source_condition='TE', harmonics=1.0, num_flights=2.0, points_per_wl=30,
            num_cpml=20, buffer_zhat=20, nx_clad_wl=0.75, ny_clad_wl=0.75)
 
+_TM = dict(eps_rel_bg=1.5 ** 2, eps_rel_fg=3.5 ** 2, sgl_wg_length=30e-6, sgl_wg_width=200e-9, port_length=5e-6,
+           rough_std=15.0e-9, rough_acl=700.0e-9, tol_std=10.0, tol_acl=10.0, rough_toggle=True, ctype=3,
+           output_dir='Results', ez_fft_name='ez_fft', hx_fft_name='hx_fft', hy_fft_name='hy_fft',
+           profile_number=0, source_type=1, source_amp=5.0, wave_packet_bw=0.8, gauss_pulse_deg=-6,
+           f0=194.8e12, harmonics=1.0, num_flights=2.0, points_per_wl=40, num_cpml=20, buffer_xhat=10,
+           ny_clad_wl=2.0)
+
 DEFAULTS = {
+    ('tm', 'scattering_loss'): _TM,
+    ('tm', 'wave_impedance'): {**{k: v for k, v in _TM.items() if not k.endswith('_fft_name')},
+                               'rough_toggle': False},
     ('te', 'scattering_loss'): _TE,
     ('te', 'wave_impedance'): {**{k: v for k, v in _TE.items() if not k.endswith('_fft_name')},
                                'sgl_wg_length': 25.0e-6, 'rough_toggle': False},
@@ -104,7 +121,26 @@ def config(scheme, study, **over):
     dx, dt = float(c.delta_x), float(c.delta_t)
     nt = int(c.sim_time / dt)
     N = c.num_cpml
-    if scheme == 'te':
+    if scheme == 'tm':
+        c.buffer_yhat = int(c.ny_clad_wl * c.wl_clad / dx)
+        c.cpml_range = N + 1
+        c.nx_swg = int(c.sgl_wg_length / dx)
+        c.ny_swg = int(c.sgl_wg_width / dx)
+        c.bx = N + c.buffer_xhat
+        c.by = N + c.buffer_yhat
+        c.nx = 2 * c.bx + c.nx_swg
+        c.ny = 2 * c.by + c.ny_swg
+        c.nt = nt
+        c.cx, c.cy = c.nx // 2, c.ny // 2
+        c.cx_swg, c.cy_swg = c.nx_swg // 2, c.ny_swg // 2
+        c.sx = c.bx + 0
+        c.sy = c.by + c.cy_swg
+        c.J_SRC = tm_waveform(c)
+        c.p1x = c.sx + int(c.port_length / dx)
+        c.p2x = c.nx - c.p1x
+        if c.p2x <= c.p1x:
+            raise Exception('Port settling ranges overlap. Make the WG longer or decrease port settling range!')
+    elif scheme == 'te':
         c.buffer_yhat = int(c.ny_clad_wl * c.wl_clad / dx)
         c.nx_swg = int(c.sgl_wg_length / dx)
         c.ny_swg = int(c.sgl_wg_width / dx)
@@ -150,6 +186,116 @@ def config(scheme, study, **over):
     c.up = './rough_profiles/profile_upper' + c.roughness_profile
     c.lp = './rough_profiles/profile_lower' + c.roughness_profile
     return c
+
+
+def tm_waveform(c):
+    """J_SRC of FDTD_2D_TM/scattering_loss/fdtd_config.py:145-174: pulse peaks
+    at 5 spreads, no step extension, the ramped harmonic scaled by dx**2."""
+    dt = float(c.delta_t)
+    gp_tsp = np.sqrt(-2 * np.log(10 ** (c.gauss_pulse_deg / 20))) / (2 * np.pi * c.f0)
+    gp_tpk = 5 * gp_tsp
+    wp_tsp = 2 * np.sqrt(np.log(2)) / (c.wave_packet_bw * 2 * np.pi * c.f0)
+    wp_tpk = 5 * wp_tsp
+    t = np.arange(c.nt, dtype=np.float64) * dt
+    f32 = np.float32
+    if c.source_type == 1:
+        return (c.source_amp * np.exp(-0.5 * ((t - gp_tpk) / gp_tsp) ** 2).astype(f32)).astype(f32)
+    if c.source_type == 2:
+        wp = (np.exp(-0.5 * ((t - wp_tpk) / wp_tsp) ** 2) * np.exp(2j * np.pi * c.f0 * (t - wp_tpk)).real).astype(f32)
+        return (c.source_amp * wp).astype(f32)
+    if c.source_type != 3:
+        raise Exception('Pick a valid source type')
+    gp = np.exp(-0.5 * ((t - gp_tpk) / gp_tsp) ** 2).astype(f32)
+    ramp = np.zeros(c.nt, f32)
+    acc = f32(0)
+    for n in range(c.nt):
+        acc = f32(acc + c.delta_t * gp[n])
+        ramp[n] = acc
+    ramp = ramp / np.max(ramp)
+    th = np.exp(2j * np.pi * c.f0 * (t - gp_tpk)).real.astype(f32)
+    amp = f32(c.source_amp * float(c.delta_x * c.delta_x))   # float64 scalar product, applied in float32
+    return ((amp * ramp) * th).astype(f32)
+
+
+def lossy_tm_coefficients(c, fg):
+    """CURL_E, CURL_H, MOD_E, BE, CE, BH, CH, DEN_EX/EY/HX/HY of
+    FDTD_2D_TM/scattering_loss/fdtd_main.py:38-119 with numpy 1.20 element
+    types (float32 arrays met by scalars stay float32; scalar-scalar
+    arithmetic is float64)."""
+    f32 = np.float32
+    dt, dx = f32(c.delta_t), f32(c.delta_x)
+    sigma = np.zeros((c.nx, c.ny), f32)
+    eps_mask = (c.eps * (c.eps_rel_fg * fg + 1 * ~fg)).astype(f32)
+    out = {'CURL_E': f32(float(dt) / (c.mu * float(dx)))}
+    den = (dx * (f32(2) * eps_mask + sigma * dt)).astype(f32)
+    out['CURL_H'] = (f32(2 * float(dt)) / den).astype(f32)
+    out['MOD_E'] = ((f32(2) * eps_mask - sigma * dt) / (f32(2) * eps_mask + sigma * dt)).astype(f32)
+    N, R = c.num_cpml, c.cpml_range
+    m_grade, m_alpha = 4, 1
+    sigma_max = 1.2 * (0.8 * (m_grade + 1) / (float(c.eta) * float(dx)))
+    alpha_max, kappa_max = 0.05, 5
+    A = {k: np.zeros(R, f32) for k in ('AE', 'KE', 'SE', 'AH', 'KH', 'SH', 'BE', 'CE', 'BH', 'CH')}
+    den_ex, den_ey = np.ones(c.nx, f32), np.ones(c.ny, f32)
+    den_hx, den_hy = np.ones(c.nx, f32), np.ones(c.ny, f32)
+    for q in range(N):
+        for t, pos in (('E', q), ('H', q + 0.5)):
+            off = 1 if t == 'E' else 1.5
+            A['A' + t][q] = alpha_max * (pos / N) ** m_alpha
+            A['K' + t][q] = 1 + (kappa_max - 1) * ((R - q - off) / N) ** m_grade
+            A['S' + t][q] = sigma_max * ((R - q - off) / N) ** m_grade
+            S_, K_, A_ = A['S' + t][q], A['K' + t][q], A['A' + t][q]
+            A['B' + t][q] = np.exp(-1.0 * float(S_ / K_ + A_) * float(dt) / c.eps)
+            A['C' + t][q] = float(S_ / (S_ + K_ * A_) / K_) * (float(A['B' + t][q]) - 1.0)
+        den_ex[q] = den_ex[c.nx - 1 - q] = A['KE'][q]
+        den_ey[q] = den_ey[c.ny - 1 - q] = A['KE'][q]
+        den_hx[q] = den_hx[c.nx - 2 - q] = A['KH'][q]
+        den_hy[q] = den_hy[c.ny - 2 - q] = A['KH'][q]
+    out.update({k: A[k] for k in ('BE', 'CE', 'BH', 'CH')})
+    out.update(DEN_EX=den_ex, DEN_EY=den_ey, DEN_HX=den_hx, DEN_HY=den_hy)
+    return out
+
+
+class LossyTmArrays:
+    """Run description and arrays of a lossy TMz study run (C ABI scheme
+    OIDT_TMZ_LOSSY); kept alive while the library reads them."""
+
+    def __init__(self, c, fg):
+        self.c = c
+        self.k = lossy_tm_coefficients(c, fg)
+        self.cb = [np.ascontiguousarray(self.k['CURL_H']), np.ascontiguousarray(self.k['MOD_E'])]
+        self.jsrc = np.ascontiguousarray(c.J_SRC[:c.nt], np.float32)
+
+    def desc(self):
+        c = self.c
+        d = _lib.RunDesc()
+        d.abi_version = _lib.ABI_VERSION
+        d.scheme = _lib.TMZ_LOSSY
+        d.precision = _lib.F32
+        d.nx, d.ny, d.nz = c.nx, c.ny, 1
+        d.num_cpml = c.cpml_range
+        d.offset = 1
+        d.nt = c.nt
+        d.nf = 0
+        d.src_field = _lib.EZ
+        box = (c.sx, c.sx + 1, c.by, c.ny - c.by, 0, 1)   # update_source of fdtd_main.py:202
+        for a in range(6):
+            d.src_box[a] = box[a]
+        d.port[0], d.port[1] = c.p1x, c.p2x
+        d.dx = float(c.delta_x)
+        d.dt = float(c.delta_t)
+        d.db = float(self.k['CURL_E'])
+        return d
+
+    def inputs(self):
+        k = self.k
+        r = _lib.RunInputs()
+        r.cb[0], r.cb[1] = ptr(self.cb[0]), ptr(self.cb[1])
+        r.kappa_e[0], r.kappa_e[1] = ptr(k['DEN_EX']), ptr(k['DEN_EY'])
+        r.kappa_h[0], r.kappa_h[1] = ptr(k['DEN_HX']), ptr(k['DEN_HY'])
+        r.be, r.ce, r.bh, r.ch = ptr(k['BE']), ptr(k['CE']), ptr(k['BH']), ptr(k['CH'])
+        r.jsrc = ptr(self.jsrc)
+        r.freqs = None
+        return r
 
 
 def phase_tables_f32(f0, delta_t, nt):
@@ -212,6 +358,19 @@ H_AVG = {'hx': ([(0, 0, 0), (0, 1, 0), (0, 0, 1), (0, 1, 1)], 0.25),
 def monitors(c):
     """{output name: Monitor} of the study, with the reference's array names."""
     nt = c.nt
+    if c.scheme == 'tm':
+        if c.study == 'scattering_loss':
+            # simul_fft of Ez at n + 0.5, Hx and Hy at n + 1.0 after the whole step
+            # (FDTD_2D_TM/scattering_loss/fdtd_main.py:216-218)
+            box = [(0, c.nx), (0, c.ny)]
+            return {k: Monitor(k, [(0, 0)], 1.0, box, nt, after_h=True,
+                               cs=phases_f64(c.f0, c.delta_t, np.arange(nt) + (0.5 if k == 'ez' else 1.0)))
+                    for k in ('ez', 'hx', 'hy')}
+        # map_efield_zwave / map_hfield_zwave at x = cx (FDTD_2D_TM/wave_impedance/
+        # fdtd_funcs.py:155-168); Hx[cx, -1] wraps to Hx[cx, ny-1], a cell no update writes
+        box = [(c.cx, c.cx + 1), (0, c.ny)]
+        return {'ez': Monitor('ez', [(0, 0)], 1.0, box, nt, after_h=True),
+                'hx': Monitor('hx', [(0, 0), (0, -1)], 0.5, box, nt, after_h=True)}
     if c.scheme == 'te':
         if c.study == 'scattering_loss':
             # simul_fft of Hz at step n + 0.0 and Ey at n + 0.5 over the whole grid
@@ -254,6 +413,9 @@ def monitors(c):
 
 def build(c):
     """(setup, fg mask, cb maps or None) of a study config."""
+    if c.scheme == 'tm':
+        up, lo = _profiles(c, c.p2x - c.p1x)
+        return None, tm_mask(c, up, lo), None
     s = setup_of(c)
     if c.scheme == 'te':
         up, lo = _profiles(c, c.p2x - c.p1x)
@@ -263,11 +425,22 @@ def build(c):
     return s, M.fg_mask(s, up, lo), None
 
 
+def tm_mask(c, upper, lower):
+    """FG_REG of gen_fg_mask_sgl (FDTD_2D_TM/scattering_loss/fdtd_funcs.py:
+    32-74): core iff by + lower[i] <= j < by + ny_swg + upper[i], profiles
+    padded by p1x zeros at both ends."""
+    L = c.p2x - c.p1x
+    up = M.padded(np.zeros(L) if upper is None else upper, c.p1x, c.nx - c.p2x)
+    lo = M.padded(np.zeros(L) if lower is None else lower, c.p1x, c.nx - c.p2x)
+    j = np.arange(c.ny, dtype=np.float64)[None, :]
+    return (j >= c.by + lo[:, None]) & (j < c.by + c.ny_swg + up[:, None])
+
+
 def run(c, launch=(2, 0, 0)):
     """Step the study on the device; returns {name: array} in the reference's
     output shapes and types (complex64 phasors, float32 time records)."""
     s, fg, cb = build(c)
-    sol = Solver(s, fg, cb=cb)
+    sol = Solver.from_arrays(LossyTmArrays(c, fg), c.nt) if c.scheme == 'tm' else Solver(s, fg, cb=cb)
     try:
         variant, xchunk, grid = launch
         if variant == 3:
@@ -282,6 +455,12 @@ def run(c, launch=(2, 0, 0)):
 
 
 def _output(c, key, mon, val):
+    if c.scheme == 'tm':
+        if c.study == 'scattering_loss':
+            re, im = val
+            # dEZfr.copy_to_host() - 1j*dEZfi.copy_to_host()  (fdtd_main.py:260-262)
+            return (re - 1j * im.astype(np.complex64)).astype(np.complex64)[:, :, 0]
+        return val.reshape(c.nt, c.ny)
     if c.scheme == 'te':
         if c.study == 'scattering_loss':
             re, im = val
@@ -302,6 +481,10 @@ def _output(c, key, mon, val):
 
 def output_names(c):
     """{output key: .npy file stem} as the reference's np.save calls."""
+    if c.scheme == 'tm':
+        if c.study == 'scattering_loss':
+            return {k: getattr(c, k + '_fft_name') + c.roughness_profile for k in ('ez', 'hx', 'hy')}
+        return {'ez': 'ez_zwave', 'hx': 'hx_zwave'}
     if c.scheme == 'te':
         if c.study == 'scattering_loss':
             return {'hz': c.hz_fft_name + c.roughness_profile, 'ey': c.ey_fft_name + c.roughness_profile}
@@ -326,7 +509,7 @@ def save(c, outputs, workdir='.'):
 
 
 def main(argv=None):
-    """``python -m oidt_amd.variants <te|3d> <scattering_loss|wave_impedance> [key=value ...]``."""
+    """``python -m oidt_amd.variants <tm|te|3d> <scattering_loss|wave_impedance> [key=value ...]``."""
     import sys
     argv = sys.argv[1:] if argv is None else argv
     if len(argv) < 2:

@@ -9,6 +9,10 @@ group in the reference launch order:
                update_pml_ex_yinc, update_pml_ey_xinc, then recording);
                recording simul_fft (:113-120) or map_*_zwave
                (FDTD_2D_TE/wave_impedance/fdtd_funcs.py:169-181, loop :220-221).
+* TMz studies  FDTD/GPU/FDTD_2D_TM/scattering_loss/fdtd_funcs.py:77-152 (lossy
+               update form), loop fdtd_main.py:197-218, recording simul_fft
+               (:138-144) or map_*_zwave (FDTD_2D_TM/wave_impedance/
+               fdtd_funcs.py:155-168, loop :210-211); arrays fdtd_main.py:38-119.
 * 3D studies   the S-parameter 3D step (``fdtd.Yee3D``; the study kernels are
                the same statements, FDTD_3D/scattering_loss/fdtd_funcs.py:
                15-215) followed by simul_fft_efield / simul_fft_hfield at both
@@ -219,5 +223,149 @@ class Yee3DStudy(F.Yee3D):
         return self.rec
 
 
+
+
+def lossy_tm_coefficients(cfg, FG):
+    """Arrays of FDTD_2D_TM/scattering_loss/fdtd_main.py:38-119, literal loop
+    (numpy 1.20 element types written out with float())."""
+    f32 = np.float32
+    dt, dx = f32(cfg.delta_t), f32(cfg.delta_x)
+    SIGMA_MASK = np.zeros([cfg.nx, cfg.ny], dtype=f32)
+    EPS_MASK = (cfg.eps * (cfg.eps_rel_fg * FG + 1 * ~FG)).astype(f32)
+    CURL_E = f32(float(dt) / (cfg.mu * float(dx)))
+    CURL_H = f32(2 * float(dt)) / (dx * (f32(2) * EPS_MASK + SIGMA_MASK * dt))
+    MOD_E = (f32(2) * EPS_MASK - SIGMA_MASK * dt) / (f32(2) * EPS_MASK + SIGMA_MASK * dt)
+    N, R = cfg.num_cpml, cfg.num_cpml + 1
+    sigma_max = 1.2 * (0.8 * (4 + 1) / (float(cfg.eta) * float(dx)))
+    AE, KE, SE, AH, KH, SH, BE, CE, BH, CH = (np.zeros(R, f32) for _ in range(10))
+    DEN_EX, DEN_EY, DEN_HX, DEN_HY = (np.ones(cfg.nx, f32), np.ones(cfg.ny, f32), np.ones(cfg.nx, f32),
+                                      np.ones(cfg.ny, f32))
+    for q in range(N):
+        AE[q] = 0.05 * (q / N) ** 1
+        KE[q] = 1 + (5 - 1) * ((R - q - 1) / N) ** 4
+        SE[q] = sigma_max * ((R - q - 1) / N) ** 4
+        BE[q] = np.exp(-1 * float(SE[q] / KE[q] + AE[q]) * float(dt) / cfg.eps)
+        CE[q] = float(SE[q] / (SE[q] + KE[q] * AE[q]) / KE[q]) * (float(BE[q]) - 1)
+        AH[q] = 0.05 * ((q + 0.5) / N) ** 1
+        KH[q] = 1 + (5 - 1) * ((R - q - 1.5) / N) ** 4
+        SH[q] = sigma_max * ((R - q - 1.5) / N) ** 4
+        BH[q] = np.exp(-1 * float(SH[q] / KH[q] + AH[q]) * float(dt) / cfg.eps)
+        CH[q] = float(SH[q] / (SH[q] + KH[q] * AH[q]) / KH[q]) * (float(BH[q]) - 1)
+        DEN_EX[q] = KE[q]
+        DEN_EX[cfg.nx - 1 - q] = KE[q]
+        DEN_EY[q] = KE[q]
+        DEN_EY[cfg.ny - 1 - q] = KE[q]
+        DEN_HX[q] = KH[q]
+        DEN_HX[cfg.nx - 2 - q] = KH[q]
+        DEN_HY[q] = KH[q]
+        DEN_HY[cfg.ny - 2 - q] = KH[q]
+    return dict(CURL_E=CURL_E, CURL_H=CURL_H, MOD_E=MOD_E, BE=BE, CE=CE, BH=BH, CH=CH, DEN_EX=DEN_EX,
+                DEN_EY=DEN_EY, DEN_HX=DEN_HX, DEN_HY=DEN_HY)
+
+
+def tm_mask(cfg, upper=None, lower=None):
+    """gen_fg_mask_sgl (FDTD_2D_TM/scattering_loss/fdtd_funcs.py:32-74), literal loop."""
+    neff = cfg.nx - 2 * cfg.p1x
+    upper = np.zeros(neff) if upper is None else np.asarray(upper, np.float64)
+    lower = np.zeros(neff) if lower is None else np.asarray(lower, np.float64)
+    up = np.append(np.append(np.zeros(cfg.p1x), upper), np.zeros(cfg.p1x))
+    lo = np.append(np.append(np.zeros(cfg.p1x), lower), np.zeros(cfg.p1x))
+    fg = np.zeros([cfg.nx, cfg.ny], dtype=bool)
+    for j in range(cfg.ny):
+        for i in range(cfg.nx):
+            if j >= (cfg.by + lo[i]) and j < (cfg.by + cfg.ny_swg + up[i]):
+                fg[i, j] = True
+    return fg
+
+
+class TMzLossyStudy:
+    """2D TM study loop (FDTD_2D_TM/scattering_loss/fdtd_main.py:197-218,
+    kernels fdtd_funcs.py:77-152), FP64, vectorised."""
+
+    def __init__(self, cfg, upper=None, lower=None, FG=None):
+        self.cfg = cfg
+        self.FG = tm_mask(cfg, upper, lower) if FG is None else FG
+        k = lossy_tm_coefficients(cfg, self.FG)
+        f = np.float64
+        self.k = {n: (np.asarray(v).astype(f) if np.ndim(v) else float(v)) for n, v in k.items()}
+        self.R = cfg.num_cpml + 1
+        nx, ny, R = cfg.nx, cfg.ny, self.R
+        z = lambda *s: np.zeros(s)
+        self.Ez, self.Hx, self.Hy = z(nx, ny), z(nx, ny), z(nx, ny)
+        self.p_ez_xlo, self.p_ez_xhi, self.p_hy_xlo, self.p_hy_xhi = z(R, ny), z(R, ny), z(R, ny), z(R, ny)
+        self.p_ez_ylo, self.p_ez_yhi, self.p_hx_ylo, self.p_hx_yhi = z(nx, R), z(nx, R), z(nx, R), z(nx, R)
+        self.J = np.asarray(cfg.J_SRC, np.float32).astype(f)
+        self.dx, self.dt, self.mu = float(np.float32(cfg.delta_x)), float(np.float32(cfg.delta_t)), float(
+            np.float32(cfg.mu))
+        self.rec = {}
+
+    def step(self, n):
+        cfg, k, R, dx = self.cfg, self.k, self.R, self.dx
+        Ez, Hx, Hy = self.Ez, self.Hx, self.Hy
+        nx, ny = cfg.nx, cfg.ny
+        ch, me = k['CURL_H'], k['MOD_E']
+        I, J = slice(1, nx - 1), slice(1, ny - 1)
+        # update_ez
+        Ez[I, J] = me[I, J] * Ez[I, J] + ch[I, J] * ((Hy[I, J] - Hy[0:nx - 2, J]) / k['DEN_EX'][I, None]
+                                                     + (Hx[I, 0:ny - 2] - Hx[I, J]) / k['DEN_EY'][None, J])
+        # update_source: Ez[sx, by:ny-by] -= J_SRC[n]
+        Ez[cfg.sx, cfg.by:ny - cfg.by] -= self.J[n]
+        q = np.arange(1, R)
+        be, ce = k['BE'][q], k['CE'][q]
+        # update_ez_cpml_x: cells q and nx-1-q, j in (0, ny-1)
+        lo, hi = q, nx - 1 - q
+        self.p_ez_xlo[q, J] = be[:, None] * self.p_ez_xlo[q, J] + ce[:, None] / dx * (Hy[lo, J] - Hy[lo - 1, J])
+        Ez[lo, J] += ch[lo, J] * dx * self.p_ez_xlo[q, J]
+        self.p_ez_xhi[q, J] = be[:, None] * self.p_ez_xhi[q, J] + ce[:, None] / dx * (Hy[hi, J] - Hy[hi - 1, J])
+        Ez[hi, J] += ch[hi, J] * dx * self.p_ez_xhi[q, J]
+        # update_ez_cpml_y: cells q and ny-1-q, i in (0, nx-1)
+        lo, hi = q, ny - 1 - q
+        self.p_ez_ylo[I, q] = be[None, :] * self.p_ez_ylo[I, q] + ce[None, :] / dx * (Hx[I, lo - 1] - Hx[I, lo])
+        Ez[I, lo] += ch[I, lo] * dx * self.p_ez_ylo[I, q]
+        self.p_ez_yhi[I, q] = be[None, :] * self.p_ez_yhi[I, q] + ce[None, :] / dx * (Hx[I, hi - 1] - Hx[I, hi])
+        Ez[I, hi] += ch[I, hi] * dx * self.p_ez_yhi[I, q]
+        # update_hx_hy on i < nx-1, j < ny-1
+        I0, J0 = slice(0, nx - 1), slice(0, ny - 1)
+        ce_ = k['CURL_E']
+        Hx[I0, J0] += ce_ * (Ez[I0, J0] - Ez[I0, 1:ny]) / k['DEN_HY'][None, J0]
+        Hy[I0, J0] += ce_ * (Ez[1:nx, J0] - Ez[I0, J0]) / k['DEN_HX'][I0, None]
+        bh, chh = k['BH'][q], k['CH'][q]
+        dtmu = float(np.float32(self.dt) / np.float32(self.mu))
+        # update_hy_cpml_x: cells q and nx-2-q, j < ny-1
+        lo, hi = q, nx - 2 - q
+        self.p_hy_xlo[q, J0] = bh[:, None] * self.p_hy_xlo[q, J0] + chh[:, None] / dx * (Ez[lo + 1, J0] - Ez[lo, J0])
+        Hy[lo, J0] += dtmu * self.p_hy_xlo[q, J0]
+        self.p_hy_xhi[q, J0] = bh[:, None] * self.p_hy_xhi[q, J0] + chh[:, None] / dx * (Ez[hi + 1, J0] - Ez[hi, J0])
+        Hy[hi, J0] += dtmu * self.p_hy_xhi[q, J0]
+        # update_hx_cpml_y: cells q and ny-2-q, i < nx-1
+        lo, hi = q, ny - 2 - q
+        self.p_hx_ylo[I0, q] = bh[None, :] * self.p_hx_ylo[I0, q] + chh[None, :] / dx * (Ez[I0, lo] - Ez[I0, lo + 1])
+        Hx[I0, lo] += dtmu * self.p_hx_ylo[I0, q]
+        self.p_hx_yhi[I0, q] = bh[None, :] * self.p_hx_yhi[I0, q] + chh[None, :] / dx * (Ez[I0, hi] - Ez[I0, hi + 1])
+        Hx[I0, hi] += dtmu * self.p_hx_yhi[I0, q]
+
+    def record(self, n):
+        cfg = self.cfg
+        if cfg.study == 'scattering_loss':
+            if not self.rec:
+                self.rec = {k: np.zeros((cfg.nx, cfg.ny), complex) for k in ('ez', 'hx', 'hy')}
+            for key, F_, st in (('ez', self.Ez, n + 0.5), ('hx', self.Hx, n + 1.0), ('hy', self.Hy, n + 1.0)):
+                w = simul_fft_weight(cfg.f0, st, cfg.delta_t)
+                self.rec[key] += F_ * np.cos(w) - 1j * (F_ * np.sin(w))
+            return
+        if not self.rec:
+            self.rec = {k: np.zeros((cfg.nt, cfg.ny)) for k in ('ez', 'hx')}
+        cx = cfg.cx
+        self.rec['ez'][n] = self.Ez[cx, :]
+        # Hx[cx, j-1] at j = 0 wraps to Hx[cx, ny-1], which no update writes
+        self.rec['hx'][n] = 0.5 * (self.Hx[cx, :] + np.roll(self.Hx[cx, :], 1))
+
+    def run(self, nsteps=None):
+        for n in range(self.cfg.nt if nsteps is None else nsteps):
+            self.step(n)
+            self.record(n)
+        return self.rec
+
+
 def make(cfg, upper=None, lower=None):
-    return (TEzStudy if cfg.scheme == 'te' else Yee3DStudy)(cfg, upper, lower)
+    return {'tm': TMzLossyStudy, 'te': TEzStudy, '3d': Yee3DStudy}[cfg.scheme](cfg, upper, lower)

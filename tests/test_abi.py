@@ -121,3 +121,20 @@ def test_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, 'size')] == C.sizeof(py), cname
         for f in py._fields_:
             assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
+
+
+def test_lossy_tm_description_checks():
+    """The lossy TMz scheme: offset 1, no port DFT, source inside the update range."""
+    from oidt_amd import variants
+    c = variants.config('tm', 'wave_impedance', sgl_wg_length=1.5e-6, port_length=0.3e-6, points_per_wl=16,
+                        ny_clad_wl=0.4, buffer_xhat=5, num_cpml=6)
+    a = variants.LossyTmArrays(c, variants.tm_mask(c, None, None))
+    fp = lambda d: _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64()))
+    assert fp(a.desc()) == _lib.OK
+    for field, value in (('offset', 2), ('nf', 1)):
+        d = a.desc()
+        setattr(d, field, value)
+        assert fp(d) == _lib.EINVAL
+    d = a.desc()
+    d.src_box[3] = c.ny
+    assert fp(d) == _lib.EINVAL

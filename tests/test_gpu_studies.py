@@ -1,5 +1,6 @@
 """Loss and wave-impedance study drivers on the device against the FP64 study
-oracle (oracle/studies.py), through the C ABI's field monitors.
+oracle (oracle/studies.py), through the C ABI's field monitors (and, for the
+TM studies, the lossy TMz scheme).
 
 Tolerance: the study drivers run in float32 (their kernels are float32 only);
 device outputs are within relative L2 1e-4 of the FP64 oracle, the FP32 field
@@ -15,7 +16,9 @@ from oidt_amd.solver import Monitor, Solver
 from oracle import studies as ost
 
 from helpers import rel_l2
-from test_studies import SMALL_3D, SMALL_TE
+from test_studies import SMALL_3D, SMALL_TE, SMALL_TM
+
+SMALL = {'tm': SMALL_TM, 'te': SMALL_TE, '3d': SMALL_3D}
 
 CASES = sorted(variants.DEFAULTS)
 
@@ -23,10 +26,10 @@ CASES = sorted(variants.DEFAULTS)
 @pytest.mark.gpu
 @pytest.mark.parametrize('scheme,study', CASES)
 def test_study_outputs_match_oracle(scheme, study):
-    c = variants.config(scheme, study, **(SMALL_TE if scheme == 'te' else SMALL_3D))
+    c = variants.config(scheme, study, **SMALL[scheme])
     print('case', scheme, study, c.nx, c.ny, getattr(c, 'nz', 1), c.nt, flush=True)
     dev = variants.run(c)
-    length = (c.p2x - c.p1x) if scheme == 'te' else (c.p2z - c.p1z)
+    length = (c.p2z - c.p1z) if scheme == '3d' else (c.p2x - c.p1x)
     up, lo = variants._profiles(c, length)
     ref = ost.make(c, up, lo).run()
     for k, v in dev.items():
@@ -87,3 +90,19 @@ def test_monitor_argument_checks():
         assert rc == _lib.EINVAL                    # wrong buffer size
     finally:
         sol.close()
+
+
+@pytest.mark.gpu
+def test_lossy_tm_rejects_fused_and_port_dft():
+    c = variants.config('tm', 'wave_impedance', **SMALL_TM)
+    arrays = variants.LossyTmArrays(c, variants.tm_mask(c, None, None))
+    sol = Solver.from_arrays(arrays, c.nt)
+    try:
+        with pytest.raises(_lib.OidtError) as e:
+            sol.set_launch(8, 0, 3)
+        assert e.value.code == _lib.EINVAL
+    finally:
+        sol.close()
+    d = arrays.desc()
+    d.nf = 2
+    assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.EINVAL
