@@ -108,6 +108,117 @@ def workload(name):
     raise SystemExit('unknown workload ' + name)
 
 
+STUDIES = {'sl_tm': ('tm', 'scattering_loss'), 'sl_te': ('te', 'scattering_loss'), 'sl_3d': ('3d', 'scattering_loss'),
+           'wi_tm': ('tm', 'wave_impedance'), 'wi_te': ('te', 'wave_impedance'), 'wi_3d': ('3d', 'wave_impedance')}
+
+
+def monitor_bytes(mons, es, after_h):
+    """Algorithmic bytes of one monitor launch: per cell the field read once,
+    plus the running sums read and written (DFT) or the record written."""
+    total = 0
+    for m in mons.values():
+        if m.after_h != after_h:
+            continue
+        cells = int(np.prod(m.extent))
+        total += cells * es * (5 if m.cs is not None else 2)
+    return total
+
+
+def bench_study(args):
+    """A loss or wave-impedance study at the reference's default size, cut to
+    warmup + steps time steps (field monitors included in every step)."""
+    import ctypes as C
+    from oidt_amd import _lib, variants
+    scheme, study = STUDIES[args.workload]
+    chunk = _lib.chunk_steps()
+    warm = -(-args.warmup // chunk) * chunk
+    K = -(-args.steps // chunk) * chunk
+    c = variants.truncate(variants.config(scheme, study), warm + K + (1 if scheme == 'te' else 0))
+    # x-chunk of the two-kernel form: the given one, or the fastest step
+    # (E + H + monitor launches, HIP events, two chunks each)
+    tune = {}
+    for xc in ([args.xchunk] if args.xchunk else [1, 2, 4, 8]):
+        sol, mons, ids = variants.prepare(c, (2, xc, 0))
+        out = (C.c_double * 6)()
+        try:
+            _lib.check(_lib.lib().oidt_solver_profile_all(sol.h, 2, out))
+        finally:
+            sol.close()
+        tune[xc] = out[0] + out[1] + out[4] + out[5]
+    xchunk = min(tune, key=tune.get)
+    sol, mons, ids = variants.prepare(c, (2, xchunk, 0))
+    try:
+        sol.advance(warm).sync()
+        t0 = time.perf_counter()
+        sol.advance(K).sync()
+        wall = time.perf_counter() - t0
+    finally:
+        sol.close()
+    cells = c.nx * c.ny * (c.nz if scheme == '3d' else 1)
+    value = cells * K / wall / 1e6
+    sol, mons, ids = variants.prepare(c, (2, xchunk, 0))
+    out = (C.c_double * 6)()
+    try:
+        _lib.check(_lib.lib().oidt_solver_profile_all(sol.h, int(args.profile_chunks), out))
+    finally:
+        sol.close()
+    ms_e, ms_h, ms_dft, ms_adv, ms_me, ms_mh = list(out)
+    es = 4
+    be, bh = algorithmic_bytes('tm' if scheme == 'tm' else scheme, es)
+    stem = {'tm': 'k_tmzl', 'te': 'k_tez', '3d': 'k_3d'}[scheme]
+    cands = [(stem + ('_e' if scheme == 'tm' else '_e2'), ms_e, be * cells),
+             (stem + ('_h' if scheme == 'tm' else '_h2'), ms_h, bh * cells),
+             ('k_monitor (after E)', ms_me, monitor_bytes(mons, es, False)),
+             ('k_monitor (after H)', ms_mh, monitor_bytes(mons, es, True))]
+    kname, ms_k, bytes_k = max(cands, key=lambda t: t[1])
+    achieved = bytes_k / (ms_k * 1e-3) / 1e9
+    traffic, traffic_src = committed_traffic(kname.split()[0], args.workload, 2, xchunk)
+    roof = {'bound': 'hbm', 'kernel': kname, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
+            'traffic_source': traffic_src, 'bytes_per_launch': bytes_k, 'ms_per_launch': round(ms_k, 5),
+            'ms_e': round(ms_e, 5), 'ms_h': round(ms_h, 5), 'ms_monitor_e': round(ms_me, 5),
+            'ms_monitor_h': round(ms_mh, 5)}
+    line = {
+        'metric': 'Mcell-updates/s', 'value': round(value, 1), 'unit': 'Mcell-updates/s', 'n_gpus': 1,
+        'steps': K, 'warmup': warm, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': '%s %s study, reference default configuration (%s cells), first %d steps of %d'
+                               % (scheme, study, 'x'.join(str(n) for n in ((c.nx, c.ny, c.nz) if scheme == '3d'
+                                                                           else (c.nx, c.ny))),
+                                  warm + K, variants.config(scheme, study).nt),
+                   'cells_per_step': cells, 'monitors': sorted(mons), 'variant': 2, 'xchunk': xchunk,
+                   'xchunk_ms_per_step': {str(k): round(v, 5) for k, v in tune.items()},
+                   'parallelism': 'single GPU'},
+        'roofline': roof,
+    }
+    if not args.no_cpu and scheme != '3d':
+        line['cpu_baseline'] = study_cpu_baseline(scheme, study, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+def study_cpu_baseline(scheme, study, seconds):
+    """The FP64 study oracle (oracle/studies.py, one thread) on the study's own
+    grid, whole steps (recording included) for about ``seconds``."""
+    from oidt_amd import variants
+    from oracle import studies as ost
+    c = variants.config(scheme, study)
+    o = ost.make(c)
+    o.step(0)
+    o.record(0)
+    n, t0 = 1, time.perf_counter()
+    while True:
+        o.step(n)
+        o.record(n)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    cells = c.nx * c.ny
+    return {'value': round(cells * (n - 1) / dt / 1e6, 3), 'unit': 'Mcell-updates/s', 'cores': 1, 'kind': 'port',
+            'sample': '%d steps (after 1 warm-up) of the FP64 NumPy study oracle on the %dx%d grid, %.1f s'
+                      % (n - 1, c.nx, c.ny, dt)}
+
+
 def bench_serving(args):
     """Serving mix: plain one-at-a-time path, saturated continuous batching, and
     Poisson arrivals at 0.8x the saturated capacity (latency percentiles)."""
@@ -213,7 +324,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=6016)
     ap.add_argument('--warmup', type=int, default=256)
-    ap.add_argument('--workload', default='c1', help='c0, c1, c2, c2te, c3 or c4 (serving)')
+    ap.add_argument('--workload', default='c1',
+                    help='c0, c1, c2, c2te, c3, c4 (serving), or a study: sl_tm, sl_te, sl_3d, wi_tm, wi_te, wi_3d')
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=0,
                     help='kernel form: 1 march, 2 batched loads, 3 fused step; 0: autotuner picks form and xchunk')
@@ -244,6 +356,9 @@ def main():
 
     if args.workload == 'c4':
         bench_serving(args)
+        return
+    if args.workload in STUDIES:
+        bench_study(args)
         return
     descs, label = workload(args.workload)
     if args.plain:

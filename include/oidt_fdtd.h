@@ -278,6 +278,11 @@ int oidt_batch_destroy(oidt_batch* b);
 int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4);
 int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4);
 
+/* As oidt_batch_profile, plus out_ms6[4] / [5]: mean duration of the field
+ * monitor launch after the E / H half-step (0 when there is none). */
+int oidt_batch_profile_all(oidt_batch* b, int32_t nchunks, double* out_ms6);
+int oidt_solver_profile_all(oidt_solver* s, int32_t nchunks, double* out_ms6);
+
 /* Mean duration (ms) of a plain device copy kernel moving nbytes (read nbytes,
  * write nbytes), over reps launches: the reference point for HBM counters. */
 int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms);

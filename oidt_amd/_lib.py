@@ -84,6 +84,8 @@ SIGNATURES = {
     'oidt_batch_destroy': (C.c_int, [C.c_void_p]),
     'oidt_batch_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     'oidt_solver_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    'oidt_batch_profile_all': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    'oidt_solver_profile_all': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     'oidt_copy_probe': (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(C.c_double)]),
 }
 

@@ -492,7 +492,7 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
 bool march_tiles(const oidt_batch* b) { return b->variant == 3; }
 
 // Monitor table and the two work lists (after the E and after the H
-// half-step): one item per kThreads consecutive box cells.
+// half-step): one item per kMonCells consecutive box cells.
 template <typename T>
 int rebuild_monitors(oidt_batch* b) {
   std::vector<Monitor<T>> table;
@@ -515,7 +515,7 @@ int rebuild_monitors(oidt_batch* b) {
       m.shift = d.shift; m.nrec = d.nrec;
       const int id = (int)table.size();
       table.push_back(m);
-      for (size_t c0 = 0; c0 < mh.cells; c0 += kThreads) work[d.after_h ? 1 : 0].push_back(make_int4(id, (int)c0, 0, 0));
+      for (size_t c0 = 0; c0 < mh.cells; c0 += kMonCells) work[d.after_h ? 1 : 0].push_back(make_int4(id, (int)c0, 0, 0));
     }
   }
   int rc;
@@ -712,43 +712,42 @@ int enqueue_chunk(oidt_batch* b) {
 template <typename T>
 int profile_chunks(oidt_batch* b, int nchunks, double* out) {
   const int gs = stencil_grid(b), gd = dft_grid(b);
-  const int per = 2 * kChunk + 2;
-  std::vector<hipEvent_t> ev(2 * per * nchunks);
-  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-  int k = 0;
+  // kinds: 0 E half-step, 1 H half-step, 2 DFT fold, 3 advance, 4 / 5 monitors after E / H
+  std::vector<std::pair<int, hipEvent_t>> ev;
+  auto timed = [&](int kind, auto&& launch) -> int {
+    hipEvent_t a, z;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&z));
+    ev.emplace_back(kind, a);
+    ev.emplace_back(kind, z);
+    HIP_TRY(hipEventRecord(a, b->stream));
+    launch();
+    HIP_TRY(hipEventRecord(z, b->stream));
+    return OIDT_OK;
+  };
+  int rc;
   for (int c = 0; c < nchunks; ++c) {
     for (int t = 0; t < kChunk; ++t)
       for (int h = 0; h < 2; ++h) {
-        HIP_TRY(hipEventRecord(ev[k++], b->stream));
-        launch_stencil<T>(b, gs, t, h == 0);
-        HIP_TRY(hipEventRecord(ev[k++], b->stream));
-        launch_monitors<T>(b, t, h);   // untimed; keeps the runs' records whole
+        if (!(b->variant == 3 && h == 1) &&
+            (rc = timed(h, [&] { launch_stencil<T>(b, gs, t, h == 0); })) != OIDT_OK) return rc;
+        if (b->nmwork[h] > 0 && (rc = timed(4 + h, [&] { launch_monitors<T>(b, t, h); })) != OIDT_OK) return rc;
       }
-    HIP_TRY(hipEventRecord(ev[k++], b->stream));
-    launch_dft<T>(b, gd);
-    HIP_TRY(hipEventRecord(ev[k++], b->stream));
-    HIP_TRY(hipEventRecord(ev[k++], b->stream));
-    launch_advance<T>(b);
-    HIP_TRY(hipEventRecord(ev[k++], b->stream));
+    if ((rc = timed(2, [&] { launch_dft<T>(b, gd); })) != OIDT_OK) return rc;
+    if ((rc = timed(3, [&] { launch_advance<T>(b); })) != OIDT_OK) return rc;
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(b->stream));
-  double sum[4] = {0, 0, 0, 0};
-  int cnt[4] = {0, 0, 0, 0};
-  k = 0;
-  for (int c = 0; c < nchunks; ++c) {
-    for (int i = 0; i < per; ++i) {
-      float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-      k += 2;
-      const int kind = i < 2 * kChunk ? (i & 1) : (i == 2 * kChunk ? 2 : 3);
-      if (b->variant == 3 && kind == 1) continue;   // fused step: no separate H launch
-      sum[kind] += ms;
-      cnt[kind] += 1;
-    }
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  int cnt[6] = {0, 0, 0, 0, 0, 0};
+  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[i].second, ev[i + 1].second));
+    sum[ev[i].first] += ms;
+    cnt[ev[i].first] += 1;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  for (int i = 0; i < 4; ++i) out[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
+  for (auto& e : ev) (void)hipEventDestroy(e.second);
+  for (int i = 0; i < 6; ++i) out[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
   b->chunks_enqueued += nchunks;
   b->chunks_done = b->chunks_enqueued;
   return OIDT_OK;
@@ -1180,8 +1179,8 @@ int oidt_batch_read_monitor(oidt_batch* b, int32_t slot, int32_t monitor, void* 
   return OIDT_OK;
 }
 
-int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4) {
-  if (!b || !out_ms4) return fail(OIDT_EINVAL, "null argument");
+int oidt_batch_profile_all(oidt_batch* b, int32_t nchunks, double* out_ms6) {
+  if (!b || !out_ms6) return fail(OIDT_EINVAL, "null argument");
   if (nchunks < 1) return fail(OIDT_EINVAL, "need at least one chunk");
   int rc;
   if (b->dirty) {
@@ -1190,8 +1189,17 @@ int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4) {
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
   reap_marks(b);
-  return b->precision == OIDT_F64 ? profile_chunks<double>(b, nchunks, out_ms4)
-                                  : profile_chunks<float>(b, nchunks, out_ms4);
+  return b->precision == OIDT_F64 ? profile_chunks<double>(b, nchunks, out_ms6)
+                                  : profile_chunks<float>(b, nchunks, out_ms6);
+}
+
+int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4) {
+  if (!out_ms4) return fail(OIDT_EINVAL, "null argument");
+  double all[6];
+  const int rc = oidt_batch_profile_all(b, nchunks, all);
+  if (rc == OIDT_OK)
+    for (int i = 0; i < 4; ++i) out_ms4[i] = all[i];
+  return rc;
 }
 
 int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms) {
@@ -1227,6 +1235,11 @@ int oidt_copy_probe(uint64_t nbytes, int32_t reps, double* ms) {
 int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4) {
   if (!s) return fail(OIDT_EINVAL, "null solver");
   return oidt_batch_profile(s->b, nchunks, out_ms4);
+}
+
+int oidt_solver_profile_all(oidt_solver* s, int32_t nchunks, double* out_ms6) {
+  if (!s) return fail(OIDT_EINVAL, "null solver");
+  return oidt_batch_profile_all(s->b, nchunks, out_ms6);
 }
 
 int oidt_batch_destroy(oidt_batch* b) {

@@ -2623,10 +2623,15 @@ __global__ __launch_bounds__(kThreads) void k_tmzl_h(const Slot<T>* __restrict__
 }
 
 // Field monitors taken after one half-step: one work item = (monitor, first
-// of kThreads consecutive box cells, -, -).  Reads the current field set (the
-// two-kernel forms keep the state in set A).  DFT: the stencil sum is formed
-// in T point by point, scaled and weighted in double, and the running sum is
-// rounded back to T each step, as the reference's T accumulators are.
+// of kMonCells consecutive box cells, -, -); each lane takes kMonCells / 64
+// cells kThreads apart, loads all of them first, then folds and stores.
+// Reads the current field set (the two-kernel forms keep the state in set A).
+// DFT: the stencil sum is formed in T point by point, scaled and weighted in
+// double, and the running sum is rounded back to T each step, as the
+// reference's T accumulators are.
+constexpr int kMonPer = 4;
+constexpr int kMonCells = kThreads * kMonPer;
+
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_monitor(const Slot<T>* __restrict__ slots,
                                                       const Monitor<T>* __restrict__ mons,
@@ -2640,27 +2645,47 @@ __global__ __launch_bounds__(kThreads) void k_monitor(const Slot<T>* __restrict_
     const int r = n + m.shift;
     if (r < 0 || r >= m.nrec) continue;
     const int cells = m.ex * m.ey * m.ez;
-    const int c = it.y + (int)threadIdx.x;
-    if (c >= cells) continue;
-    const int cz = c % m.ez, cxy = c / m.ez;
-    const int x = m.x0 + cxy / m.ey, y = m.y0 + cxy % m.ey, z = m.z0 + cz;
     const T* __restrict__ f = s.f[m.field];
     const bool is3d = s.scheme == SCHEME_3D;
-    T sum = T(0);
-    for (int q = 0; q < m.npts; ++q) {
-      const int xx = x + m.ox[q], yy = y + m.oy[q], zz = z + m.oz[q];
-      T v = T(0);
-      if (xx >= 0 && xx < m.gx && yy >= 0 && yy < m.gy && zz >= 0 && zz < m.gz)
-        v = f[is3d ? ((size_t)xx * s.ny + yy) * s.nz + zz : (size_t)xx * s.ny + yy];
-      sum = q == 0 ? v : sum + v;
+    T sum[kMonPer];
+#pragma unroll
+    for (int k = 0; k < kMonPer; ++k) {
+      const int c = it.y + (int)threadIdx.x + k * kThreads;
+      sum[k] = T(0);
+      if (c >= cells) continue;
+      const int cz = c % m.ez, cxy = c / m.ez;
+      const int x = m.x0 + cxy / m.ey, y = m.y0 + cxy % m.ey, z = m.z0 + cz;
+      for (int q = 0; q < m.npts; ++q) {
+        const int xx = x + m.ox[q], yy = y + m.oy[q], zz = z + m.oz[q];
+        T v = T(0);
+        if (xx >= 0 && xx < m.gx && yy >= 0 && yy < m.gy && zz >= 0 && zz < m.gz)
+          v = f[is3d ? ((size_t)xx * s.ny + yy) * s.nz + zz : (size_t)xx * s.ny + yy];
+        sum[k] = q == 0 ? v : sum[k] + v;
+      }
     }
-    const double v = m.scale * (double)sum;
     if (m.mode == 0) {
       const double cr = m.cs[2 * r], ci = m.cs[2 * r + 1];
-      m.re[c] = (T)((double)m.re[c] + v * cr);
-      m.im[c] = (T)((double)m.im[c] + v * ci);
+      T re[kMonPer], im[kMonPer];
+#pragma unroll
+      for (int k = 0; k < kMonPer; ++k) {
+        const int c = it.y + (int)threadIdx.x + k * kThreads;
+        re[k] = c < cells ? m.re[c] : T(0);
+        im[k] = c < cells ? m.im[c] : T(0);
+      }
+#pragma unroll
+      for (int k = 0; k < kMonPer; ++k) {
+        const int c = it.y + (int)threadIdx.x + k * kThreads;
+        if (c >= cells) continue;
+        const double v = m.scale * (double)sum[k];
+        m.re[c] = (T)((double)re[k] + v * cr);
+        m.im[c] = (T)((double)im[k] + v * ci);
+      }
     } else {
-      m.re[(size_t)r * cells + c] = (T)v;
+#pragma unroll
+      for (int k = 0; k < kMonPer; ++k) {
+        const int c = it.y + (int)threadIdx.x + k * kThreads;
+        if (c < cells) m.re[(size_t)r * cells + c] = (T)(m.scale * (double)sum[k]);
+      }
     }
   }
 }

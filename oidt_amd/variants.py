@@ -436,18 +436,39 @@ def tm_mask(c, upper, lower):
     return (j >= c.by + lo[:, None]) & (j < c.by + c.ny_swg + up[:, None])
 
 
-def run(c, launch=(2, 0, 0)):
-    """Step the study on the device; returns {name: array} in the reference's
-    output shapes and types (complex64 phasors, float32 time records)."""
+def truncate(c, nt):
+    """The study cut to its first nt steps (source and phase tables too)."""
+    c.nt = int(nt)
+    c.J_SRC = c.J_SRC[:c.nt]
+    for k in ('cosE', 'sinE', 'cosH', 'sinH'):
+        if hasattr(c, k):
+            setattr(c, k, getattr(c, k)[:c.nt])
+    return c
+
+
+def prepare(c, launch=(2, 0, 0)):
+    """(solver, monitors, monitor ids): the study's run on the device with its
+    monitors attached, not yet advanced."""
+    variant, xchunk, grid = launch
+    if variant == 3:
+        raise ValueError('the study drivers need a two-kernel form (monitors sit between the half-steps)')
     s, fg, cb = build(c)
     sol = Solver.from_arrays(LossyTmArrays(c, fg), c.nt) if c.scheme == 'tm' else Solver(s, fg, cb=cb)
     try:
-        variant, xchunk, grid = launch
-        if variant == 3:
-            raise ValueError('the study drivers need a two-kernel form (monitors sit between the half-steps)')
         sol.set_launch(xchunk, grid, variant)
         mons = monitors(c)
         ids = {k: sol.add_monitor(m) for k, m in mons.items()}
+    except Exception:
+        sol.close()
+        raise
+    return sol, mons, ids
+
+
+def run(c, launch=(2, 0, 0)):
+    """Step the study on the device; returns {name: array} in the reference's
+    output shapes and types (complex64 phasors, float32 time records)."""
+    sol, mons, ids = prepare(c, launch)
+    try:
         sol.run()
         return {k: _output(c, k, mons[k], sol.read_monitor(ids[k])) for k in mons}
     finally:
