@@ -243,13 +243,33 @@ class Batch:
         self.h = h
         self.runs = {}
 
-    def add(self, setup, fg, direction='f', line=0):
-        arr = RunArrays(setup, fg, direction, line)
-        d, r = arr.desc(), arr.inputs()
+    def add(self, setup, fg, direction='f', line=0, cb=None):
+        return self.add_arrays(RunArrays(setup, fg, direction, line, cb), setup)
+
+    def add_arrays(self, arrays, setup):
+        """Add a run from any object with ``desc()`` / ``inputs()``; ``setup``
+        describes it to the Python side (nt, precision, shape, frequencies)."""
+        d, r = arrays.desc(), arrays.inputs()
         slot = C.c_int32()
         check(lib().oidt_batch_add(self.h, C.byref(d), C.byref(r), C.byref(slot)))
         self.runs[slot.value] = setup
+        self.monitors = getattr(self, 'monitors', {})
+        self.monitors[slot.value] = []
         return slot.value
+
+    def add_monitor(self, slot, mon):
+        """Attach a ``Monitor`` to run ``slot`` before it starts; returns its index."""
+        d = mon.desc()
+        idx = C.c_int32()
+        check(lib().oidt_batch_add_monitor(self.h, int(slot), C.byref(d), C.byref(idx)))
+        self.monitors[slot].append(mon)
+        return idx.value
+
+    def read_monitor(self, slot, idx):
+        mon = self.monitors[slot][idx]
+        re, im = mon.buffers(self.runs[slot].precision)
+        check(lib().oidt_batch_read_monitor(self.h, int(slot), int(idx), ptr(re), ptr(im), re.nbytes))
+        return mon.result(re, im)
 
     def set_launch(self, xchunk=8, grid_blocks=0, variant=2):
         check(lib().oidt_batch_set_launch(self.h, int(xchunk), int(grid_blocks), int(variant)))

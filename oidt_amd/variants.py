@@ -475,6 +475,41 @@ def run(c, launch=(2, 0, 0)):
         sol.close()
 
 
+def run_batch(cs, launch=(2, 0, 0)):
+    """Several studies of one scheme (e.g. the roughness realisations that
+    automated_results_collection.py runs one process each) as one device
+    batch; returns one output dict per configuration, as ``run``."""
+    from .solver import Batch
+    schemes = {c.scheme for c in cs}
+    if len(schemes) != 1:
+        raise ValueError('a batch holds studies of one scheme')
+    scheme = schemes.pop()
+    variant, xchunk, grid = launch
+    if variant == 3:
+        raise ValueError('the study drivers need a two-kernel form (monitors sit between the half-steps)')
+    b = Batch('tm_lossy' if scheme == 'tm' else scheme, np.float32, max_slots=len(cs))
+    try:
+        b.set_launch(xchunk or (1 if scheme == '3d' else 4), grid, variant)
+        runs = []
+        for c in cs:
+            s, fg, cb = build(c)
+            if scheme == 'tm':
+                meta = SimpleNamespace(nt=c.nt, precision=np.float32, shape=(c.nx, c.ny), sparam_freqs=np.zeros(0))
+                slot = b.add_arrays(LossyTmArrays(c, fg), meta)
+            else:
+                slot = b.add(s, fg, cb=cb)
+            mons = monitors(c)
+            ids = {k: b.add_monitor(slot, m) for k, m in mons.items()}
+            runs.append((c, slot, mons, ids, s.nt if s is not None else c.nt))
+        chunk = _lib.chunk_steps()
+        b.advance(-(-max(r[4] for r in runs) // chunk))
+        b.sync()
+        return [{k: _output(c, k, mons[k], b.read_monitor(slot, ids[k])) for k in mons}
+                for c, slot, mons, ids, _ in runs]
+    finally:
+        b.close()
+
+
 def _output(c, key, mon, val):
     if c.scheme == 'tm':
         if c.study == 'scattering_loss':

@@ -106,3 +106,20 @@ def test_lossy_tm_rejects_fused_and_port_dft():
     d = arrays.desc()
     d.nf = 2
     assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.EINVAL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('scheme', ['tm', 'te'])
+def test_study_realisations_batched_equal_alone(scheme):
+    """Roughness realisations of a scattering-loss study in one batch (and a
+    wave-impedance run beside them) give bit-identical arrays to solo runs."""
+    cs = [variants.config(scheme, 'scattering_loss', profile_number=p, **SMALL[scheme]) for p in (0, 1)]
+    cs.append(variants.config(scheme, 'wave_impedance', **SMALL[scheme]))
+    together = variants.run_batch(cs)
+    for c, got in zip(cs, together):
+        alone = variants.run(c, (2, 4, 0))
+        assert set(got) == set(alone)
+        for k in alone:
+            np.testing.assert_array_equal(got[k], alone[k], err_msg=(c.study, c.profile_number, k))
+    assert not np.array_equal(together[0]['ez' if scheme == 'tm' else 'hz'],
+                              together[1]['ez' if scheme == 'tm' else 'hz'])
