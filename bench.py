@@ -134,6 +134,8 @@ def bench_study(args):
     warm = -(-args.warmup // chunk) * chunk
     K = -(-args.steps // chunk) * chunk
     c = variants.truncate(variants.config(scheme, study), warm + K + (1 if scheme == 'te' else 0))
+    if args.realisations > 1:
+        return bench_study_batch(args, scheme, study, warm, K)
     # x-chunk of the two-kernel form: the given one, or the fastest step
     # (E + H + monitor launches, HIP events, two chunks each)
     tune = {}
@@ -193,6 +195,51 @@ def bench_study(args):
     }
     if not args.no_cpu and scheme != '3d':
         line['cpu_baseline'] = study_cpu_baseline(scheme, study, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+def bench_study_batch(args, scheme, study, warm, K):
+    """``--realisations R``: R roughness realisations of a study (profile
+    numbers 0..R-1) as one batch, against the same R run one at a time."""
+    from oidt_amd import variants
+    R = args.realisations
+    steps = warm + K + (1 if scheme == 'te' else 0)
+    cs = [variants.truncate(variants.config(scheme, study, profile_number=p, rough_toggle=True), steps)
+          for p in range(R)]
+    xc = args.xchunk or (1 if scheme == '3d' else 4)
+    b, runs = variants.prepare_batch(cs, (2, xc, 0))
+    try:
+        b.advance(warm // 32)
+        b.sync()
+        t0 = time.perf_counter()
+        b.advance(K // 32)
+        b.sync()
+        wall = time.perf_counter() - t0
+    finally:
+        b.close()
+    cells = sum(c.nx * c.ny * (c.nz if scheme == '3d' else 1) for c in cs)
+    value = cells * K / wall / 1e6
+    # plain: the same realisations one at a time, each its own run (create, upload and read back included)
+    t0 = time.perf_counter()
+    for c in cs:
+        sol, mons, ids = variants.prepare(c, (2, xc, 0))
+        try:
+            sol.advance(warm + K).sync()
+            for k in mons:
+                sol.read_monitor(ids[k])
+        finally:
+            sol.close()
+    plain_wall = time.perf_counter() - t0
+    plain = cells * (warm + K) / plain_wall / 1e6
+    line = {
+        'metric': 'Mcell-updates/s', 'value': round(value, 1), 'unit': 'Mcell-updates/s', 'n_gpus': 1,
+        'steps': K, 'warmup': warm, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': '%d roughness realisations of the %s %s study, one batch, reference default '
+                               'configuration, first %d steps' % (R, scheme, study, warm + K),
+                   'cells_per_step': cells, 'variant': 2, 'xchunk': xc, 'parallelism': 'single GPU'},
+        'plain_one_at_a_time': round(plain, 1), 'batched_over_plain': round(value / plain, 3),
+    }
     print(json.dumps(line), flush=True)
 
 
@@ -335,6 +382,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--plain', action='store_true', help='ensembles: one run at a time instead of one batch')
     ap.add_argument('--requests', type=int, default=48, help='serving workload: number of requests')
+    ap.add_argument('--realisations', type=int, default=1,
+                    help='study workloads: roughness realisations in one batch (against the same run one at a time)')
     ap.add_argument('--copy-probe', type=float, default=0.0,
                     help='only run the copy kernel on this many MiB (counter calibration) and exit')
     args = ap.parse_args()

@@ -475,10 +475,10 @@ def run(c, launch=(2, 0, 0)):
         sol.close()
 
 
-def run_batch(cs, launch=(2, 0, 0)):
-    """Several studies of one scheme (e.g. the roughness realisations that
-    automated_results_collection.py runs one process each) as one device
-    batch; returns one output dict per configuration, as ``run``."""
+def prepare_batch(cs, launch=(2, 0, 0)):
+    """(batch, runs): several studies of one scheme in one device batch with
+    their monitors attached, not yet advanced; runs = [(config, slot,
+    monitors, monitor ids, steps)]."""
     from .solver import Batch
     schemes = {c.scheme for c in cs}
     if len(schemes) != 1:
@@ -501,6 +501,18 @@ def run_batch(cs, launch=(2, 0, 0)):
             mons = monitors(c)
             ids = {k: b.add_monitor(slot, m) for k, m in mons.items()}
             runs.append((c, slot, mons, ids, s.nt if s is not None else c.nt))
+    except Exception:
+        b.close()
+        raise
+    return b, runs
+
+
+def run_batch(cs, launch=(2, 0, 0)):
+    """Several studies of one scheme (e.g. the roughness realisations that
+    automated_results_collection.py runs one process each) as one device
+    batch; returns one output dict per configuration, as ``run``."""
+    b, runs = prepare_batch(cs, launch)
+    try:
         chunk = _lib.chunk_steps()
         b.advance(-(-max(r[4] for r in runs) // chunk))
         b.sync()
