@@ -145,13 +145,15 @@ int validate(const oidt_run_desc* d) {
     // both CPML slabs must fit inside the update range without overlapping
     if (dims[a] - 2 * o < 2 * N + 1) return fail(OIDT_EINVAL, "grid too small for the CPML thickness");
   }
-  // Kernels index cells with 32-bit offsets into the padded layout (the
-  // unit-stride axis rounded up to 128 bytes).
-  const int q = d->precision == OIDT_F64 ? 16 : 32;
+  // Kernels address a field with a 32-bit byte offset into the padded layout
+  // (the unit-stride axis rounded up to 128 bytes): each field array must
+  // stay below 4 GiB.
+  const int es = d->precision == OIDT_F64 ? 8 : 4;
+  const int q = 128 / es;
   const double fast = is3d ? (double)((d->nz + q - 1) / q * q) : (double)((d->ny + q - 1) / q * q);
   const double slow = is3d ? (double)d->nx * d->ny : (double)d->nx;
-  if ((double)d->nx * d->ny * d->nz > 4.0e9 || slow * fast > 4.0e9)
-    return fail(OIDT_EINVAL, "grid larger than 2^32 cells (after padding the unit-stride axis to 128 bytes)");
+  if (slow * fast * es > 4294967295.0)
+    return fail(OIDT_EINVAL, "a field array would exceed 4 GiB (after padding the unit-stride axis to 128 bytes)");
   if (d->nt < 0 || d->nf < 0) return fail(OIDT_EINVAL, "negative nt or nf");
   const int plim = is3d ? d->nz : d->nx;
   for (int p = 0; p < 2; ++p)
@@ -442,25 +444,25 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   s.dxT = (T)d.dx;
   s.dtmu = (T)d.dt / (T)kMu0;
   s.dt = d.dt;
-  for (int f = 0; f < 6; ++f) s.f[f] = L.off_f[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f[f]);
-  for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi[i]);
-  for (int f = 0; f < 6; ++f) s.f2[f] = L.off_f2[f] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_f2[f]);
-  for (int i = 0; i < 36; ++i) s.psi2[i] = L.off_psi2[i] == (size_t)-1 ? nullptr : reinterpret_cast<T*>(base + L.off_psi2[i]);
-  s.mat = reinterpret_cast<const uint8_t*>(base + L.off_mat);
+  for (int f = 0; f < 6; ++f) s.f[f] = L.off_f[f] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_f[f]);
+  for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_psi[i]);
+  for (int f = 0; f < 6; ++f) s.f2[f] = L.off_f2[f] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_f2[f]);
+  for (int i = 0; i < 36; ++i) s.psi2[i] = L.off_psi2[i] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_psi2[i]);
+  s.mat = (OIDT_GLOBAL const uint8_t*)(base + L.off_mat);
   for (int c = 0; c < 3; ++c)
     for (int q = 0; q < kMatMax; ++q) s.cbv[c][q] = cbv[c][q];
   for (int a = 0; a < 3; ++a) {
-    s.ike[a] = reinterpret_cast<const T*>(base + L.off_ike[a]);
-    s.ikh[a] = reinterpret_cast<const T*>(base + L.off_ikh[a]);
+    s.ike[a] = (OIDT_GLOBAL const T*)(base + L.off_ike[a]);
+    s.ikh[a] = (OIDT_GLOBAL const T*)(base + L.off_ikh[a]);
   }
-  s.be = reinterpret_cast<const T*>(base + L.off_be);
-  s.ceo = reinterpret_cast<const T*>(base + L.off_ceo);
-  s.bh = reinterpret_cast<const T*>(base + L.off_bh);
-  s.cho = reinterpret_cast<const T*>(base + L.off_cho);
-  s.jsrc = reinterpret_cast<const T*>(base + L.off_jsrc);
-  s.ring = reinterpret_cast<T*>(base + L.off_ring);
-  s.acc = reinterpret_cast<double*>(base + L.off_acc);
-  s.w = reinterpret_cast<const double*>(base + L.off_w);
+  s.be = (OIDT_GLOBAL const T*)(base + L.off_be);
+  s.ceo = (OIDT_GLOBAL const T*)(base + L.off_ceo);
+  s.bh = (OIDT_GLOBAL const T*)(base + L.off_bh);
+  s.cho = (OIDT_GLOBAL const T*)(base + L.off_cho);
+  s.jsrc = (OIDT_GLOBAL const T*)(base + L.off_jsrc);
+  s.ring = (OIDT_GLOBAL T*)(base + L.off_ring);
+  s.acc = (OIDT_GLOBAL double*)(base + L.off_acc);
+  s.w = (OIDT_GLOBAL const double*)(base + L.off_w);
   sh.slot.assign(reinterpret_cast<unsigned char*>(&s), reinterpret_cast<unsigned char*>(&s) + sizeof(s));
   sh.nplanes = L.nplanes; sh.nq = L.nq; sh.wcells = L.wcells; sh.pcells = L.pcells; sh.wyn = L.wy1 - L.wy0;
   return OIDT_OK;
@@ -503,9 +505,9 @@ int rebuild_monitors(oidt_batch* b) {
     for (const MonHost& mh : sh.mons) {
       const oidt_monitor_desc& d = mh.d;
       Monitor<T> m{};
-      m.re = static_cast<T*>(mh.re);
-      m.im = static_cast<T*>(mh.im);
-      m.cs = mh.cs;
+      m.re = (OIDT_GLOBAL T*)mh.re;
+      m.im = (OIDT_GLOBAL T*)mh.im;
+      m.cs = (OIDT_GLOBAL const double*)mh.cs;
       m.scale = d.scale;
       m.slot = si; m.field = d.field; m.mode = d.mode; m.npts = d.npts;
       for (int q = 0; q < 4; ++q) { m.ox[q] = d.off[q][0]; m.oy[q] = d.off[q][1]; m.oz[q] = d.off[q][2]; }

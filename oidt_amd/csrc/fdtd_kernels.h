@@ -51,6 +51,21 @@ __device__ __forceinline__ double lane_get(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Global-memory access by a 32-bit byte offset.  Every pointer of a run is a
+// device-global allocation; saying so (address space 1) replaces flat
+// instructions (64-bit per-lane addresses, counted on both the vector-memory
+// and the LDS counter) with global ones addressed as an SGPR base plus a
+// 32-bit VGPR offset.  Arrays addressed this way are below 4 GiB (checked
+// when a run is described).
+template <typename T>
+__device__ __forceinline__ T ldg(const T* base, unsigned boff) {
+  return *(const OIDT_GLOBAL T*)(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* base, unsigned boff, T v) {
+  *(OIDT_GLOBAL T*)(reinterpret_cast<char*>(base) + boff) = v;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that the blocks
 // of one XCD take a contiguous range of tiles (shared L2 for x-neighbours).
 __device__ __forceinline__ int xcd_block() {
@@ -82,8 +97,8 @@ __device__ __forceinline__ T cbsel(const Slot<T>& s, uint8_t m, int c) {
   return q == 0 ? s.cbv[c][0] : (q == 1 ? s.cbv[c][1] : (q == 2 ? s.cbv[c][2] : s.cbv[c][3]));
 }
 
-template <typename T>
-__device__ __forceinline__ T psi_upd(T* p, T b, T c, T diff) {
+template <typename P, typename T>
+__device__ __forceinline__ T psi_upd(P p, T b, T c, T diff) {
   const T v = b * (*p) + c * diff;
   *p = v;
   return v;
@@ -928,16 +943,17 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
     int dj = 0, cj = 0, dk = 0, ck = 0;
     const int sj = jin ? slab_e(j, jb, je, N, dj, cj) : -1;
     const int sk = kin ? slab_e(k, kb, ke, N, dk, ck) : -1;
-    const T iky = jin ? s.ike[1][j] : T(0);
-    const T ikz = kin ? s.ike[2][k] : T(0);
-    const T bej = sj >= 0 ? s.be[cj] : T(0), cej = sj >= 0 ? s.ceo[cj] : T(0);
-    const T bek = sk >= 0 ? s.be[ck] : T(0), cek = sk >= 0 ? s.ceo[ck] : T(0);
+    constexpr unsigned es = sizeof(T);
+    const T iky = jin ? ldg<T>(s.ike[1], j * es) : T(0);
+    const T ikz = kin ? ldg<T>(s.ike[2], k * es) : T(0);
+    const T bej = sj >= 0 ? ldg<T>(s.be, cj * es) : T(0), cej = sj >= 0 ? ldg<T>(s.ceo, cj * es) : T(0);
+    const T bek = sk >= 0 ? ldg<T>(s.be, ck * es) : T(0), cek = sk >= 0 ? ldg<T>(s.ceo, ck * es) : T(0);
     T* __restrict__ pExy = sj >= 0 ? s.psi[psi_id(F_EX, 1, sj)] : nullptr;
     T* __restrict__ pEzy = sj >= 0 ? s.psi[psi_id(F_EZ, 1, sj)] : nullptr;
     T* __restrict__ pExz = sk >= 0 ? s.psi[psi_id(F_EX, 2, sk)] : nullptr;
     T* __restrict__ pEyz = sk >= 0 ? s.psi[psi_id(F_EY, 2, sk)] : nullptr;
     const bool jksrc = s.src_field == F_EY && j >= s.sy0 && j < s.sy1 && k >= s.sz0 && k < s.sz1;
-    const T jv = jksrc ? s.jsrc[n] : T(0);
+    const T jv = jksrc ? ldg<T>(s.jsrc, n * es) : T(0);
     const bool jcap = j >= s.wy0 && j <= s.wy1;
     const int cap = !jcap ? -1 : (k == s.port[0] ? 0 : (k == s.port[1] ? 1 : -1));
     const int cap2 = (jcap && s.port[0] == s.port[1] && k == s.port[0]) ? 1 : -1;
@@ -950,28 +966,29 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
     for (int u = 0; u <= XC; ++u) {
       const int i = min(i0 + u, nx - 1);
       const unsigned idx = (unsigned)i * pl + jk;
-      hy[u] = Hy[idx];
-      hz[u] = Hz[idx];
+      const unsigned bo = idx * es;
+      hy[u] = ldg<T>(Hy, bo);
+      hz[u] = ldg<T>(Hz, bo);
       if (u < XC) {
-        ex[u] = Ex[idx];
-        ey[u] = Ey[idx];
-        ez[u] = Ez[idx];
-        hx[u] = Hx[idx];
-        hxk[u] = Hx[idx + offk];
-        hxj[u] = Hx[idx + offj];
-        hyk[u] = Hy[idx + offk];
-        hzj[u] = Hz[idx + offj];
-        m[u] = mat[idx];
-        const size_t iy = ((size_t)i * N + dj) * nz + k, iz = ((size_t)i * ny + j) * N + dk;
-        qxy[u] = pExy ? pExy[iy] : T(0);
-        qzy[u] = pEzy ? pEzy[iy] : T(0);
-        qxz[u] = pExz ? pExz[iz] : T(0);
-        qyz[u] = pEyz ? pEyz[iz] : T(0);
+        ex[u] = ldg<T>(Ex, bo);
+        ey[u] = ldg<T>(Ey, bo);
+        ez[u] = ldg<T>(Ez, bo);
+        hx[u] = ldg<T>(Hx, bo);
+        hxk[u] = ldg<T>(Hx, bo + offk * es);
+        hxj[u] = ldg<T>(Hx, bo + offj * es);
+        hyk[u] = ldg<T>(Hy, bo + offk * es);
+        hzj[u] = ldg<T>(Hz, bo + offj * es);
+        m[u] = ldg<uint8_t>(mat, idx);
+        const unsigned iy = (((unsigned)i * N + dj) * nz + k) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+        qxy[u] = pExy ? ldg<T>(pExy, iy) : T(0);
+        qzy[u] = pEzy ? ldg<T>(pEzy, iy) : T(0);
+        qxz[u] = pExz ? ldg<T>(pExz, iz) : T(0);
+        qyz[u] = pEyz ? ldg<T>(pEyz, iz) : T(0);
         int di = 0, ci = 0;
         const int si = slab_e(i, ib, ie, N, di, ci);
-        const size_t ix = ((size_t)di * ny + j) * nz + k;
-        qyx[u] = si >= 0 ? s.psi[psi_id(F_EY, 0, si)][ix] : T(0);
-        qzx[u] = si >= 0 ? s.psi[psi_id(F_EZ, 0, si)][ix] : T(0);
+        const unsigned ix = (((unsigned)di * ny + j) * nz + k) * es;
+        qyx[u] = si >= 0 ? ldg<T>(s.psi[psi_id(F_EY, 0, si)], ix) : T(0);
+        qzx[u] = si >= 0 ? ldg<T>(s.psi[psi_id(F_EZ, 0, si)], ix) : T(0);
       }
     }
 #pragma unroll
@@ -988,39 +1005,49 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
       if (!(bx || by || bz || src || icap)) continue;
       int di = 0, ci = 0;
       const int si = slab_e(i, ib, ie, N, di, ci);
-      const size_t iy = ((size_t)i * N + dj) * nz + k, iz = ((size_t)i * ny + j) * N + dk;
-      const size_t ix = ((size_t)di * ny + j) * nz + k;
+      const unsigned iy = (((unsigned)i * N + dj) * nz + k) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+      const unsigned ix = (((unsigned)di * ny + j) * nz + k) * es;
+      const unsigned bo = idx * es;
+      const T ikx = ldg<T>(s.ike[0], i * es);
       T vx = ex[u], vy = ey[u];
       if (bx) {
         const T cb = cbsel(s, m[u], 0);
         const T dHz = hzj[u] - hz[u];
         const T dHy = hyk[u] - hy[u];
         vx = vx + cb * (dHz * iky - dHy * ikz);
-        if (sj >= 0) { const T p = bej * qxy[u] + cej * dHz; pExy[iy] = p; vx += cb * p; }
-        if (sk >= 0) { const T p = bek * qxz[u] + cek * dHy; pExz[iz] = p; vx -= cb * p; }
-        Ex[idx] = vx;
+        if (sj >= 0) { const T p = bej * qxy[u] + cej * dHz; stg<T>(pExy, iy, p); vx += cb * p; }
+        if (sk >= 0) { const T p = bek * qxz[u] + cek * dHy; stg<T>(pExz, iz, p); vx -= cb * p; }
+        stg<T>(Ex, bo, vx);
       }
       if (by) {
         const T cb = cbsel(s, m[u], 1);
         const T dHx = hxk[u] - hx[u];
         const T dHz = hz[u + 1] - hz[u];
-        vy = vy + cb * (dHx * ikz - dHz * s.ike[0][i]);
+        vy = vy + cb * (dHx * ikz - dHz * ikx);
         if (src) vy -= jv;
-        if (si >= 0) { const T p = s.be[ci] * qyx[u] + s.ceo[ci] * dHz; s.psi[psi_id(F_EY, 0, si)][ix] = p; vy -= cb * p; }
-        if (sk >= 0) { const T p = bek * qyz[u] + cek * dHx; pEyz[iz] = p; vy += cb * p; }
-        Ey[idx] = vy;
+        if (si >= 0) {
+          const T p = ldg<T>(s.be, ci * es) * qyx[u] + ldg<T>(s.ceo, ci * es) * dHz;
+          stg<T>(s.psi[psi_id(F_EY, 0, si)], ix, p);
+          vy -= cb * p;
+        }
+        if (sk >= 0) { const T p = bek * qyz[u] + cek * dHx; stg<T>(pEyz, iz, p); vy += cb * p; }
+        stg<T>(Ey, bo, vy);
       } else if (src) {
         vy -= jv;
-        Ey[idx] = vy;
+        stg<T>(Ey, bo, vy);
       }
       if (bz) {
         const T cb = cbsel(s, m[u], 2);
         const T dHy = hy[u + 1] - hy[u];
         const T dHx = hxj[u] - hx[u];
-        T v = ez[u] + cb * (dHy * s.ike[0][i] - dHx * iky);
-        if (si >= 0) { const T p = s.be[ci] * qzx[u] + s.ceo[ci] * dHy; s.psi[psi_id(F_EZ, 0, si)][ix] = p; v += cb * p; }
-        if (sj >= 0) { const T p = bej * qzy[u] + cej * dHx; pEzy[iy] = p; v -= cb * p; }
-        Ez[idx] = v;
+        T v = ez[u] + cb * (dHy * ikx - dHx * iky);
+        if (si >= 0) {
+          const T p = ldg<T>(s.be, ci * es) * qzx[u] + ldg<T>(s.ceo, ci * es) * dHy;
+          stg<T>(s.psi[psi_id(F_EZ, 0, si)], ix, p);
+          v += cb * p;
+        }
+        if (sj >= 0) { const T p = bej * qzy[u] + cej * dHx; stg<T>(pEzy, iy, p); v -= cb * p; }
+        stg<T>(Ez, bo, v);
       }
       if (icap) {
         const int c = (i - s.wx0) * s.prow + (j - s.wy0);
@@ -1060,16 +1087,17 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
     int dj = 0, cj = 0, dk = 0, ck = 0;
     const int sj = jin1 ? slab_h(j, jb, je, N, dj, cj) : -1;
     const int sk = kin1 ? slab_h(k, kb, ke, N, dk, ck) : -1;
-    const T iky = jin1 ? s.ikh[1][j] : T(0);
-    const T ikz = kin1 ? s.ikh[2][k] : T(0);
-    const T bhj = sj >= 0 ? s.bh[cj] : T(0), chj = sj >= 0 ? s.cho[cj] : T(0);
-    const T bhk = sk >= 0 ? s.bh[ck] : T(0), chk = sk >= 0 ? s.cho[ck] : T(0);
+    constexpr unsigned es = sizeof(T);
+    const T iky = jin1 ? ldg<T>(s.ikh[1], j * es) : T(0);
+    const T ikz = kin1 ? ldg<T>(s.ikh[2], k * es) : T(0);
+    const T bhj = sj >= 0 ? ldg<T>(s.bh, cj * es) : T(0), chj = sj >= 0 ? ldg<T>(s.cho, cj * es) : T(0);
+    const T bhk = sk >= 0 ? ldg<T>(s.bh, ck * es) : T(0), chk = sk >= 0 ? ldg<T>(s.cho, ck * es) : T(0);
     T* __restrict__ pHxy = sj >= 0 ? s.psi[psi_id(F_HX, 1, sj)] : nullptr;
     T* __restrict__ pHzy = sj >= 0 ? s.psi[psi_id(F_HZ, 1, sj)] : nullptr;
     T* __restrict__ pHxz = sk >= 0 ? s.psi[psi_id(F_HX, 2, sk)] : nullptr;
     T* __restrict__ pHyz = sk >= 0 ? s.psi[psi_id(F_HY, 2, sk)] : nullptr;
     const bool jksrc = s.src_field == F_HY && j >= s.sy0 && j < s.sy1 && k >= s.sz0 && k < s.sz1;
-    const T jv = jksrc ? s.jsrc[n] : T(0);
+    const T jv = jksrc ? ldg<T>(s.jsrc, n * es) : T(0);
     const bool jcap = j >= s.wy0 && j <= s.wy1;
     int capA = -1, capB = -1, capC = -1, capD = -1;
     if (jcap) {
@@ -1087,29 +1115,30 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
 #pragma unroll
     for (int u = 0; u <= XC; ++u) {
       const unsigned pidx = (unsigned)min(max(i0 - 1 + u, 0), nx - 1) * pl + jk;
-      ey[u] = Ey[pidx];
-      ez[u] = Ez[pidx];
+      ey[u] = ldg<T>(Ey, pidx * es);
+      ez[u] = ldg<T>(Ez, pidx * es);
       if (u < XC) {
         const int i = min(i0 + u, nx - 1);
         const unsigned idx = (unsigned)i * pl + jk;
-        hx[u] = Hx[idx];
-        hy[u] = Hy[idx];
-        hz[u] = Hz[idx];
-        ex[u] = Ex[idx];
-        exk[u] = Ex[idx - offk];
-        exj[u] = Ex[idx - offj];
-        eyk[u] = Ey[idx - offk];
-        ezj[u] = Ez[idx - offj];
-        const size_t iy = ((size_t)i * N + dj) * nz + k, iz = ((size_t)i * ny + j) * N + dk;
-        qxy[u] = pHxy ? pHxy[iy] : T(0);
-        qzy[u] = pHzy ? pHzy[iy] : T(0);
-        qxz[u] = pHxz ? pHxz[iz] : T(0);
-        qyz[u] = pHyz ? pHyz[iz] : T(0);
+        const unsigned bo = idx * es;
+        hx[u] = ldg<T>(Hx, bo);
+        hy[u] = ldg<T>(Hy, bo);
+        hz[u] = ldg<T>(Hz, bo);
+        ex[u] = ldg<T>(Ex, bo);
+        exk[u] = ldg<T>(Ex, bo - offk * es);
+        exj[u] = ldg<T>(Ex, bo - offj * es);
+        eyk[u] = ldg<T>(Ey, bo - offk * es);
+        ezj[u] = ldg<T>(Ez, bo - offj * es);
+        const unsigned iy = (((unsigned)i * N + dj) * nz + k) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+        qxy[u] = pHxy ? ldg<T>(pHxy, iy) : T(0);
+        qzy[u] = pHzy ? ldg<T>(pHzy, iy) : T(0);
+        qxz[u] = pHxz ? ldg<T>(pHxz, iz) : T(0);
+        qyz[u] = pHyz ? ldg<T>(pHyz, iz) : T(0);
         int di = 0, ci = 0;
         const int si = slab_h(i, ib, ie, N, di, ci);
-        const size_t ix = ((size_t)di * ny + j) * nz + k;
-        qyx[u] = si >= 0 ? s.psi[psi_id(F_HY, 0, si)][ix] : T(0);
-        qzx[u] = si >= 0 ? s.psi[psi_id(F_HZ, 0, si)][ix] : T(0);
+        const unsigned ix = (((unsigned)di * ny + j) * nz + k) * es;
+        qyx[u] = si >= 0 ? ldg<T>(s.psi[psi_id(F_HY, 0, si)], ix) : T(0);
+        qzx[u] = si >= 0 ? ldg<T>(s.psi[psi_id(F_HZ, 0, si)], ix) : T(0);
       }
     }
 #pragma unroll
@@ -1126,35 +1155,45 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
       if (!(bx || by || bz || src || icap)) continue;
       int di = 0, ci = 0;
       const int si = slab_h(i, ib, ie, N, di, ci);
-      const size_t iy = ((size_t)i * N + dj) * nz + k, iz = ((size_t)i * ny + j) * N + dk;
-      const size_t ix = ((size_t)di * ny + j) * nz + k;
+      const unsigned iy = (((unsigned)i * N + dj) * nz + k) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+      const unsigned ix = (((unsigned)di * ny + j) * nz + k) * es;
+      const unsigned bo = idx * es;
+      const T ikx = ldg<T>(s.ikh[0], i * es);
       const T eyc = ey[u + 1], ezc = ez[u + 1];
       T vx = hx[u];
       if (bx) {
         const T dEy = eyc - eyk[u];
         const T dEz = ezc - ezj[u];
         vx = vx + db * (dEy * ikz - dEz * iky);
-        if (sj >= 0) { const T p = bhj * qxy[u] + chj * dEz; pHxy[iy] = p; vx -= db * p; }
-        if (sk >= 0) { const T p = bhk * qxz[u] + chk * dEy; pHxz[iz] = p; vx += db * p; }
-        Hx[idx] = vx;
+        if (sj >= 0) { const T p = bhj * qxy[u] + chj * dEz; stg<T>(pHxy, iy, p); vx -= db * p; }
+        if (sk >= 0) { const T p = bhk * qxz[u] + chk * dEy; stg<T>(pHxz, iz, p); vx += db * p; }
+        stg<T>(Hx, bo, vx);
       }
       T vy = hy[u];
       if (src) vy -= jv;
       if (by) {
         const T dEz = ezc - ez[u];
         const T dEx = ex[u] - exk[u];
-        vy = vy + db * (dEz * s.ikh[0][i] - dEx * ikz);
-        if (si >= 0) { const T p = s.bh[ci] * qyx[u] + s.cho[ci] * dEz; s.psi[psi_id(F_HY, 0, si)][ix] = p; vy += db * p; }
-        if (sk >= 0) { const T p = bhk * qyz[u] + chk * dEx; pHyz[iz] = p; vy -= db * p; }
+        vy = vy + db * (dEz * ikx - dEx * ikz);
+        if (si >= 0) {
+          const T p = ldg<T>(s.bh, ci * es) * qyx[u] + ldg<T>(s.cho, ci * es) * dEz;
+          stg<T>(s.psi[psi_id(F_HY, 0, si)], ix, p);
+          vy += db * p;
+        }
+        if (sk >= 0) { const T p = bhk * qyz[u] + chk * dEx; stg<T>(pHyz, iz, p); vy -= db * p; }
       }
-      if (by || src) Hy[idx] = vy;
+      if (by || src) stg<T>(Hy, bo, vy);
       if (bz) {
         const T dEx = ex[u] - exj[u];
         const T dEy = eyc - ey[u];
-        T v = hz[u] + db * (dEx * iky - dEy * s.ikh[0][i]);
-        if (si >= 0) { const T p = s.bh[ci] * qzx[u] + s.cho[ci] * dEy; s.psi[psi_id(F_HZ, 0, si)][ix] = p; v -= db * p; }
-        if (sj >= 0) { const T p = bhj * qzy[u] + chj * dEx; pHzy[iy] = p; v += db * p; }
-        Hz[idx] = v;
+        T v = hz[u] + db * (dEx * iky - dEy * ikx);
+        if (si >= 0) {
+          const T p = ldg<T>(s.bh, ci * es) * qzx[u] + ldg<T>(s.cho, ci * es) * dEy;
+          stg<T>(s.psi[psi_id(F_HZ, 0, si)], ix, p);
+          v -= db * p;
+        }
+        if (sj >= 0) { const T p = bhj * qzy[u] + chj * dEx; stg<T>(pHzy, iy, p); v += db * p; }
+        stg<T>(Hz, bo, v);
       }
       if (icap) {
         const int c = (i - s.wx0) * s.prow + (j - s.wy0);
@@ -1264,7 +1303,7 @@ __device__ __forceinline__ TmzJ<T> tmz_jcoef(const TmzView<T>& v, const TmzWhere
 template <typename T>
 __device__ __forceinline__ TmzIn<T> tmz_load(const TmzView<T>& v, int i, int j, const TmzWhere& f,
                                              const T* __restrict__ EzA, const T* __restrict__ HxA,
-                                             const T* __restrict__ HyA, T* const* psiA, bool load_hy,
+                                             const T* __restrict__ HyA, OIDT_GLOBAL T* const* psiA, bool load_hy,
                                              const T* pzy_lane = nullptr) {
   TmzIn<T> a;
   const int nx = v.nx, ny = v.ny;
@@ -1320,7 +1359,7 @@ __device__ __forceinline__ TmzE<T> tmz_e(const TmzView<T>& v, const TmzWhere& f,
 
 template <typename T>
 __device__ __forceinline__ T tmz_ez_cell(const TmzView<T>& v, int i, int j, T jv, const T* __restrict__ EzA,
-                                         const T* __restrict__ HxA, const T* __restrict__ HyA, T* const* psiA) {
+                                         const T* __restrict__ HxA, const T* __restrict__ HyA, OIDT_GLOBAL T* const* psiA) {
   const TmzWhere f = tmz_where(v, i, j);
   return tmz_e(v, f, tmz_load(v, i, j, f, EzA, HxA, HyA, psiA, true), tmz_jcoef(v, f, j), jv).ez;
 }
@@ -1373,8 +1412,8 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
     T* __restrict__ EzB = odd ? s.f[F_EZ] : s.f2[F_EZ];
     T* __restrict__ HxB = odd ? s.f[F_HX] : s.f2[F_HX];
     T* __restrict__ HyB = odd ? s.f[F_HY] : s.f2[F_HY];
-    T* const* psiAll = odd ? s.psi2 : s.psi;
-    T* const* psiBll = odd ? s.psi : s.psi2;
+    OIDT_GLOBAL T* const* psiAll = odd ? s.psi2 : s.psi;
+    OIDT_GLOBAL T* const* psiBll = odd ? s.psi : s.psi2;
     const T jv = v.src_ez ? s.jsrc[n] : T(0);
     const int nx = v.nx, ny = v.ny, N = v.N;
     const int i0 = tl.y, j0 = tl.z;
@@ -1595,7 +1634,7 @@ struct TezIn {
 template <typename T>
 __device__ __forceinline__ TezIn<T> tez_load(const TezView<T>& v, int i, int j, const TezWhere& f,
                                              const T* __restrict__ ExA, const T* __restrict__ EyA,
-                                             const T* __restrict__ HzA, T* const* psiA) {
+                                             const T* __restrict__ HzA, OIDT_GLOBAL T* const* psiA) {
   TezIn<T> a;
   const int nx = v.nx, ny = v.ny;
   const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);
@@ -1657,7 +1696,7 @@ __device__ __forceinline__ TezE<T> tez_e(const TezView<T>& v, const TezWhere& f,
 template <typename T>
 __device__ __forceinline__ TezE<T> tez_e_cell(const TezView<T>& v, int i, int j, const T* __restrict__ ExA,
                                               const T* __restrict__ EyA, const T* __restrict__ HzA,
-                                              T* const* psiA) {
+                                              OIDT_GLOBAL T* const* psiA) {
   const TezWhere f = tez_where(v, i, j);
   return tez_e(v, f, tez_load(v, i, j, f, ExA, EyA, HzA, psiA));
 }
@@ -1689,8 +1728,8 @@ __global__ __launch_bounds__(kThreads) void k_tez_step(const Slot<T>* __restrict
     T* __restrict__ ExB = odd ? s.f[F_EX] : s.f2[F_EX];
     T* __restrict__ EyB = odd ? s.f[F_EY] : s.f2[F_EY];
     T* __restrict__ HzB = odd ? s.f[F_HZ] : s.f2[F_HZ];
-    T* const* psiAll = odd ? s.psi2 : s.psi;
-    T* const* psiBll = odd ? s.psi : s.psi2;
+    OIDT_GLOBAL T* const* psiAll = odd ? s.psi2 : s.psi;
+    OIDT_GLOBAL T* const* psiBll = odd ? s.psi : s.psi2;
     const T jv = v.src_hz ? s.jsrc[n] : T(0);
     const int nx = v.nx, ny = v.ny, N = v.N;
     const int i0 = tl.y, j0 = tl.z;
@@ -1946,7 +1985,7 @@ __device__ __forceinline__ E3In<T> e3_load(const View3<T>& v, int i, int j, int 
                                            const T* __restrict__ Ex, const T* __restrict__ Ey,
                                            const T* __restrict__ Ez, const T* __restrict__ Hx,
                                            const T* __restrict__ Hy, const T* __restrict__ Hz,
-                                           T* const* psiA) {
+                                           OIDT_GLOBAL T* const* psiA) {
   E3In<T> a;
   const int nx = v.nx, ny = v.ny, nz = v.nz, N = v.N;
   const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
@@ -2081,7 +2120,7 @@ struct H3In {
 
 template <typename T>
 __device__ __forceinline__ H3In<T> h3_load(const View3<T>& v, int i, int j, int k, const H3Where& h,
-                                           T* const* psiA) {
+                                           OIDT_GLOBAL T* const* psiA) {
   H3In<T> q;
   const int ny = v.ny, nz = v.nz, N = v.N;
   const int ic = min(max(i, 0), v.nx - 1), jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
@@ -2115,16 +2154,16 @@ __global__ __launch_bounds__(kThreads) void k_3d_step(const Slot<T>* __restrict_
     if (!s.active || n >= s.nt) continue;
     const View3<T> v = view3(s);
     const bool odd = (n & 1) != 0;
-    T* const* FA = odd ? s.f2 : s.f;
-    T* const* FB = odd ? s.f : s.f2;
+    OIDT_GLOBAL T* const* FA = odd ? s.f2 : s.f;
+    OIDT_GLOBAL T* const* FB = odd ? s.f : s.f2;
     const T* __restrict__ Ex = FA[F_EX];
     const T* __restrict__ Ey = FA[F_EY];
     const T* __restrict__ Ez = FA[F_EZ];
     const T* __restrict__ Hx = FA[F_HX];
     const T* __restrict__ Hy = FA[F_HY];
     const T* __restrict__ Hz = FA[F_HZ];
-    T* const* psiA = odd ? s.psi2 : s.psi;
-    T* const* psiB = odd ? s.psi : s.psi2;
+    OIDT_GLOBAL T* const* psiA = odd ? s.psi2 : s.psi;
+    OIDT_GLOBAL T* const* psiB = odd ? s.psi : s.psi2;
     const T jvE = v.src_field == F_EY ? s.jsrc[n] : T(0);
     const T jvH = v.src_field == F_HY ? s.jsrc[n] : T(0);
     const int nx = v.nx, ny = v.ny, nz = v.nz, N = v.N;
@@ -2422,7 +2461,7 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
     T smp[kChunk];
 #pragma unroll
     for (int t = 0; t < kChunk; ++t)
-      smp[t] = (t < tv) ? sample_at(s, s.ring + (size_t)t * s.nplanes * s.pcells, q, c) : T(0);
+      smp[t] = (t < tv) ? sample_at<T>(s, s.ring + (size_t)t * s.nplanes * s.pcells, q, c) : T(0);
     const int fend = min(kFB, s.nf - f0);
     for (int fl = 0; fl < fend; ++fl) {
       double re = 0.0, im = 0.0;

@@ -6,6 +6,18 @@
 
 namespace oidt {
 
+// Every pointer in a run descriptor points into device-global memory.  The
+// device pass says so in the type (address space 1): loads and stores through
+// them, and through the locals initialised from them, compile to global
+// instructions rather than flat ones (which take 64-bit per-lane addresses
+// and count on both the vector-memory and the LDS counter).  The host pass
+// sees plain pointers; the layout is the same (8-byte pointers).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define OIDT_GLOBAL __attribute__((address_space(1)))
+#else
+#define OIDT_GLOBAL
+#endif
+
 // Steps per DFT chunk: the stencil kernels record port-plane samples into a
 // ring of this many steps; the DFT kernel folds a whole chunk into the FP64
 // accumulators at once.
@@ -58,22 +70,22 @@ struct Slot {
   T dxT;    // delta_x in the run precision (lossy TMz: curl_h * dx)
   T dtmu;   // delta_t / mu in the run precision (lossy TMz H CPML)
   double dt;
-  T* f[6];       // field set A (current state of the two-kernel forms)
-  T* psi[36];
-  T* f2[6];      // field set B: the fused step reads one set and writes the other
-  T* psi2[36];
+  OIDT_GLOBAL T* f[6];       // field set A (current state of the two-kernel forms)
+  OIDT_GLOBAL T* psi[36];
+  OIDT_GLOBAL T* f2[6];      // field set B: the fused step reads one set and writes the other
+  OIDT_GLOBAL T* psi2[36];
   T cbv[3][kMatMax];    // distinct cb values of Ex, Ey, Ez
-  const uint8_t* mat;   // per cell: bits 2c..2c+1 index cbv[c]
-  const T* ike[3];      // 1 / (kappa_e * dx) per axis
-  const T* ikh[3];      // 1 / (kappa_h * dx) per axis
-  const T* be;
-  const T* ceo;         // ce / dx
-  const T* bh;
-  const T* cho;         // ch / dx
-  const T* jsrc;        // [nt]
-  T* ring;              // [kChunk][nplanes][pcells]
-  double* acc;          // [nq][nf][wcells][2]
-  const double* w;      // [nf] angular frequency 2*pi*f
+  OIDT_GLOBAL const uint8_t* mat;   // per cell: bits 2c..2c+1 index cbv[c]
+  OIDT_GLOBAL const T* ike[3];      // 1 / (kappa_e * dx) per axis
+  OIDT_GLOBAL const T* ikh[3];      // 1 / (kappa_h * dx) per axis
+  OIDT_GLOBAL const T* be;
+  OIDT_GLOBAL const T* ceo;         // ce / dx
+  OIDT_GLOBAL const T* bh;
+  OIDT_GLOBAL const T* cho;         // ch / dx
+  OIDT_GLOBAL const T* jsrc;        // [nt]
+  OIDT_GLOBAL T* ring;              // [kChunk][nplanes][pcells]
+  OIDT_GLOBAL double* acc;          // [nq][nf][wcells][2]
+  OIDT_GLOBAL const double* w;      // [nf] angular frequency 2*pi*f
 };
 
 // Field monitor (loss and impedance study drivers): the scaled stencil sum of
@@ -81,9 +93,9 @@ struct Slot {
 // running DFT (mode 0) or stored per step (mode 1).
 template <typename T>
 struct Monitor {
-  T* re;              // mode 0: real part [cells]; mode 1: records [nrec][cells]
-  T* im;              // mode 0: imaginary part [cells]
-  const double* cs;   // mode 0: (cos, sin) weight of each record [nrec][2]
+  OIDT_GLOBAL T* re;              // mode 0: real part [cells]; mode 1: records [nrec][cells]
+  OIDT_GLOBAL T* im;              // mode 0: imaginary part [cells]
+  OIDT_GLOBAL const double* cs;   // mode 0: (cos, sin) weight of each record [nrec][2]
   double scale;
   int slot, field, mode, npts;
   int ox[4], oy[4], oz[4];

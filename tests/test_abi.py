@@ -82,16 +82,16 @@ def test_footprint_scales_with_grid():
     assert 250e6 < c < 400e6
 
 
-def test_padded_cell_count_limit():
-    """The kernels index the padded layout (unit-stride axis rounded up to
-    128 bytes) with 32-bit offsets: a grid under 2^32 real cells whose padded
-    extent is over it is refused before anything is allocated."""
-    d = _desc(ny=33, nx=70_000_000)          # 2.3e9 cells, 4.5e9 once y pads to 64
+def test_padded_field_size_limit():
+    """The kernels address a field with a 32-bit byte offset into the padded
+    layout (unit-stride axis rounded up to 128 bytes): a grid whose FP32 field
+    fits 4 GiB only before padding is refused before anything is allocated."""
+    d = _desc(ny=33, nx=17_000_000)          # 2.2 GB of cells, 4.35 GB once y pads to 64
     d.port[0], d.port[1] = 20, 40
     rc = _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64()))
     assert rc == _lib.EINVAL
     assert 'padding' in _lib.lib().oidt_last_error().decode()
-    d = _desc(ny=64, nx=40_000_000)           # 2.6e9 cells, no padding needed
+    d = _desc(ny=64, nx=15_000_000)           # 3.84 GB per field, no padding needed
     d.port[0], d.port[1] = 20, 40
     assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.OK
 
