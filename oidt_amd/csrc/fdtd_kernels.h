@@ -924,7 +924,8 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
                                                     const int* __restrict__ counts, int tstep) {
   OIDT_TILE_PROLOGUE
     const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
-    const int k = tl.w + threadIdx.x, j = tl.z + threadIdx.y;
+    // a wave is one y row: j is wave-uniform (scalar registers)
+    const int k = tl.w + threadIdx.x, j = tl.z + wave_id();
     if (j >= ny || k >= nz) continue;
     const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
     T* __restrict__ Ex = s.f[F_EX];
@@ -1068,7 +1069,8 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
                                                     const int* __restrict__ counts, int tstep) {
   OIDT_TILE_PROLOGUE
     const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
-    const int k = tl.w + threadIdx.x, j = tl.z + threadIdx.y;
+    // a wave is one y row: j is wave-uniform (scalar registers)
+    const int k = tl.w + threadIdx.x, j = tl.z + wave_id();
     if (j >= ny || k >= nz) continue;
     const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
     const T* __restrict__ Ex = s.f[F_EX];
