@@ -1447,11 +1447,12 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
       TmzPlane<T> p;
       const TmzX x = tmz_x(v, i);
       const unsigned idx = (unsigned)i * ny + jc;
-      p.ez = EzA[idx];
-      p.hx = HxA[idx];
-      p.hxj = HxA[jc + 1 < ny ? idx + 1 : idx];
-      p.hyi = HyA[i + 1 < nx ? idx + ny : idx];
-      p.m = v.mat[idx];
+      constexpr unsigned es = sizeof(T);
+      p.ez = ldg<T>(EzA, idx * es);
+      p.hx = ldg<T>(HxA, idx * es);
+      p.hxj = ldg<T>(HxA, (jc + 1 < ny ? idx + 1 : idx) * es);
+      p.hyi = ldg<T>(HyA, (i + 1 < nx ? idx + ny : idx) * es);
+      p.m = ldg<uint8_t>(v.mat, idx);
       const bool bulk = x.ibulk && jbulk;
       p.pzx = bulk && x.sxe >= 0 ? psiAll[psi_id(F_EZ, 0, x.sxe)][(size_t)x.dxe * ny + jc] : T(0);
       p.pzy = bulk && sye >= 0 ? pzyA[(size_t)i * N + dye] : T(0);
@@ -1510,11 +1511,11 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
       const TmzE<T> e = tmz_e(v, f, a, jcf, jv);
       const unsigned idx = (unsigned)i * ny + jc;
       if (f.bulk) {
-        __builtin_nontemporal_store(e.ez, EzB + idx);
+        __builtin_nontemporal_store(e.ez, (OIDT_GLOBAL T*)(reinterpret_cast<char*>(EzB) + idx * (unsigned)sizeof(T)));
         if (f.sx >= 0) psiBll[psi_id(F_EZ, 0, f.sx)][(size_t)f.dx * ny + j] = e.pzx;
         if (f.sy >= 0) pzyB[(size_t)i * N + f.dy] = e.pzy;
       } else if (f.src) {
-        __builtin_nontemporal_store(e.ez, EzB + idx);
+        __builtin_nontemporal_store(e.ez, (OIDT_GLOBAL T*)(reinterpret_cast<char*>(EzB) + idx * (unsigned)sizeof(T)));
       }
       if (jcap) {
         const size_t c = (size_t)(j - v.wy0);
@@ -1533,7 +1534,7 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
             pxyB[(size_t)i * N + dyh] = q;
             hv -= db * q;
           }
-          __builtin_nontemporal_store(hv, HxB + idx);
+          __builtin_nontemporal_store(hv, (OIDT_GLOBAL T*)(reinterpret_cast<char*>(HxB) + idx * (unsigned)sizeof(T)));
         }
         T hv = hyC;
         if (x.hbulk && jbulk) {
@@ -1544,7 +1545,7 @@ __global__ __launch_bounds__(kThreads) void k_tmz_step(const Slot<T>* __restrict
             psiBll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + j] = q;
             hv += db * q;
           }
-          __builtin_nontemporal_store(hv, HyB + idx);
+          __builtin_nontemporal_store(hv, (OIDT_GLOBAL T*)(reinterpret_cast<char*>(HyB) + idx * (unsigned)sizeof(T)));
         }
         if (jcap) {
           const size_t c = (size_t)(j - v.wy0);
