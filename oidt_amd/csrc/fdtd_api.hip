@@ -847,7 +847,7 @@ int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt
   b->precision = precision;
   b->max_slots = max_slots;
   // measured defaults on MI355X: 3D is best with one cell per thread along x
-  // (54 VGPRs, full occupancy), 2D with four
+  // (44 VGPRs, full occupancy), 2D with four
   b->xchunk = scheme == SCHEME_3D ? 1 : 4;
   b->slots.resize(max_slots);
   auto bad = [&](hipError_t e, const char* what) {
