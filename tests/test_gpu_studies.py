@@ -123,3 +123,38 @@ def test_study_realisations_batched_equal_alone(scheme):
             np.testing.assert_array_equal(got[k], alone[k], err_msg=(c.study, c.profile_number, k))
     assert not np.array_equal(together[0]['ez' if scheme == 'tm' else 'hz'],
                               together[1]['ez' if scheme == 'tm' else 'hz'])
+
+
+@pytest.mark.gpu
+def test_fp64_monitors_match_oracle_fields():
+    """Monitors in a double-precision run: a line record (2-point average)
+    and a whole-grid DFT (weights of the caller's choosing) against the FP64
+    S-parameter oracle's own fields, step by step, to 1e-10."""
+    from oidt_amd import configs
+    from helpers import device_inputs
+    from oracle import fdtd as ofdtd, geometry as ogeo, setup as osetup
+    p = configs.small('te', 'float64', nt=96)
+    s, up, lo, fg = device_inputs(p)
+    nt, cx = s.nt, s.nx // 2
+    w = np.linspace(0.0, 3.0, nt)
+    cs = np.stack([np.cos(w), np.sin(w)], 1)
+    sol = Solver(s, fg)
+    try:
+        i_line = sol.add_monitor(Monitor('hz', [(0, 0), (1, 0)], 0.5, [(cx, cx + 1), (0, s.ny)], nt))
+        i_dft = sol.add_monitor(Monitor('ey', [(0, 0)], 1.0, [(0, s.nx), (0, s.ny)], nt, cs=cs))
+        sol.run()
+        line = sol.read_monitor(i_line)[:, 0, :, 0]
+        re, im = sol.read_monitor(i_dft)
+    finally:
+        sol.close()
+    c = osetup.explicit_setup(p)
+    o = ofdtd.make(c, ogeo.fg_mask(c, up, lo))
+    ref_line = np.zeros((nt, s.ny))
+    ref = np.zeros((s.nx, s.ny), complex)
+    for n in range(nt):
+        o.step(n)
+        ref_line[n] = 0.5 * (o.Hz[cx] + o.Hz[cx + 1])
+        ref += o.Ey * (cs[n, 0] + 1j * cs[n, 1])
+    assert np.abs(ref_line).max() > 0
+    assert rel_l2(line, ref_line) < 1e-10
+    assert rel_l2(re[:, :, 0] + 1j * im[:, :, 0], ref) < 1e-10
