@@ -24,10 +24,12 @@ namespace oidt {
 
 // Blocks are (64, 4): threadIdx.y is the wavefront index, the same for every
 // lane of a wave.  Reading it through readfirstlane lets the compiler keep
-// everything derived from it in scalar registers.  The marching fused step
-// gains from that (its per-plane values become scalars); the half-step
-// kernels measured slower with it (their CPML branches turn into scalar
-// branches between the loads), so they read threadIdx.y directly.
+// everything derived from it in scalar registers: the fused steps take their
+// tile index from it, and the 3D half-step kernels their y row (so the y
+// conditions and the scalar part of every address are computed once per
+// wave).  Splitting those kernels further into an interior body without the
+// per-lane z conditions measured slower (its remaining conditions become
+// scalar branches between the loads), so they keep one body.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
 
 // Cross-lane moves that stay in registers (no LDS, so no wait on the

@@ -73,6 +73,45 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('precision,xc,direction,extra', [
+    ('float64', 1, 'f', {}),                                   # runs [64,128) and [128,192) outside the slabs
+    ('float64', 2, 'b', {'source_condition': 'TM'}),
+    ('float64', 1, 'f', {'sz': 70}),                           # source slab inside the first run
+    ('float32', 4, 'f', {'p1': 63, 'p2': 127}),                # H samples planes 64 and 128
+    ('float32', 1, 'b', {}),
+])
+def test_3d_long_z_runs(precision, xc, direction, extra):
+    """A guide long enough in z (nz = 250) for four 64-cell z runs per row:
+    runs wholly outside the z-CPML slabs, runs that hold a source slab or a
+    port plane (E samples plane p, H samples p and p+1, here on both sides of
+    a run boundary), and the partial last run all give the oracle's fields and
+    port DFT in the two-kernel 3D form."""
+    p = configs.strip3d(250, nx=36, ny=34, nt=200, nf=6, precision=precision, width=12, height=8, port_len=10,
+                        buffer_z=6, source_type=1, gauss_pulse_deg=-1.0)
+    p.update(extra)
+    p.update(rough_std=15e-9, rough_acl=40e-9, seed=3)
+    s, up, lo, fg = device_inputs(p)
+    o, _ = oracle_run(p, up, lo, direction)
+    print('case', precision, 'xchunk', xc, direction, extra, flush=True)
+    tol = 1e-10 if precision == 'float64' else 1e-4
+    b = Batch('3d', np.dtype(precision).type, max_slots=1)
+    b.set_launch(xc, 0, 2)
+    slot = b.add(s, fg, direction)
+    b.advance(-(-s.nt // 32))
+    b.sync()
+    assert b.poll(slot) == s.nt
+    for name, ref in o.fields().items():
+        err = rel_l2(b.field(slot, name), ref)
+        assert err < tol, (name, err)
+    w = window('3d', o.c)
+    od = o.dft()
+    for key, z in b.dft(slot).items():
+        err = rel_l2(z[w], od[key][w])
+        assert err < tol, (key, err)
+    b.close()
+
+
+@pytest.mark.gpu
 def test_launch_form_validation():
     """Tile depths outside a form's set are rejected, also when kept from an
     earlier call."""
