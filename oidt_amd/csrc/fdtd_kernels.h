@@ -67,6 +67,12 @@ template <typename T>
 __device__ __forceinline__ void stg(T* base, unsigned boff, T v) {
   *(OIDT_GLOBAL T*)(reinterpret_cast<char*>(base) + boff) = v;
 }
+// Non-temporal store of a field value that no later access in the same
+// launch reads.
+template <typename T>
+__device__ __forceinline__ void stg_nt(T* base, unsigned boff, T v) {
+  __builtin_nontemporal_store(v, (OIDT_GLOBAL T*)(reinterpret_cast<char*>(base) + boff));
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that the blocks
 // of one XCD take a contiguous range of tiles (shared L2 for x-neighbours).
@@ -1020,7 +1026,7 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
         vx = vx + cb * (dHz * iky - dHy * ikz);
         if (sj >= 0) { const T p = bej * qxy[u] + cej * dHz; stg<T>(pExy, iy, p); vx += cb * p; }
         if (sk >= 0) { const T p = bek * qxz[u] + cek * dHy; stg<T>(pExz, iz, p); vx -= cb * p; }
-        stg<T>(Ex, bo, vx);
+        stg_nt<T>(Ex, bo, vx);
       }
       if (by) {
         const T cb = cbsel(s, m[u], 1);
@@ -1034,10 +1040,10 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
           vy -= cb * p;
         }
         if (sk >= 0) { const T p = bek * qyz[u] + cek * dHx; stg<T>(pEyz, iz, p); vy += cb * p; }
-        stg<T>(Ey, bo, vy);
+        stg_nt<T>(Ey, bo, vy);
       } else if (src) {
         vy -= jv;
-        stg<T>(Ey, bo, vy);
+        stg_nt<T>(Ey, bo, vy);
       }
       if (bz) {
         const T cb = cbsel(s, m[u], 2);
@@ -1050,7 +1056,7 @@ __global__ __launch_bounds__(kThreads) void k_3d_e2(const Slot<T>* __restrict__ 
           v += cb * p;
         }
         if (sj >= 0) { const T p = bej * qzy[u] + cej * dHx; stg<T>(pEzy, iy, p); v -= cb * p; }
-        stg<T>(Ez, bo, v);
+        stg_nt<T>(Ez, bo, v);
       }
       if (icap) {
         const int c = (i - s.wx0) * s.prow + (j - s.wy0);
@@ -1171,7 +1177,7 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
         vx = vx + db * (dEy * ikz - dEz * iky);
         if (sj >= 0) { const T p = bhj * qxy[u] + chj * dEz; stg<T>(pHxy, iy, p); vx -= db * p; }
         if (sk >= 0) { const T p = bhk * qxz[u] + chk * dEy; stg<T>(pHxz, iz, p); vx += db * p; }
-        stg<T>(Hx, bo, vx);
+        stg_nt<T>(Hx, bo, vx);
       }
       T vy = hy[u];
       if (src) vy -= jv;
@@ -1186,7 +1192,7 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
         }
         if (sk >= 0) { const T p = bhk * qyz[u] + chk * dEx; stg<T>(pHyz, iz, p); vy -= db * p; }
       }
-      if (by || src) stg<T>(Hy, bo, vy);
+      if (by || src) stg_nt<T>(Hy, bo, vy);
       if (bz) {
         const T dEx = ex[u] - exj[u];
         const T dEy = eyc - ey[u];
@@ -1197,7 +1203,7 @@ __global__ __launch_bounds__(kThreads) void k_3d_h2(const Slot<T>* __restrict__ 
           v -= db * p;
         }
         if (sj >= 0) { const T p = bhj * qzy[u] + chj * dEx; stg<T>(pHzy, iy, p); v += db * p; }
-        stg<T>(Hz, bo, v);
+        stg_nt<T>(Hz, bo, v);
       }
       if (icap) {
         const int c = (i - s.wx0) * s.prow + (j - s.wy0);
