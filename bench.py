@@ -136,19 +136,23 @@ def bench_study(args):
     c = variants.truncate(variants.config(scheme, study), warm + K + (1 if scheme == 'te' else 0))
     if args.realisations > 1:
         return bench_study_batch(args, scheme, study, warm, K)
-    # x-chunk of the two-kernel form: the given one, or the fastest step
-    # (E + H + monitor launches, HIP events, two chunks each)
+    # two-kernel form and x-chunk: the given ones, or the fastest step
+    # (E + H + monitor launches, HIP events, two chunks each); the 3D march
+    # form is a candidate too
+    forms = [(2, xc) for xc in (1, 2, 4, 8)] + ([(1, xc) for xc in (2, 4, 8)] if scheme == '3d' else [])
+    forms = [(v, xc) for v, xc in forms if (not args.variant or v == args.variant) and
+             (not args.xchunk or xc == args.xchunk)] or [(args.variant or 2, args.xchunk or 4)]
     tune = {}
-    for xc in ([args.xchunk] if args.xchunk else [1, 2, 4, 8]):
-        sol, mons, ids = variants.prepare(c, (2, xc, 0))
+    for v, xc in forms:
+        sol, mons, ids = variants.prepare(c, (v, xc, 0))
         out = (C.c_double * 6)()
         try:
             _lib.check(_lib.lib().oidt_solver_profile_all(sol.h, 2, out))
         finally:
             sol.close()
-        tune[xc] = out[0] + out[1] + out[4] + out[5]
-    xchunk = min(tune, key=tune.get)
-    sol, mons, ids = variants.prepare(c, (2, xchunk, 0))
+        tune[(v, xc)] = out[0] + out[1] + out[4] + out[5]
+    variant, xchunk = min(tune, key=tune.get)
+    sol, mons, ids = variants.prepare(c, (variant, xchunk, 0))
     try:
         sol.advance(warm).sync()
         t0 = time.perf_counter()
@@ -158,7 +162,7 @@ def bench_study(args):
         sol.close()
     cells = c.nx * c.ny * (c.nz if scheme == '3d' else 1)
     value = cells * K / wall / 1e6
-    sol, mons, ids = variants.prepare(c, (2, xchunk, 0))
+    sol, mons, ids = variants.prepare(c, (variant, xchunk, 0))
     out = (C.c_double * 6)()
     try:
         _lib.check(_lib.lib().oidt_solver_profile_all(sol.h, int(args.profile_chunks), out))
@@ -168,13 +172,14 @@ def bench_study(args):
     es = 4
     be, bh = algorithmic_bytes('tm' if scheme == 'tm' else scheme, es)
     stem = {'tm': 'k_tmzl', 'te': 'k_tez', '3d': 'k_3d'}[scheme]
-    cands = [(stem + ('_e' if scheme == 'tm' else '_e2'), ms_e, be * cells),
-             (stem + ('_h' if scheme == 'tm' else '_h2'), ms_h, bh * cells),
+    suffix = '' if scheme == 'tm' or variant == 1 else '2'
+    cands = [(stem + '_e' + suffix, ms_e, be * cells),
+             (stem + '_h' + suffix, ms_h, bh * cells),
              ('k_monitor (after E)', ms_me, monitor_bytes(mons, es, False)),
              ('k_monitor (after H)', ms_mh, monitor_bytes(mons, es, True))]
     kname, ms_k, bytes_k = max(cands, key=lambda t: t[1])
     achieved = bytes_k / (ms_k * 1e-3) / 1e9
-    traffic, traffic_src = committed_traffic(kname.split()[0], args.workload, 2, xchunk)
+    traffic, traffic_src = committed_traffic(kname.split()[0], args.workload, variant, xchunk)
     roof = {'bound': 'hbm', 'kernel': kname, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
             'traffic_source': traffic_src, 'bytes_per_launch': bytes_k, 'ms_per_launch': round(ms_k, 5),
@@ -188,8 +193,8 @@ def bench_study(args):
                                % (scheme, study, 'x'.join(str(n) for n in ((c.nx, c.ny, c.nz) if scheme == '3d'
                                                                            else (c.nx, c.ny))),
                                   warm + K, variants.config(scheme, study).nt),
-                   'cells_per_step': cells, 'monitors': sorted(mons), 'variant': 2, 'xchunk': xchunk,
-                   'xchunk_ms_per_step': {str(k): round(v, 5) for k, v in tune.items()},
+                   'cells_per_step': cells, 'monitors': sorted(mons), 'variant': variant, 'xchunk': xchunk,
+                   'form_ms_per_step': {'%d,%d' % k: round(v, 5) for k, v in tune.items()},
                    'parallelism': 'single GPU'},
         'roofline': roof,
     }
@@ -206,8 +211,8 @@ def bench_study_batch(args, scheme, study, warm, K):
     steps = warm + K + (1 if scheme == 'te' else 0)
     cs = [variants.truncate(variants.config(scheme, study, profile_number=p, rough_toggle=True), steps)
           for p in range(R)]
-    xc = args.xchunk or (1 if scheme == '3d' else 4)
-    b, runs = variants.prepare_batch(cs, (2, xc, 0))
+    b, runs = variants.prepare_batch(cs, (args.variant, args.xchunk, 0))
+    variant, xc, _ = b.launch()
     try:
         b.advance(warm // 32)
         b.sync()
@@ -222,7 +227,7 @@ def bench_study_batch(args, scheme, study, warm, K):
     # plain: the same realisations one at a time, each its own run (create, upload and read back included)
     t0 = time.perf_counter()
     for c in cs:
-        sol, mons, ids = variants.prepare(c, (2, xc, 0))
+        sol, mons, ids = variants.prepare(c, (variant, xc, 0))
         try:
             sol.advance(warm + K).sync()
             for k in mons:
@@ -237,7 +242,7 @@ def bench_study_batch(args, scheme, study, warm, K):
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
         'config': {'workload': '%d roughness realisations of the %s %s study, one batch, reference default '
                                'configuration, first %d steps' % (R, scheme, study, warm + K),
-                   'cells_per_step': cells, 'variant': 2, 'xchunk': xc, 'parallelism': 'single GPU'},
+                   'cells_per_step': cells, 'variant': variant, 'xchunk': xc, 'parallelism': 'single GPU'},
         'plain_one_at_a_time': round(plain, 1), 'batched_over_plain': round(value / plain, 3),
     }
     print(json.dumps(line), flush=True)

@@ -254,11 +254,12 @@ int oidt_batch_read_monitor(oidt_batch* b, int32_t slot, int32_t monitor, void* 
 int oidt_batch_remove(oidt_batch* b, int32_t slot);
 
 /* Launch knobs for the autotuner: x-chunk depth of the stencil tiles (cells per
- * thread along x: 1, 2, 4, 8, 16 or 32; 0 keeps the current value; default 1
- * for 3D and 4 for 2D), the persistent grid size (0 = automatic) and the
- * stencil kernel form (1 = per-plane march, 2 = batched loads per x-chunk, the
- * default; 3 = fused Yee step, one launch per step, all schemes, xchunk also
- * 12, 20 or 24, at most 16 in 3D).  The kernel form of a run cannot change between the fused
+ * thread along x: 1, 2, 4, 8, 16 or 32; 0 keeps the current value; default
+ * 4), the persistent grid size (0 = automatic) and the stencil kernel form
+ * (0 keeps the current one; 1 = per-plane march, the 3D default, which in 3D
+ * issues the next plane's loads before updating the current one; 2 = batched
+ * loads per x-chunk, the 2D default; 3 = fused Yee step, one launch per step,
+ * all schemes, xchunk also 12, 20 or 24, at most 16 in 3D).  The kernel form of a run cannot change between the fused
  * and the two-kernel forms once the run has started (OIDT_ESTATE).
  * No counterpart in the reference (fixed launch shapes, fdtd_main.py:263-273). */
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant);

@@ -154,6 +154,16 @@ int validate(const oidt_run_desc* d) {
   const double slow = is3d ? (double)d->nx * d->ny : (double)d->nx;
   if (slow * fast * es > 4294967295.0)
     return fail(OIDT_EINVAL, "a field array would exceed 4 GiB (after padding the unit-stride axis to 128 bytes)");
+  // The 3D half-step kernels address a high-side psi slab from the low-side
+  // array of the same component (both sides and the fused step's second set
+  // lie in between): three slabs must stay below 4 GiB.
+  for (int a = 0; is3d && a < 3; ++a) {
+    double cells = N;
+    for (int b = 0; b < 3; ++b)
+      if (b != a) cells *= b == 2 ? fast : (double)dims[b];
+    if (3.0 * (cells * es + 256.0) > 4294967295.0)
+      return fail(OIDT_EINVAL, "a CPML psi slab pair would exceed the 32-bit offset range");
+  }
   if (d->nt < 0 || d->nf < 0) return fail(OIDT_EINVAL, "negative nt or nf");
   const int plim = is3d ? d->nz : d->nx;
   for (int p = 0; p < 2; ++p)
@@ -448,6 +458,11 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   for (int i = 0; i < 36; ++i) s.psi[i] = L.off_psi[i] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_psi[i]);
   for (int f = 0; f < 6; ++f) s.f2[f] = L.off_f2[f] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_f2[f]);
   for (int i = 0; i < 36; ++i) s.psi2[i] = L.off_psi2[i] == (size_t)-1 ? nullptr : (OIDT_GLOBAL T*)(base + L.off_psi2[i]);
+  for (int a = 0; a < 3; ++a) s.psi_hi[a] = 0;
+  for (auto pr : psi_pairs(d.scheme)) {
+    const size_t lo = L.off_psi[psi_id(pr.first, pr.second, 0)], hi = L.off_psi[psi_id(pr.first, pr.second, 1)];
+    s.psi_hi[pr.second] = (unsigned)(hi - lo);
+  }
   s.mat = (OIDT_GLOBAL const uint8_t*)(base + L.off_mat);
   for (int c = 0; c < 3; ++c)
     for (int q = 0; q < kMatMax; ++q) s.cbv[c][q] = cbv[c][q];
@@ -846,9 +861,10 @@ int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt
   b->scheme = scheme;
   b->precision = precision;
   b->max_slots = max_slots;
-  // measured defaults on MI355X: 3D is best with one cell per thread along x
-  // (44 VGPRs, full occupancy), 2D with four
-  b->xchunk = scheme == SCHEME_3D ? 1 : 4;
+  // measured defaults on MI355X: the prefetching march over 4-plane tiles in
+  // 3D, the batched-load form with four cells per thread in 2D
+  b->variant = scheme == SCHEME_3D ? 1 : 2;
+  b->xchunk = 4;
   b->slots.resize(max_slots);
   auto bad = [&](hipError_t e, const char* what) {
     g_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -896,7 +912,8 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
-  if (variant < 1 || variant > 3) return fail(OIDT_EINVAL, "variant must be 1, 2 or 3");
+  if (variant < 0 || variant > 3) return fail(OIDT_EINVAL, "variant must be 0 (keep), 1, 2 or 3");
+  if (variant == 0) variant = b->variant;
   for (const auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.fused != (variant == 3))
       return fail(OIDT_ESTATE, "cannot switch between fused and two-kernel forms while a run is in progress");

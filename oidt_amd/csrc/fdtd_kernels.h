@@ -367,23 +367,54 @@ __global__ __launch_bounds__(kThreads) void k_tez_h(const Slot<T>* __restrict__ 
 }
 
 // -------------------------------------------------------------------- 3D ---
-// Block: threadIdx.x -> k (64 lanes, unit stride), threadIdx.y -> j (4 rows);
-// each thread marches XC cells along x.
+// 3D half-steps as a prefetching march.  A wave owns one (j row, 64 z-cells)
+// strip of XC planes and walks it along x; the loads of plane i+1 are issued
+// before plane i is updated, so every wave keeps a plane of loads in flight
+// while it computes and stores.  Hy and Hz of plane i+1 (E) or Ey and Ez of
+// plane i (H) come from the neighbouring plane's loads.  The row (y) values
+// are wave-uniform; the plane's x-only coefficients are loaded once per tile,
+// one plane per lane, and broadcast with readlane; the run values used in the
+// loop are copied to registers first (stores through the field pointers could
+// otherwise alias the descriptor and force reloads).
+
+// Bits u of the tile's planes i0 + u (u < 64) that lie in [lo, hi).
+__device__ __forceinline__ unsigned long long plane_bits(int i0, int lo, int hi) {
+  const int a = min(max(lo - i0, 0), 64), b = min(max(hi - i0, 0), 64);
+  if (b <= a) return 0ull;
+  const unsigned long long upto = b >= 64 ? ~0ull : (1ull << b) - 1ull;
+  return upto & ~((1ull << a) - 1ull);
+}
+
+template <typename T>
+__device__ __forceinline__ T cb_of(T v0, T v1, T v2, T v3, unsigned m, int c) {
+  const int q = (m >> (2 * c)) & 3;
+  return q == 0 ? v0 : (q == 1 ? v1 : (q == 2 ? v2 : v3));
+}
+
+// What the E march loads for one cell of plane i.
+template <typename T>
+struct E3Plane {
+  T ex, ey, ez, hx, hxk, hxj, hyk, hzj, hy1, hz1, qxy, qzy, qxz, qyz, qyx, qzx;
+  unsigned m;   // material byte
+};
 
 template <typename T, int XC>
 __global__ __launch_bounds__(kThreads) void k_3d_e(const Slot<T>* __restrict__ slots,
                                                    const int4* __restrict__ tiles,
                                                    const int* __restrict__ counts, int tstep) {
+  static_assert(XC <= kBX, "one plane's x-only coefficients per lane");
+  constexpr unsigned es = sizeof(T);
   const int ntiles = counts[0];
+  const int lane = threadIdx.x;
   for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
     const int4 tl = tiles[tt];
     const Slot<T>& s = slots[tl.x];
     const int n = s.base + tstep;
     if (!s.active || n >= s.nt) continue;
-    const int ny = s.ny, nz = s.nz, N = s.N;
-    const int k = tl.w + threadIdx.x, j = tl.z + threadIdx.y;
-    if (j >= ny || k >= nz) continue;
-    const int i0 = tl.y, i1 = min(tl.y + XC, s.nx);
+    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+    const int j = tl.z + wave_id();
+    if (j >= ny) continue;
+    const int k = tl.w + lane, kc = min(k, nz - 1);   // lanes past nz load clamped and store nothing
     const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
     T* __restrict__ Ex = s.f[F_EX];
     T* __restrict__ Ey = s.f[F_EY];
@@ -391,104 +422,194 @@ __global__ __launch_bounds__(kThreads) void k_3d_e(const Slot<T>* __restrict__ s
     const T* __restrict__ Hx = s.f[F_HX];
     const T* __restrict__ Hy = s.f[F_HY];
     const T* __restrict__ Hz = s.f[F_HZ];
-    const size_t pl = (size_t)ny * nz;
+    const uint8_t* __restrict__ mat = s.mat;
+    const unsigned pl = (unsigned)ny * nz;
+    const unsigned jk = (unsigned)j * nz + kc;
+    const unsigned offk = (kc + 1 < nz) ? es : 0u;
+    const unsigned offj = (j + 1 < ny) ? (unsigned)nz * es : 0u;
     const bool jin = j >= jb && j < je, jin1 = j > jb && j < je;
     const bool kin = k >= kb && k < ke, kin1 = k > kb && k < ke;
-    int dj, cj, dk, ck;
+    int dj = 0, cj = 0, dk = 0, ck = 0;
     const int sj = jin ? slab_e(j, jb, je, N, dj, cj) : -1;
     const int sk = kin ? slab_e(k, kb, ke, N, dk, ck) : -1;
-    const T iky = jin ? s.ike[1][j] : T(0);
-    const T ikz = kin ? s.ike[2][k] : T(0);
+    const T iky = jin ? ldg<T>(s.ike[1], j * es) : T(0);
+    const T ikz = kin ? ldg<T>(s.ike[2], k * es) : T(0);
+    const T bej = sj >= 0 ? ldg<T>(s.be, cj * es) : T(0), cej = sj >= 0 ? ldg<T>(s.ceo, cj * es) : T(0);
+    const T bek = sk >= 0 ? ldg<T>(s.be, ck * es) : T(0), cek = sk >= 0 ? ldg<T>(s.ceo, ck * es) : T(0);
+    T* __restrict__ pExy = sj >= 0 ? s.psi[psi_id(F_EX, 1, sj)] : nullptr;
+    T* __restrict__ pEzy = sj >= 0 ? s.psi[psi_id(F_EZ, 1, sj)] : nullptr;
+    // z- and x-slab psi: the low-side array, plus the side's byte distance
+    T* __restrict__ pExz = s.psi[psi_id(F_EX, 2, 0)];
+    T* __restrict__ pEyz = s.psi[psi_id(F_EY, 2, 0)];
+    T* __restrict__ pEyx = s.psi[psi_id(F_EY, 0, 0)];
+    T* __restrict__ pEzx = s.psi[psi_id(F_EZ, 0, 0)];
+    const unsigned zside = sk == 1 ? s.psi_hi[2] : 0u, xhi = s.psi_hi[0];
     const bool jksrc = s.src_field == F_EY && j >= s.sy0 && j < s.sy1 && k >= s.sz0 && k < s.sz1;
-    const T jv = jksrc ? s.jsrc[n] : T(0);
+    const T jv = jksrc ? ldg<T>(s.jsrc, n * es) : T(0);
     const bool jcap = j >= s.wy0 && j <= s.wy1;
     const int cap = !jcap ? -1 : (k == s.port[0] ? 0 : (k == s.port[1] ? 1 : -1));
     const int cap2 = (jcap && s.port[0] == s.port[1] && k == s.port[0]) ? 1 : -1;
-    const size_t jk = (size_t)j * nz + k;
-    T hz_next = (i0 < i1) ? Hz[(size_t)i0 * pl + jk] : T(0);
-    T hy_next = (i0 < i1) ? Hy[(size_t)i0 * pl + jk] : T(0);
-    for (int i = i0; i < i1; ++i) {
-      const size_t idx = (size_t)i * pl + jk;
-      const T hz0 = hz_next, hy0 = hy_next;
-      const bool more = i + 1 < s.nx;
-      hz_next = more ? Hz[idx + pl] : T(0);
-      hy_next = more ? Hy[idx + pl] : T(0);
+    const T cx0 = s.cbv[0][0], cx1 = s.cbv[0][1], cx2 = s.cbv[0][2], cx3 = s.cbv[0][3];
+    const T cy0 = s.cbv[1][0], cy1 = s.cbv[1][1], cy2 = s.cbv[1][2], cy3 = s.cbv[1][3];
+    const T cz0 = s.cbv[2][0], cz1 = s.cbv[2][1], cz2 = s.cbv[2][2], cz3 = s.cbv[2][3];
+    const int i0 = tl.y, iend = min(i0 + XC, nx);
+    // planes of the tile inside the source box and the capture window
+    const unsigned long long srcm = plane_bits(i0, s.sx0, s.sx1), capm = plane_bits(i0, s.wx0, s.wx1 + 1);
+    // x-only coefficients: lane u holds plane i0 + u
+    T lk_ikx = T(0), lk_bex = T(0), lk_cex = T(0);
+    if (lane < XC && i0 + lane < nx) {
+      const int i = i0 + lane;
+      lk_ikx = ldg<T>(s.ike[0], i * es);
+      int di = 0, ci = 0;
+      if (slab_e(i, ib, ie, N, di, ci) >= 0) {
+        lk_bex = ldg<T>(s.be, ci * es);
+        lk_cex = ldg<T>(s.ceo, ci * es);
+      }
+    }
+    auto load = [&](int i) {
+      E3Plane<T> p;
+      const unsigned idx = (unsigned)i * pl + jk;
+      const unsigned bo = idx * es;
+      const unsigned bo1 = (i + 1 < nx ? idx + pl : idx) * es;
+      p.ex = ldg<T>(Ex, bo);
+      p.ey = ldg<T>(Ey, bo);
+      p.ez = ldg<T>(Ez, bo);
+      p.hx = ldg<T>(Hx, bo);
+      p.hxk = ldg<T>(Hx, bo + offk);
+      p.hxj = ldg<T>(Hx, bo + offj);
+      p.hyk = ldg<T>(Hy, bo + offk);
+      p.hzj = ldg<T>(Hz, bo + offj);
+      p.hy1 = ldg<T>(Hy, bo1);
+      p.hz1 = ldg<T>(Hz, bo1);
+      p.m = ldg<uint8_t>(mat, idx);
+      const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+      p.qxy = pExy ? ldg<T>(pExy, iy) : T(0);
+      p.qzy = pEzy ? ldg<T>(pEzy, iy) : T(0);
+      p.qxz = sk >= 0 ? ldg<T>(pExz, iz + zside) : T(0);
+      p.qyz = sk >= 0 ? ldg<T>(pEyz, iz + zside) : T(0);
+      int di = 0, ci = 0;
+      const int si = slab_e(i, ib, ie, N, di, ci);
+      const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
+      p.qyx = si >= 0 ? ldg<T>(pEyx, ix) : T(0);
+      p.qzx = si >= 0 ? ldg<T>(pEzx, ix) : T(0);
+      return p;
+    };
+    const unsigned b0 = ((unsigned)i0 * pl + jk) * es;
+    T hy0 = ldg<T>(Hy, b0), hz0 = ldg<T>(Hz, b0);   // Hy, Hz of the plane being updated
+    auto update = [&](const int i, const E3Plane<T>& p0) {
       const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
       const bool bx = iin1 && jin && kin;
       const bool by = iin && jin1 && kin;
       const bool bz = iin && jin && kin1;
-      const bool src = jksrc && i >= s.sx0 && i < s.sx1;
-      const bool icap = cap >= 0 && i >= s.wx0 && i <= s.wx1;
-      if (!(bx || by || bz || src || icap)) continue;
-      const uint8_t m = (bx || by || bz) ? s.mat[idx] : 0;
-      int di, ci;
-      const int si = slab_e(i, ib, ie, N, di, ci);
-      T vx = T(0), vy = T(0);
-      const bool needx = bx || icap;
-      // update_ex: i (ib,ie), j [jb,je), k [kb,ke); psi ex_y (+), ex_z (-)
-      if (needx) vx = Ex[idx];
-      if (bx) {
-        const T cb = cbsel(s, m, 0);
-        const T dHz = Hz[idx + nz] - hz0;
-        const T dHy = Hy[idx + 1] - hy0;
-        vx = vx + cb * (dHz * iky - dHy * ikz);
-        if (sj >= 0) vx += cb * psi_upd(s.psi[psi_id(F_EX, 1, sj)] + ((size_t)i * N + dj) * nz + k, s.be[cj], s.ceo[cj], dHz);
-        if (sk >= 0) vx -= cb * psi_upd(s.psi[psi_id(F_EX, 2, sk)] + ((size_t)i * ny + j) * N + dk, s.be[ck], s.ceo[ck], dHy);
-        Ex[idx] = vx;
-      }
-      // update_ey: i [ib,ie), j (jb,je), k [kb,ke); source (TE); psi ey_x (-), ey_z (+)
-      const bool needy = by || src || icap;
-      if (needy) vy = Ey[idx];
-      if (by) {
-        const T cb = cbsel(s, m, 1);
-        const T dHx = Hx[idx + 1] - Hx[idx];
-        const T dHz = hz_next - hz0;
-        vy = vy + cb * (dHx * ikz - dHz * s.ike[0][i]);
-        if (src) vy -= jv;
-        if (si >= 0) vy -= cb * psi_upd(s.psi[psi_id(F_EY, 0, si)] + ((size_t)di * ny + j) * nz + k, s.be[ci], s.ceo[ci], dHz);
-        if (sk >= 0) vy += cb * psi_upd(s.psi[psi_id(F_EY, 2, sk)] + ((size_t)i * ny + j) * N + dk, s.be[ck], s.ceo[ck], dHx);
-        Ey[idx] = vy;
-      } else if (src) {
-        vy -= jv;
-        Ey[idx] = vy;
-      }
-      // update_ez: i [ib,ie), j [jb,je), k (kb,ke); psi ez_x (+), ez_y (-)
-      if (bz) {
-        const T cb = cbsel(s, m, 2);
-        const T dHy = hy_next - hy0;
-        const T dHx = Hx[idx + nz] - Hx[idx];
-        T v = Ez[idx] + cb * (dHy * s.ike[0][i] - dHx * iky);
-        if (si >= 0) v += cb * psi_upd(s.psi[psi_id(F_EZ, 0, si)] + ((size_t)di * ny + j) * nz + k, s.be[ci], s.ceo[ci], dHy);
-        if (sj >= 0) v -= cb * psi_upd(s.psi[psi_id(F_EZ, 1, sj)] + ((size_t)i * N + dj) * nz + k, s.be[cj], s.ceo[cj], dHx);
-        Ez[idx] = v;
-      }
-      if (icap) {
-        const int c = (i - s.wx0) * s.prow + (j - s.wy0);
-        capture(s, tstep, cap * 6 + 0, c, vx);
-        capture(s, tstep, cap * 6 + 1, c, vy);
-        if (cap2 >= 0) {
-          capture(s, tstep, 6, c, vx);
-          capture(s, tstep, 7, c, vy);
+      const bool src = jksrc && ((srcm >> (i - i0)) & 1ull);
+      const bool icap = cap >= 0 && ((capm >> (i - i0)) & 1ull);
+      if (bx || by || bz || src || icap) {
+        const int u = i - i0;
+        int di = 0, ci = 0;
+        const int si = slab_e(i, ib, ie, N, di, ci);
+        const unsigned idx = (unsigned)i * pl + jk;
+        const unsigned bo = idx * es;
+        const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+        const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
+        const T ikx = lane_get(lk_ikx, u);
+        T vx = p0.ex, vy = p0.ey;
+        // update_ex: i (ib,ie), j [jb,je), k [kb,ke); psi ex_y (+), ex_z (-)
+        if (bx) {
+          const T cb = cb_of(cx0, cx1, cx2, cx3, p0.m, 0);
+          const T dHz = p0.hzj - hz0;
+          const T dHy = p0.hyk - hy0;
+          vx = vx + cb * (dHz * iky - dHy * ikz);
+          if (sj >= 0) { const T q = bej * p0.qxy + cej * dHz; stg<T>(pExy, iy, q); vx += cb * q; }
+          if (sk >= 0) { const T q = bek * p0.qxz + cek * dHy; stg<T>(pExz, iz + zside, q); vx -= cb * q; }
+          stg_nt<T>(Ex, bo, vx);
+        }
+        // update_ey: i [ib,ie), j (jb,je), k [kb,ke); source (TE); psi ey_x (-), ey_z (+)
+        if (by) {
+          const T cb = cb_of(cy0, cy1, cy2, cy3, p0.m, 1);
+          const T dHx = p0.hxk - p0.hx;
+          const T dHz = p0.hz1 - hz0;
+          vy = vy + cb * (dHx * ikz - dHz * ikx);
+          if (src) vy -= jv;
+          if (si >= 0) {
+            const T q = lane_get(lk_bex, u) * p0.qyx + lane_get(lk_cex, u) * dHz;
+            stg<T>(pEyx, ix, q);
+            vy -= cb * q;
+          }
+          if (sk >= 0) { const T q = bek * p0.qyz + cek * dHx; stg<T>(pEyz, iz + zside, q); vy += cb * q; }
+          stg_nt<T>(Ey, bo, vy);
+        } else if (src) {
+          vy -= jv;
+          stg_nt<T>(Ey, bo, vy);
+        }
+        // update_ez: i [ib,ie), j [jb,je), k (kb,ke); psi ez_x (+), ez_y (-)
+        if (bz) {
+          const T cb = cb_of(cz0, cz1, cz2, cz3, p0.m, 2);
+          const T dHy = p0.hy1 - hy0;
+          const T dHx = p0.hxj - p0.hx;
+          T v = p0.ez + cb * (dHy * ikx - dHx * iky);
+          if (si >= 0) {
+            const T q = lane_get(lk_bex, u) * p0.qzx + lane_get(lk_cex, u) * dHy;
+            stg<T>(pEzx, ix, q);
+            v += cb * q;
+          }
+          if (sj >= 0) { const T q = bej * p0.qzy + cej * dHx; stg<T>(pEzy, iy, q); v -= cb * q; }
+          stg_nt<T>(Ez, bo, v);
+        }
+        if (icap) {
+          const size_t c = (size_t)((i - s.wx0) * s.prow + (j - s.wy0));
+          const size_t base = (size_t)tstep * s.nplanes;
+          const size_t pcells = s.pcells;
+          T* __restrict__ ring = s.ring;
+          ring[(base + cap * 6 + 0) * pcells + c] = vx;
+          ring[(base + cap * 6 + 1) * pcells + c] = vy;
+          if (cap2 >= 0) {
+            ring[(base + 6) * pcells + c] = vx;
+            ring[(base + 7) * pcells + c] = vy;
+          }
         }
       }
+      hy0 = p0.hy1;
+      hz0 = p0.hz1;
+    };
+    // Fully unrolled over the tile's planes, so the next plane's registers
+    // are new ones (no copy waits for its loads).
+    E3Plane<T> pc = load(i0);
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      const int i = i0 + u;
+      if (i >= iend) break;
+      E3Plane<T> pn;
+      if (u + 1 < XC) pn = load(min(i + 1, iend - 1));
+      update(i, pc);
+      if (u + 1 < XC) pc = pn;
     }
   }
 }
+
+// What the H march loads for one cell of plane i (its Ey and Ez are the
+// i-1 neighbours of plane i+1).
+template <typename T>
+struct H3Plane {
+  T hx, hy, hz, ex, exk, exj, eyk, ezj, ey, ez, qxy, qzy, qxz, qyz, qyx, qzx;
+};
 
 template <typename T, int XC>
 __global__ __launch_bounds__(kThreads) void k_3d_h(const Slot<T>* __restrict__ slots,
                                                    const int4* __restrict__ tiles,
                                                    const int* __restrict__ counts, int tstep) {
+  static_assert(XC <= kBX, "one plane's x-only coefficients per lane");
+  constexpr unsigned es = sizeof(T);
   const int ntiles = counts[0];
+  const int lane = threadIdx.x;
   for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
     const int4 tl = tiles[tt];
     const Slot<T>& s = slots[tl.x];
     const int n = s.base + tstep;
     if (!s.active || n >= s.nt) continue;
-    const int ny = s.ny, nz = s.nz, N = s.N;
-    const int k = tl.w + threadIdx.x, j = tl.z + threadIdx.y;
-    if (j >= ny || k >= nz) continue;
-    const int i0 = tl.y, i1 = min(tl.y + XC, s.nx);
+    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+    const int j = tl.z + wave_id();
+    if (j >= ny) continue;
+    const int k = tl.w + lane, kc = min(k, nz - 1);   // lanes past nz load clamped and store nothing
     const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
     const T* __restrict__ Ex = s.f[F_EX];
     const T* __restrict__ Ey = s.f[F_EY];
@@ -497,17 +618,30 @@ __global__ __launch_bounds__(kThreads) void k_3d_h(const Slot<T>* __restrict__ s
     T* __restrict__ Hy = s.f[F_HY];
     T* __restrict__ Hz = s.f[F_HZ];
     const T db = s.db;
-    const size_t pl = (size_t)ny * nz;
+    const unsigned pl = (unsigned)ny * nz;
+    const unsigned jk = (unsigned)j * nz + kc;
+    const unsigned offk = (kc > 0) ? es : 0u;
+    const unsigned offj = (j > 0) ? (unsigned)nz * es : 0u;
     const bool jin = j >= jb && j < je, jin1 = j > jb && j < je;
     const bool kin = k >= kb && k < ke, kin1 = k > kb && k < ke;
-    int dj, cj, dk, ck;
+    int dj = 0, cj = 0, dk = 0, ck = 0;
     const int sj = jin1 ? slab_h(j, jb, je, N, dj, cj) : -1;
     const int sk = kin1 ? slab_h(k, kb, ke, N, dk, ck) : -1;
-    const T iky = jin1 ? s.ikh[1][j] : T(0);
-    const T ikz = kin1 ? s.ikh[2][k] : T(0);
+    const T iky = jin1 ? ldg<T>(s.ikh[1], j * es) : T(0);
+    const T ikz = kin1 ? ldg<T>(s.ikh[2], k * es) : T(0);
+    const T bhj = sj >= 0 ? ldg<T>(s.bh, cj * es) : T(0), chj = sj >= 0 ? ldg<T>(s.cho, cj * es) : T(0);
+    const T bhk = sk >= 0 ? ldg<T>(s.bh, ck * es) : T(0), chk = sk >= 0 ? ldg<T>(s.cho, ck * es) : T(0);
+    T* __restrict__ pHxy = sj >= 0 ? s.psi[psi_id(F_HX, 1, sj)] : nullptr;
+    T* __restrict__ pHzy = sj >= 0 ? s.psi[psi_id(F_HZ, 1, sj)] : nullptr;
+    // z- and x-slab psi: the low-side array, plus the side's byte distance
+    T* __restrict__ pHxz = s.psi[psi_id(F_HX, 2, 0)];
+    T* __restrict__ pHyz = s.psi[psi_id(F_HY, 2, 0)];
+    T* __restrict__ pHyx = s.psi[psi_id(F_HY, 0, 0)];
+    T* __restrict__ pHzx = s.psi[psi_id(F_HZ, 0, 0)];
+    const unsigned zside = sk == 1 ? s.psi_hi[2] : 0u, xhi = s.psi_hi[0];
     // 3D TM source on Hy (fdtd_main.py:369-370), applied before the H curl
     const bool jksrc = s.src_field == F_HY && j >= s.sy0 && j < s.sy1 && k >= s.sz0 && k < s.sz1;
-    const T jv = jksrc ? s.jsrc[n] : T(0);
+    const T jv = jksrc ? ldg<T>(s.jsrc, n * es) : T(0);
     const bool jcap = j >= s.wy0 && j <= s.wy1;
     // capture planes: Hx/Hy at kc (planes 2,4) and kc+1 (planes 3,5) of each port
     int capA = -1, capB = -1, capC = -1, capD = -1;
@@ -518,61 +652,126 @@ __global__ __launch_bounds__(kThreads) void k_3d_h(const Slot<T>* __restrict__ s
       if (k == s.port[1] + 1) capD = 1;
     }
     const bool anycap = capA >= 0 || capB >= 0 || capC >= 0 || capD >= 0;
-    const size_t jk = (size_t)j * nz + k;
-    T ez_prev = (i0 > 0 && i0 < i1) ? Ez[(size_t)(i0 - 1) * pl + jk] : T(0);
-    T ey_prev = (i0 > 0 && i0 < i1) ? Ey[(size_t)(i0 - 1) * pl + jk] : T(0);
-    for (int i = i0; i < i1; ++i) {
-      const size_t idx = (size_t)i * pl + jk;
-      const T ez = Ez[idx], ey = Ey[idx];
-      const T ezm = ez_prev, eym = ey_prev;
-      ez_prev = ez;
-      ey_prev = ey;
+    const int i0 = tl.y, iend = min(i0 + XC, nx);
+    // planes of the tile inside the source box and the capture window
+    const unsigned long long srcm = plane_bits(i0, s.sx0, s.sx1), capm = plane_bits(i0, s.wx0, s.wx1 + 1);
+    T lk_ikx = T(0), lk_bhx = T(0), lk_chx = T(0);
+    if (lane < XC && i0 + lane < nx) {
+      const int i = i0 + lane;
+      lk_ikx = ldg<T>(s.ikh[0], i * es);
+      int di = 0, ci = 0;
+      if (slab_h(i, ib, ie, N, di, ci) >= 0) {
+        lk_bhx = ldg<T>(s.bh, ci * es);
+        lk_chx = ldg<T>(s.cho, ci * es);
+      }
+    }
+    auto load = [&](int i) {
+      H3Plane<T> p;
+      const unsigned idx = (unsigned)i * pl + jk;
+      const unsigned bo = idx * es;
+      p.hx = ldg<T>(Hx, bo);
+      p.hy = ldg<T>(Hy, bo);
+      p.hz = ldg<T>(Hz, bo);
+      p.ex = ldg<T>(Ex, bo);
+      p.exk = ldg<T>(Ex, bo - offk);
+      p.exj = ldg<T>(Ex, bo - offj);
+      p.eyk = ldg<T>(Ey, bo - offk);
+      p.ezj = ldg<T>(Ez, bo - offj);
+      p.ey = ldg<T>(Ey, bo);
+      p.ez = ldg<T>(Ez, bo);
+      const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+      p.qxy = pHxy ? ldg<T>(pHxy, iy) : T(0);
+      p.qzy = pHzy ? ldg<T>(pHzy, iy) : T(0);
+      p.qxz = sk >= 0 ? ldg<T>(pHxz, iz + zside) : T(0);
+      p.qyz = sk >= 0 ? ldg<T>(pHyz, iz + zside) : T(0);
+      int di = 0, ci = 0;
+      const int si = slab_h(i, ib, ie, N, di, ci);
+      const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
+      p.qyx = si >= 0 ? ldg<T>(pHyx, ix) : T(0);
+      p.qzx = si >= 0 ? ldg<T>(pHzx, ix) : T(0);
+      return p;
+    };
+    const unsigned bm = ((unsigned)max(i0 - 1, 0) * pl + jk) * es;
+    T eym = ldg<T>(Ey, bm), ezm = ldg<T>(Ez, bm);   // Ey, Ez of plane i-1
+    auto update = [&](const int i, const H3Plane<T>& p0) {
       const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
       const bool bx = iin && jin1 && kin1;
       const bool by = iin1 && jin && kin1;
       const bool bz = iin1 && jin1 && kin;
-      const bool src = jksrc && i >= s.sx0 && i < s.sx1;
-      const bool icap = anycap && i >= s.wx0 && i <= s.wx1;
-      if (!(bx || by || bz || src || icap)) continue;
-      int di, ci;
-      const int si = slab_h(i, ib, ie, N, di, ci);
-      // update_hx: i [ib,ie), j (jb,je), k (kb,ke); psi hx_y (-), hx_z (+)
-      T vx = (bx || icap) ? Hx[idx] : T(0);
-      if (bx) {
-        const T dEy = ey - Ey[idx - 1];
-        const T dEz = ez - Ez[idx - nz];
-        vx = vx + db * (dEy * ikz - dEz * iky);
-        if (sj >= 0) vx -= db * psi_upd(s.psi[psi_id(F_HX, 1, sj)] + ((size_t)i * N + dj) * nz + k, s.bh[cj], s.cho[cj], dEz);
-        if (sk >= 0) vx += db * psi_upd(s.psi[psi_id(F_HX, 2, sk)] + ((size_t)i * ny + j) * N + dk, s.bh[ck], s.cho[ck], dEy);
-        Hx[idx] = vx;
+      const bool src = jksrc && ((srcm >> (i - i0)) & 1ull);
+      const bool icap = anycap && ((capm >> (i - i0)) & 1ull);
+      if (bx || by || bz || src || icap) {
+        const int u = i - i0;
+        int di = 0, ci = 0;
+        const int si = slab_h(i, ib, ie, N, di, ci);
+        const unsigned idx = (unsigned)i * pl + jk;
+        const unsigned bo = idx * es;
+        const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
+        const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
+        const T ikx = lane_get(lk_ikx, u);
+        // update_hx: i [ib,ie), j (jb,je), k (kb,ke); psi hx_y (-), hx_z (+)
+        T vx = p0.hx;
+        if (bx) {
+          const T dEy = p0.ey - p0.eyk;
+          const T dEz = p0.ez - p0.ezj;
+          vx = vx + db * (dEy * ikz - dEz * iky);
+          if (sj >= 0) { const T q = bhj * p0.qxy + chj * dEz; stg<T>(pHxy, iy, q); vx -= db * q; }
+          if (sk >= 0) { const T q = bhk * p0.qxz + chk * dEy; stg<T>(pHxz, iz + zside, q); vx += db * q; }
+          stg_nt<T>(Hx, bo, vx);
+        }
+        // update_hy: i (ib,ie), j [jb,je), k (kb,ke); psi hy_x (+), hy_z (-)
+        T vy = p0.hy;
+        if (src) vy -= jv;
+        if (by) {
+          const T dEz = p0.ez - ezm;
+          const T dEx = p0.ex - p0.exk;
+          vy = vy + db * (dEz * ikx - dEx * ikz);
+          if (si >= 0) {
+            const T q = lane_get(lk_bhx, u) * p0.qyx + lane_get(lk_chx, u) * dEz;
+            stg<T>(pHyx, ix, q);
+            vy += db * q;
+          }
+          if (sk >= 0) { const T q = bhk * p0.qyz + chk * dEx; stg<T>(pHyz, iz + zside, q); vy -= db * q; }
+        }
+        if (by || src) stg_nt<T>(Hy, bo, vy);
+        // update_hz: i (ib,ie), j (jb,je), k [kb,ke); psi hz_x (-), hz_y (+)
+        if (bz) {
+          const T dEx = p0.ex - p0.exj;
+          const T dEy = p0.ey - eym;
+          T v = p0.hz + db * (dEx * iky - dEy * ikx);
+          if (si >= 0) {
+            const T q = lane_get(lk_bhx, u) * p0.qzx + lane_get(lk_chx, u) * dEy;
+            stg<T>(pHzx, ix, q);
+            v -= db * q;
+          }
+          if (sj >= 0) { const T q = bhj * p0.qzy + chj * dEx; stg<T>(pHzy, iy, q); v += db * q; }
+          stg_nt<T>(Hz, bo, v);
+        }
+        if (icap) {
+          const size_t c = (size_t)((i - s.wx0) * s.prow + (j - s.wy0));
+          const size_t base = (size_t)tstep * s.nplanes;
+          const size_t pcells = s.pcells;
+          T* __restrict__ ring = s.ring;
+          if (capA >= 0) { ring[(base + 2) * pcells + c] = vx; ring[(base + 4) * pcells + c] = vy; }
+          if (capB >= 0) { ring[(base + 3) * pcells + c] = vx; ring[(base + 5) * pcells + c] = vy; }
+          if (capC >= 0) { ring[(base + 8) * pcells + c] = vx; ring[(base + 10) * pcells + c] = vy; }
+          if (capD >= 0) { ring[(base + 9) * pcells + c] = vx; ring[(base + 11) * pcells + c] = vy; }
+        }
       }
-      // update_hy: i (ib,ie), j [jb,je), k (kb,ke); psi hy_x (+), hy_z (-)
-      T vy = (by || src || icap) ? Hy[idx] : T(0);
-      if (src) vy -= jv;
-      if (by) {
-        const T dEz = ez - ezm;
-        const T dEx = Ex[idx] - Ex[idx - 1];
-        vy = vy + db * (dEz * s.ikh[0][i] - dEx * ikz);
-        if (si >= 0) vy += db * psi_upd(s.psi[psi_id(F_HY, 0, si)] + ((size_t)di * ny + j) * nz + k, s.bh[ci], s.cho[ci], dEz);
-        if (sk >= 0) vy -= db * psi_upd(s.psi[psi_id(F_HY, 2, sk)] + ((size_t)i * ny + j) * N + dk, s.bh[ck], s.cho[ck], dEx);
-      }
-      if (by || src) Hy[idx] = vy;
-      // update_hz: i (ib,ie), j (jb,je), k [kb,ke); psi hz_x (-), hz_y (+)
-      if (bz) {
-        const T dEx = Ex[idx] - Ex[idx - nz];
-        const T dEy = ey - eym;
-        T v = Hz[idx] + db * (dEx * iky - dEy * s.ikh[0][i]);
-        if (si >= 0) v -= db * psi_upd(s.psi[psi_id(F_HZ, 0, si)] + ((size_t)di * ny + j) * nz + k, s.bh[ci], s.cho[ci], dEy);
-        if (sj >= 0) v += db * psi_upd(s.psi[psi_id(F_HZ, 1, sj)] + ((size_t)i * N + dj) * nz + k, s.bh[cj], s.cho[cj], dEx);
-        Hz[idx] = v;
-      }
-      if (icap) {
-        const int c = (i - s.wx0) * s.prow + (j - s.wy0);
-        if (capA >= 0) { capture(s, tstep, 2, c, vx); capture(s, tstep, 4, c, vy); }
-        if (capB >= 0) { capture(s, tstep, 3, c, vx); capture(s, tstep, 5, c, vy); }
-        if (capC >= 0) { capture(s, tstep, 8, c, vx); capture(s, tstep, 10, c, vy); }
-        if (capD >= 0) { capture(s, tstep, 9, c, vx); capture(s, tstep, 11, c, vy); }
-      }
+      eym = p0.ey;
+      ezm = p0.ez;
+    };
+    // Fully unrolled over the tile's planes, so the next plane's registers
+    // are new ones (no copy waits for its loads).
+    H3Plane<T> pc = load(i0);
+#pragma unroll
+    for (int u = 0; u < XC; ++u) {
+      const int i = i0 + u;
+      if (i >= iend) break;
+      H3Plane<T> pn;
+      if (u + 1 < XC) pn = load(min(i + 1, iend - 1));
+      update(i, pc);
+      if (u + 1 < XC) pc = pn;
     }
   }
 }

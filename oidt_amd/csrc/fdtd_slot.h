@@ -66,6 +66,11 @@ struct Slot {
   int wcells;              // window cells
   int nplanes;             // raw planes per step
   int nq;                  // DFT sample components
+  // Byte distance from a component's low-side psi array to its high-side one
+  // along the same axis (the same for every component of an axis): a kernel
+  // picks the side of a cell by adding it to a 32-bit offset from the
+  // low-side array.
+  unsigned psi_hi[3];
   T db;
   T dxT;    // delta_x in the run precision (lossy TMz: curl_h * dx)
   T dtmu;   // delta_t / mu in the run precision (lossy TMz H CPML)

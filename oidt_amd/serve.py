@@ -61,7 +61,7 @@ class Request:
 class DeviceBackend:
     """The GPU: one oidt batch per (scheme, precision)."""
 
-    def __init__(self, max_slots=64, xchunk=0, variant=2, launch=None):
+    def __init__(self, max_slots=64, xchunk=0, variant=0, launch=None):
         """``launch`` maps a scheme ('tm', 'te', '3d') to its (variant, xchunk[,
         grid]), e.g. the autotuner's picks; other schemes use ``variant`` and
         ``xchunk``."""

@@ -160,8 +160,9 @@ class Solver:
         self.h = h
         return self
 
-    def set_launch(self, xchunk=0, grid_blocks=0, variant=2):
-        """Kernel form and x-chunk depth (before the first advance)."""
+    def set_launch(self, xchunk=0, grid_blocks=0, variant=0):
+        """Kernel form and x-chunk depth (before the first advance); 0 keeps
+        the current value (the library default for the scheme)."""
         check(lib().oidt_solver_set_launch(self.h, int(xchunk), int(grid_blocks), int(variant)))
         return self
 
@@ -271,7 +272,8 @@ class Batch:
         check(lib().oidt_batch_read_monitor(self.h, int(slot), int(idx), ptr(re), ptr(im), re.nbytes))
         return mon.result(re, im)
 
-    def set_launch(self, xchunk=8, grid_blocks=0, variant=2):
+    def set_launch(self, xchunk=0, grid_blocks=0, variant=0):
+        """Kernel form, x-chunk depth and grid cap; 0 keeps the current value."""
         check(lib().oidt_batch_set_launch(self.h, int(xchunk), int(grid_blocks), int(variant)))
 
     def launch(self):

@@ -446,7 +446,7 @@ def truncate(c, nt):
     return c
 
 
-def prepare(c, launch=(2, 0, 0)):
+def prepare(c, launch=(0, 0, 0)):
     """(solver, monitors, monitor ids): the study's run on the device with its
     monitors attached, not yet advanced."""
     variant, xchunk, grid = launch
@@ -464,7 +464,7 @@ def prepare(c, launch=(2, 0, 0)):
     return sol, mons, ids
 
 
-def run(c, launch=(2, 0, 0)):
+def run(c, launch=(0, 0, 0)):
     """Step the study on the device; returns {name: array} in the reference's
     output shapes and types (complex64 phasors, float32 time records)."""
     sol, mons, ids = prepare(c, launch)
@@ -475,7 +475,7 @@ def run(c, launch=(2, 0, 0)):
         sol.close()
 
 
-def prepare_batch(cs, launch=(2, 0, 0)):
+def prepare_batch(cs, launch=(0, 0, 0)):
     """(batch, runs): several studies of one scheme in one device batch with
     their monitors attached, not yet advanced; runs = [(config, slot,
     monitors, monitor ids, steps)]."""
@@ -489,7 +489,7 @@ def prepare_batch(cs, launch=(2, 0, 0)):
         raise ValueError('the study drivers need a two-kernel form (monitors sit between the half-steps)')
     b = Batch('tm_lossy' if scheme == 'tm' else scheme, np.float32, max_slots=len(cs))
     try:
-        b.set_launch(xchunk or (1 if scheme == '3d' else 4), grid, variant)
+        b.set_launch(xchunk, grid, variant)
         runs = []
         for c in cs:
             s, fg, cb = build(c)
@@ -507,7 +507,7 @@ def prepare_batch(cs, launch=(2, 0, 0)):
     return b, runs
 
 
-def run_batch(cs, launch=(2, 0, 0)):
+def run_batch(cs, launch=(0, 0, 0)):
     """Several studies of one scheme (e.g. the roughness realisations that
     automated_results_collection.py runs one process each) as one device
     batch; returns one output dict per configuration, as ``run``."""

@@ -8,7 +8,7 @@ from oidt_amd.solver import Batch
 
 from helpers import device_inputs, oracle_run, rel_l2, window
 
-FORMS = [(1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
+FORMS = [(1, 1), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
 FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32)], '3d': [(3, 12), (3, 16)], 'te': [(3, 16), (3, 32)]}
 
@@ -73,29 +73,32 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('precision,xc,direction,extra', [
-    ('float64', 1, 'f', {}),                                   # runs [64,128) and [128,192) outside the slabs
-    ('float64', 2, 'b', {'source_condition': 'TM'}),
-    ('float64', 1, 'f', {'sz': 70}),                           # source slab inside the first run
-    ('float32', 4, 'f', {'p1': 63, 'p2': 127}),                # H samples planes 64 and 128
-    ('float32', 1, 'b', {}),
+@pytest.mark.parametrize('precision,variant,xc,direction,extra', [
+    ('float64', 2, 1, 'f', {}),                                # runs [64,128) and [128,192) outside the slabs
+    ('float64', 2, 2, 'b', {'source_condition': 'TM'}),
+    ('float64', 2, 1, 'f', {'sz': 70}),                        # source slab inside the first run
+    ('float32', 2, 4, 'f', {'p1': 63, 'p2': 127}),             # H samples planes 64 and 128
+    ('float32', 2, 1, 'b', {}),
+    ('float64', 1, 4, 'f', {'sz': 70}),                        # the march form
+    ('float64', 1, 8, 'b', {'source_condition': 'TM'}),
+    ('float32', 1, 16, 'f', {'p1': 63, 'p2': 127}),
 ])
-def test_3d_long_z_runs(precision, xc, direction, extra):
+def test_3d_long_z_runs(precision, variant, xc, direction, extra):
     """A guide long enough in z (nz = 250) for four 64-cell z runs per row:
     runs wholly outside the z-CPML slabs, runs that hold a source slab or a
     port plane (E samples plane p, H samples p and p+1, here on both sides of
     a run boundary), and the partial last run all give the oracle's fields and
-    port DFT in the two-kernel 3D form."""
+    port DFT in the two-kernel 3D forms."""
     p = configs.strip3d(250, nx=36, ny=34, nt=200, nf=6, precision=precision, width=12, height=8, port_len=10,
                         buffer_z=6, source_type=1, gauss_pulse_deg=-1.0)
     p.update(extra)
     p.update(rough_std=15e-9, rough_acl=40e-9, seed=3)
     s, up, lo, fg = device_inputs(p)
     o, _ = oracle_run(p, up, lo, direction)
-    print('case', precision, 'xchunk', xc, direction, extra, flush=True)
+    print('case', precision, 'variant', variant, 'xchunk', xc, direction, extra, flush=True)
     tol = 1e-10 if precision == 'float64' else 1e-4
     b = Batch('3d', np.dtype(precision).type, max_slots=1)
-    b.set_launch(xc, 0, 2)
+    b.set_launch(xc, 0, variant)
     slot = b.add(s, fg, direction)
     b.advance(-(-s.nt // 32))
     b.sync()
@@ -142,4 +145,25 @@ def test_launch_form_validation():
         d.set_launch(12, 0, 3)
     finally:
         d.close()
+
+
+@pytest.mark.gpu
+def test_default_launch_forms():
+    """A new batch starts in its scheme's default form (3D: the march over
+    4-plane tiles; 2D: batched loads, 4 cells per thread), and 0 keeps any
+    knob as it is."""
+    d = Batch('3d', np.float32, max_slots=1)
+    t = Batch('tm', np.float32, max_slots=1)
+    try:
+        assert d.launch() == (1, 4, 0)
+        assert t.launch() == (2, 4, 0)
+        d.set_launch(0, 0, 0)
+        assert d.launch() == (1, 4, 0)
+        d.set_launch(2, 0, 0)
+        assert d.launch() == (1, 2, 0)
+        d.set_launch(0, 4096, 2)
+        assert d.launch() == (2, 2, 4096)
+    finally:
+        d.close()
+        t.close()
 
