@@ -370,12 +370,15 @@ __global__ __launch_bounds__(kThreads) void k_tez_h(const Slot<T>* __restrict__ 
 // 3D half-steps as a prefetching march.  A wave owns one (j row, 64 z-cells)
 // strip of XC planes and walks it along x; the loads of plane i+1 are issued
 // before plane i is updated, so every wave keeps a plane of loads in flight
-// while it computes and stores.  Hy and Hz of plane i+1 (E) or Ey and Ez of
-// plane i (H) come from the neighbouring plane's loads.  The row (y) values
-// are wave-uniform; the plane's x-only coefficients are loaded once per tile,
-// one plane per lane, and broadcast with readlane; the run values used in the
-// loop are copied to registers first (stores through the field pointers could
-// otherwise alias the descriptor and force reloads).
+// while it computes and stores.  The walk is fully unrolled over the tile's
+// planes, so each plane's values land in registers of their own.  Each E
+// plane record also holds Hy and Hz of plane i+1, which the next plane reuses
+// as its own; the H half-step reuses Ey and Ez of plane i-1 from the previous
+// record.  The row (y) values are wave-uniform; the plane's x-only
+// coefficients are loaded once per tile, one plane per lane, and broadcast
+// with readlane; the cb values are copied to registers first and the source
+// and capture planes become bit masks over the tile (stores through the field
+// pointers could otherwise alias the descriptor and force reloads).
 
 // Bits u of the tile's planes i0 + u (u < 64) that lie in [lo, hi).
 __device__ __forceinline__ unsigned long long plane_bits(int i0, int lo, int hi) {
