@@ -96,6 +96,19 @@ def test_padded_field_size_limit():
     assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.OK
 
 
+def test_3d_psi_slab_pair_limit():
+    """The 3D march addresses a high-side psi slab from the low-side array of
+    the same component by a 32-bit offset: a grid whose fields fit 4 GiB but
+    whose x slabs (with the other side and the second set between them) do
+    not is refused before anything is allocated."""
+    d = _desc('3d', nx=25, ny=2000, nz=20000)       # 4.0 GB per field, 3 x 1.6 GB of x slab
+    rc = _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64()))
+    assert rc == _lib.EINVAL
+    assert 'psi slab pair' in _lib.lib().oidt_last_error().decode()
+    d = _desc('3d', nx=31, ny=1700, nz=20000)       # 4.2 GB per field, 3 x 1.36 GB of x slab
+    assert _lib.lib().oidt_run_footprint(C.byref(d), C.byref(C.c_uint64())) == _lib.OK
+
+
 def test_struct_layouts_match_the_header(tmp_path):
     """ctypes mirrors of the ABI structs have the C compiler's sizes and field
     offsets (compiled against include/oidt_fdtd.h with the host gcc)."""
