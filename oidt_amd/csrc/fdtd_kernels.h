@@ -2876,7 +2876,8 @@ __global__ __launch_bounds__(kThreads) void k_tmzl_h(const Slot<T>* __restrict__
 
 // Field monitors taken after one half-step: one work item = (monitor, first
 // of kMonCells consecutive box cells, -, -); each lane takes kMonCells / 64
-// cells kThreads apart, loads all of them first, then folds and stores.
+// cells kThreads apart, loads all of them (and the running sums) first, then
+// folds and stores.
 // Reads the current field set (the two-kernel forms keep the state in set A).
 // DFT: the stencil sum is formed in T point by point, scaled and weighted in
 // double, and the running sum is rounded back to T each step, as the
@@ -2899,13 +2900,23 @@ __global__ __launch_bounds__(kThreads) void k_monitor(const Slot<T>* __restrict_
     const int cells = m.ex * m.ey * m.ez;
     const T* __restrict__ f = s.f[m.field];
     const bool is3d = s.scheme == SCHEME_3D;
+    // running sums first: their loads overlap the stencil's
+    T re[kMonPer], im[kMonPer];
+    if (m.mode == 0) {
+#pragma unroll
+      for (int k = 0; k < kMonPer; ++k) {
+        const int c = it.y + (int)threadIdx.x + k * kThreads;
+        re[k] = c < cells ? m.re[c] : T(0);
+        im[k] = c < cells ? m.im[c] : T(0);
+      }
+    }
     T sum[kMonPer];
 #pragma unroll
     for (int k = 0; k < kMonPer; ++k) {
       const int c = it.y + (int)threadIdx.x + k * kThreads;
       sum[k] = T(0);
       if (c >= cells) continue;
-      const int cz = c % m.ez, cxy = c / m.ez;
+      const int cz = m.ez == 1 ? 0 : c % m.ez, cxy = m.ez == 1 ? c : c / m.ez;   // 2D boxes: no z division
       const int x = m.x0 + cxy / m.ey, y = m.y0 + cxy % m.ey, z = m.z0 + cz;
       for (int q = 0; q < m.npts; ++q) {
         const int xx = x + m.ox[q], yy = y + m.oy[q], zz = z + m.oz[q];
@@ -2917,13 +2928,6 @@ __global__ __launch_bounds__(kThreads) void k_monitor(const Slot<T>* __restrict_
     }
     if (m.mode == 0) {
       const double cr = m.cs[2 * r], ci = m.cs[2 * r + 1];
-      T re[kMonPer], im[kMonPer];
-#pragma unroll
-      for (int k = 0; k < kMonPer; ++k) {
-        const int c = it.y + (int)threadIdx.x + k * kThreads;
-        re[k] = c < cells ? m.re[c] : T(0);
-        im[k] = c < cells ? m.im[c] : T(0);
-      }
 #pragma unroll
       for (int k = 0; k < kMonPer; ++k) {
         const int c = it.y + (int)threadIdx.x + k * kThreads;
