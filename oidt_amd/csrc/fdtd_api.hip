@@ -103,8 +103,6 @@ struct SlotHost {
   std::vector<MonHost> mons;
 };
 
-int field_count(int scheme) { return scheme == SCHEME_3D ? 6 : 3; }
-
 bool field_used(int scheme, int f) {
   if (scheme == SCHEME_3D) return true;
   if (scheme == SCHEME_TMZ || scheme == SCHEME_TMZ_LOSSY) return f == F_EZ || f == F_HX || f == F_HY;
