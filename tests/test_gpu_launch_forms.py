@@ -8,7 +8,7 @@ from oidt_amd.solver import Batch
 
 from helpers import device_inputs, oracle_run, rel_l2, window
 
-FORMS = [(1, 1), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
+FORMS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
 FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32)], '3d': [(3, 12), (3, 16)], 'te': [(3, 16), (3, 32)]}
 
