@@ -61,7 +61,8 @@ def make_profile(length_eff, rough_std, rough_acl, dx, stol, atol, mtol, seed=0,
 
     accept='tolerance' keeps the reference's rejection loop and its exception
     after ``max_tries`` draws (fdtd_funcs.py:147-162); accept='first' returns the
-    first draw (the statistics are then reported, not enforced)."""
+    first draw (the statistics are then reported, not enforced).  ``seed`` is an
+    int or a ``numpy.random.SeedSequence``."""
     rng = np.random.Generator(np.random.PCG64(seed))
     for i in range(max_tries + 1):
         p = exp_acf_profile(length_eff, rough_std, rough_acl, rng)
@@ -101,10 +102,14 @@ def generate_profiles(mode, length, setup, usr, upper_name='', lower_name='', se
     effects (fdtd_funcs.py:165-192): the profiles are saved as .npy in every mode."""
     if mode == 'gen':
         stol, atol = usr['tol_std'] / 100, usr['tol_acl'] / 100
-        up = make_profile(length, setup.rstd, setup.racl, float(setup.delta_x), stol, atol, 0.01, seed=seed,
+        # Two independent child streams of the realisation's seed: realisation
+        # k shares no draw with realisation k + 1 (seeding the lower profile
+        # with seed + 1 would repeat the next realisation's upper profile).
+        seed_up, seed_lo = np.random.SeedSequence(int(seed)).spawn(2)
+        up = make_profile(length, setup.rstd, setup.racl, float(setup.delta_x), stol, atol, 0.01, seed=seed_up,
                           max_tries=usr.get('max_profile_tries', 300000),
                           accept=usr.get('profile_accept', 'tolerance'))
-        lo = make_profile(length, setup.rstd, setup.racl, float(setup.delta_x), stol, atol, 0.01, seed=seed + 1,
+        lo = make_profile(length, setup.rstd, setup.racl, float(setup.delta_x), stol, atol, 0.01, seed=seed_lo,
                           max_tries=usr.get('max_profile_tries', 300000),
                           accept=usr.get('profile_accept', 'tolerance'))
     elif mode == 'load':

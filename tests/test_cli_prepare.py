@@ -41,3 +41,31 @@ def test_missing_run_arguments_raise(tmp_path, monkeypatch):
         pickle.dump(small_user(), fh)
     with pytest.raises(Exception, match='No S-parameter sim type selected'):
         fdtd_main.main('tm', ['fdtd_main.py', '0'], workdir=str(tmp_path))
+
+
+def test_adjacent_profile_numbers_share_no_profile(tmp_path):
+    """'gen' mode draws the two sidewalls of realisation k from independent
+    child streams of its seed: no profile of realisation k reappears in
+    realisation k + 1 (the reference's draws are independent)."""
+    got = {}
+    for k in (0, 1, 2):
+        u = small_user()
+        u['profile_number'] = k
+        d = tmp_path / ('r%d' % k)
+        d.mkdir()
+        fdtd_main.prepare('tm', u, 0, 'f', 'r', workdir=str(d))
+        files = sorted(os.listdir(d / 'rough_profiles'))
+        got[k] = [np.load(d / 'rough_profiles' / f) for f in files]     # lower, upper
+    for k in (0, 1):
+        for a in got[k]:
+            for b in got[k + 1]:
+                assert not np.array_equal(a, b)
+        lo, up = got[k]
+        assert not np.array_equal(lo, up)
+    # seeded: the same profile number gives the same profiles again
+    u = small_user()
+    d = tmp_path / 'again'
+    d.mkdir()
+    fdtd_main.prepare('tm', u, 0, 'f', 'r', workdir=str(d))
+    again = [np.load(d / 'rough_profiles' / f) for f in sorted(os.listdir(d / 'rough_profiles'))]
+    assert all(np.array_equal(a, b) for a, b in zip(again, got[0]))
