@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the FDTD time-stepping hot path on one MI355X.
 
-Default workload: the plain 3D FP32 run (strip waveguide, (x, y, z) =
-(128, 96, 320) cells, 64 DFT frequencies in FP64 accumulators), a "step" being
-one Yee time step (E and H half-steps with CPML, source and port DFT).
+Default workload: the batched 3D ensemble of 16 roughness realisations, FP32
+fields, (x, y, z) = (96, 64, nz) with nz ~ U{192..512}, 64 DFT frequencies in
+FP64 accumulators, all resident in one batch; a "step" is one Yee time step
+of every realisation (E and H half-steps with CPML, source and port DFT).
 
 Prints one JSON line with Mcell-updates/s (all grid cells per step, the
-reference's MCells/s definition), the roofline of the dominant kernel measured
-with HIP events on the library stream, and the oracle timed on the host as
-the CPU baseline.
+reference's MCells/s definition) over exactly K steps after W warm-up steps,
+the same realisations run one at a time (the plain path), the roofline of the
+dominant kernel measured with HIP events on the batch stream, the S21 error of
+the full-length FP32 ensemble against FP64, the oracle timed on the host's
+cores as the CPU baseline, and summaries of the 64-run 2D ensemble and of the
+serving mix (p50 / p99 latency).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c0|c2|c3]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c0|c1|c2|c2te|c4|<study>]
 """
 import argparse
 import json
@@ -24,6 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def log(msg):
+    """Progress of the legs on stderr (the JSON line is the only stdout)."""
+    print('bench: ' + msg, file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(scheme, es):
@@ -142,14 +151,24 @@ def bench_study(args):
     forms = [(2, xc) for xc in (1, 2, 4, 8)] + ([(1, xc) for xc in (2, 4, 8)] if scheme == '3d' else [])
     forms = [(v, xc) for v, xc in forms if (not args.variant or v == args.variant) and
              (not args.xchunk or xc == args.xchunk)] or [(args.variant or 2, args.xchunk or 4)]
+    # tuning and kernel profiles run on their own copy of the study, long
+    # enough that a warm-up chunk and every timed chunk do whole steps
+    nprof = max(2, int(args.profile_chunks))
+    cp = variants.truncate(variants.config(scheme, study), (1 + nprof) * chunk + (1 if scheme == 'te' else 0))
+
+    def profiled(form, nchunks):
+        sol_, mons_, _ = variants.prepare(cp, (form[0], form[1], 0))
+        out_ = (C.c_double * 6)()
+        try:
+            _lib.check(_lib.lib().oidt_solver_profile_all(sol_.h, 1, out_))
+            _lib.check(_lib.lib().oidt_solver_profile_all(sol_.h, int(nchunks), out_))
+        finally:
+            sol_.close()
+        return list(out_), mons_
+
     tune = {}
     for v, xc in forms:
-        sol, mons, ids = variants.prepare(c, (v, xc, 0))
-        out = (C.c_double * 6)()
-        try:
-            _lib.check(_lib.lib().oidt_solver_profile_all(sol.h, 2, out))
-        finally:
-            sol.close()
+        out, _ = profiled((v, xc), 2)
         tune[(v, xc)] = out[0] + out[1] + out[4] + out[5]
     variant, xchunk = min(tune, key=tune.get)
     sol, mons, ids = variants.prepare(c, (variant, xchunk, 0))
@@ -162,13 +181,8 @@ def bench_study(args):
         sol.close()
     cells = c.nx * c.ny * (c.nz if scheme == '3d' else 1)
     value = cells * K / wall / 1e6
-    sol, mons, ids = variants.prepare(c, (variant, xchunk, 0))
-    out = (C.c_double * 6)()
-    try:
-        _lib.check(_lib.lib().oidt_solver_profile_all(sol.h, int(args.profile_chunks), out))
-    finally:
-        sol.close()
-    ms_e, ms_h, ms_dft, ms_adv, ms_me, ms_mh = list(out)
+    out, mons = profiled((variant, xchunk), nprof)
+    ms_e, ms_h, ms_dft, ms_adv, ms_me, ms_mh = out
     es = 4
     be, bh = algorithmic_bytes('tm' if scheme == 'tm' else scheme, es)
     stem = {'tm': 'k_tmzl', 'te': 'k_tez', '3d': 'k_3d'}[scheme]
@@ -179,6 +193,9 @@ def bench_study(args):
              ('k_monitor (after H)', ms_mh, monitor_bytes(mons, es, True))]
     kname, ms_k, bytes_k = max(cands, key=lambda t: t[1])
     achieved = bytes_k / (ms_k * 1e-3) / 1e9
+    if achieved > HBM_PEAK_GBS:
+        raise SystemExit('bench.py: %s measured %.1f GB/s of algorithmic bytes, above the %.0f GB/s HBM peak: '
+                         'the kernel timing is not valid' % (kname, achieved, HBM_PEAK_GBS))
     traffic, traffic_src = committed_traffic(kname.split()[0], args.workload, variant, xchunk)
     roof = {'bound': 'hbm', 'kernel': kname, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
             'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
@@ -271,113 +288,318 @@ def study_cpu_baseline(scheme, study, seconds):
                       % (n - 1, c.nx, c.ny, dt)}
 
 
-def bench_serving(args):
-    """Serving mix: plain one-at-a-time path, saturated continuous batching, and
-    Poisson arrivals at 0.8x the saturated capacity (latency percentiles)."""
-    from oidt_amd import configs, serve
-    descs = configs.serving_requests(count=args.requests)
-    be = serve.DeviceBackend(max_slots=64, xchunk=args.xchunk, variant=args.variant or 2)
+def bench_serving(args, count=None):
+    """Serving mix (configs.serving_requests: 70 % 2D TMz, 30 % 3D, 2000..8000
+    steps): the plain path (each request alone, back to back), continuous
+    batching with every request queued at t = 0 (saturated), and Poisson
+    arrivals at 0.8x the plain path's request capacity (latency percentiles),
+    under an HBM budget that makes admission wait for memory.  Every leg
+    includes each request's upload; host preparation (roughness, masks) is
+    done once before all legs."""
+    from oidt_amd import autotune, configs, serve
+    from oidt_amd.solver import Batch
+    n = int(count or args.requests)
+    descs = configs.serving_requests(count=n)
+    be = serve.DeviceBackend(max_slots=64)
     reqs = serve.make_requests(descs, [0.0] * len(descs))
     t0 = time.perf_counter()
     for r in reqs:
         serve.prepare_request(r, be)
     prep = time.perf_counter() - t0
     tuned = {}
-    if not args.variant:
-        # autotune each scheme once on (up to) 16 of its requests
-        for scheme in sorted({r.setup.scheme for r in reqs}):
-            sub = [(r.setup, r.fg) for r in reqs if r.setup.scheme == scheme][:16]
-            variant, xchunk, grid, rec = choose_launch(args, sub)
-            be.launch[scheme] = (variant, xchunk, grid)
-            tuned[scheme] = dict(rec, choice=[variant, xchunk, grid])
-    # plain: each request alone, back to back, same kernel forms
+    log('c4: %d requests prepared in %.1f s; autotuning' % (len(reqs), prep))
+    for scheme in sorted({r.setup.scheme for r in reqs}):
+        sub = [(r.setup, r.fg) for r in reqs if r.setup.scheme == scheme][:16]
+        if args.variant:
+            be.launch[scheme] = (args.variant, args.xchunk, args.grid)
+            continue
+        t1 = time.perf_counter()
+        choice, times = autotune.tune(sub, use_cache=False, cache_path=None)
+        be.launch[scheme] = choice
+        tuned[scheme] = {'choice': list(choice), 'ms_per_step': {k: round(v, 5) for k, v in times.items()},
+                         'seconds': round(time.perf_counter() - t1, 2)}
+    log('c4: plain leg, %d requests one at a time' % len(reqs))
     t0 = time.perf_counter()
-    plain = serve.serve_plain(reqs, be.launch)
+    serve.serve_plain(reqs, be.launch)
     plain_wall = time.perf_counter() - t0
     total_cu = sum(r.cell_updates for r in reqs)
     plain_rate = total_cu / plain_wall / 1e6
-    # saturated continuous batching (everything queued at t = 0)
-    hbm = int(0.8 * __import__('oidt_amd._lib', fromlist=['x']).device_info()['hbm_free_bytes'])
+    plain_rps = len(reqs) / plain_wall
+    # HBM budget: a quarter of the summed footprints of all requests, so that
+    # admission waits for memory while many runs stay resident
+    budget = int(sum(r.footprint for r in reqs) / 4)
     for r in reqs:
         r.result = None
-    srv = serve.Server(be, hbm_budget=hbm, chunk_group=4, prepare=lambda r, b: None)
+    log('c4: saturated continuous batching')
+    srv = serve.Server(be, hbm_budget=budget, chunk_group=4, prepare=lambda r, b: None)
     t0 = time.perf_counter()
     done = srv.run(reqs)
     sat_wall = time.perf_counter() - t0
     sat = serve.summarize(done, sat_wall)
-    cap_rps = len(reqs) / sat_wall
-    # Poisson arrivals at 0.8x the saturated request capacity
-    rate = 0.8 * cap_rps
+    sat_deferred = srv.deferred_for_memory
+    # Poisson arrivals at 0.8x the plain path's capacity
+    rate = 0.8 * plain_rps
     arr = serve.poisson_arrivals(len(reqs), rate, seed=4)
     preqs = serve.make_requests(descs, arr)
     for a, b_ in zip(preqs, reqs):
         a.setup, a.fg, a.footprint = b_.setup, b_.fg, b_.footprint
-    srv2 = serve.Server(be, hbm_budget=hbm, chunk_group=4, prepare=lambda r, b: None)
+    log('c4: Poisson arrivals at %.3f requests/s' % rate)
+    srv2 = serve.Server(be, hbm_budget=budget, chunk_group=4, prepare=lambda r, b: None)
     t0 = time.perf_counter()
     pdone = srv2.run(preqs)
     pw = time.perf_counter() - t0
     pois = serve.summarize(pdone, pw)
     be.close()
+    # dominant kernel: per scheme, a batch of (up to) 16 of its requests
+    roofs = {}
+    for scheme, (v, xc, g) in ((k, (tuple(c) + (0,))[:3]) for k, c in be.launch.items()):
+        sub = [(r.setup, r.fg) for r in reqs if r.setup.scheme == scheme][:16]
+        if not sub:
+            continue
+        cells = sum(s_.cells for s_, _ in sub)
+        roofs[scheme] = roofline_of(scheme, 4, v, xc, cells, profile_kernels(sub, (v, xc, g), args.profile_chunks),
+                                    'c4')
+    # the dominant kernel: the scheme whose stencil launches take the most
+    # device time over the whole mix (its cell-updates at its profiled rate)
+    share = {}
+    for scheme, rf in roofs.items():
+        sub = [r for r in reqs if r.setup.scheme == scheme]
+        prof_cells = sum(r.setup.cells for r in sub[:16])
+        share[scheme] = sum(r.cell_updates for r in sub) / prof_cells * (rf['ms_e'] + rf['ms_h'])
+    dom = dict(roofs[max(share, key=share.get)])
     n2d = sum(1 for r in reqs if r.setup.scheme == 'tm')
-    line = {
+    rnd = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+    return {
         'metric': 'Mcell-updates/s', 'value': round(sat['mcell_updates_per_s'], 1), 'unit': 'Mcell-updates/s',
-        'n_gpus': 1, 'steps': int(sum(r.setup.nt for r in reqs)), 'warmup': 0,
-        'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
-        'data': 'synthetic',
-        'config': {'workload': 'serving mix: %d requests, %d 2D TMz (nx 400..4000, ny 200..240) and %d 3D '
-                               '((128..384)x(64..128)x(48..96)), 2000..8000 steps, FP32' % (len(reqs), n2d, len(reqs) - n2d),
-                   'max_slots_per_batch': 64, 'hbm_budget_bytes': hbm, 'launch': be.launch or None,
-                   'autotune': tuned or None},
-        'plain_mcell_updates_per_s': round(plain_rate, 1),
+        'workload': 'serving mix: %d requests, %d 2D TMz (nx 400..4000, ny 200..240) and %d 3D '
+                    '((128..384)x(64..128)x(48..96)), 2000..8000 steps, FP32' % (len(reqs), n2d, len(reqs) - n2d),
+        'cell_updates': int(total_cu), 'hbm_budget_bytes': budget,
+        'sum_footprint_bytes': int(sum(r.footprint for r in reqs)),
+        'launch': {k: list(v) for k, v in be.launch.items()}, 'autotune': tuned or None,
+        'plain_mcell_updates_per_s': round(plain_rate, 1), 'plain_requests_per_s': round(plain_rps, 3),
+        'plain_wall_s': round(plain_wall, 3),
         'served_over_plain': round(sat['mcell_updates_per_s'] / plain_rate, 3),
-        'saturated': {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sat.items()},
-        'poisson': dict({k: (round(v, 4) if isinstance(v, float) else v) for k, v in pois.items()},
-                        rate_per_s=round(rate, 3)),
-        'peak_resident_runs': srv.peak_resident, 'host_prepare_s': round(prep, 3),
+        'saturated': dict(rnd(sat), admissions_deferred_for_memory=sat_deferred,
+                          peak_resident_runs=srv.peak_resident, peak_resident_bytes=srv.peak_bytes),
+        'poisson': dict(rnd(pois), rate_per_s=round(rate, 3),
+                        admissions_deferred_for_memory=srv2.deferred_for_memory,
+                        peak_resident_runs=srv2.peak_resident),
+        'roofline': dom, 'roofline_per_scheme': roofs,
+        'host_prepare_s': round(prep, 3),
     }
-    print(json.dumps(line), flush=True)
 
 
-def bench_plain_ensemble(args, descs, label):
-    """The ensemble one run at a time (each alone on the device), same steps,
-    same kernel form as the batched path (autotuned on the first run when
-    --variant is 0)."""
+def cpu_pool_worker(args):
+    """One CPU-baseline worker: the oracle on description ``p`` for about
+    ``seconds`` of whole steps (after one warm-up step)."""
+    p, seconds = args
+    rate, dt, steps = cpu_baseline(p, seconds)
+    return rate, dt, steps
+
+
+def cpu_baseline_pool(p, seconds, label):
+    """The oracle (FP64 NumPy, one thread per process) on one worker per host
+    core of this process's affinity set, capped at 16 (the host share of one
+    GPU): aggregate Mcell-updates/s of the independent solves."""
+    import multiprocessing as mp
+    aff = len(os.sched_getaffinity(0))
+    n = max(1, min(aff, 16))
+    ctx = mp.get_context('spawn')      # fresh interpreters: nothing of the HIP runtime is inherited
+    with ctx.Pool(n) as pool:
+        res = pool.map(cpu_pool_worker, [(p, seconds)] * n)
+    rate = sum(r for r, _, _ in res)
+    steps = sum(s for _, _, s in res)
+    return {'value': round(rate, 3), 'unit': 'Mcell-updates/s', 'cores': n, 'affinity_cpus': aff,
+            'kind': 'port',
+            'sample': '%d independent FP64 NumPy oracle solves (one per core) of %s, %d whole steps in '
+                      'total after one warm-up step each, about %.0f s each' % (n, label, steps, seconds)}
+
+
+def ensemble_runs(descs, nt):
+    """[(setup, fg)] of the descriptions with ``nt`` steps each."""
     from oidt_amd import configs, materials
-    from oidt_amd.solver import Solver
-    K = args.steps
-    runs = []
+    out = []
     for p in descs:
-        p = dict(p, nt=K)
-        s, up, lo = configs.realise(p)
-        runs.append((s, materials.fg_mask(s, up, lo)))
-    variant, xchunk, grid, tuned = choose_launch(args, runs[:1])
-    # warm the device on the first run
-    Solver(*runs[0]).set_launch(xchunk, grid, variant).advance(64).sync().close()
+        s, up, lo = configs.realise(dict(p, nt=nt))
+        out.append((s, materials.fg_mask(s, up, lo)))
+    return out
+
+
+def timed_batch(runs, launch, K):
+    """Wall time (s) of the K steps of ``runs`` as one batch, uploads and
+    graph capture done before the timed region."""
+    from oidt_amd.solver import Batch
+    s0 = runs[0][0]
+    b = Batch(s0.scheme, s0.precision, max_slots=len(runs))
+    try:
+        b.set_launch(launch[1], launch[2], launch[0])
+        for s, fg in runs:
+            b.add(s, fg)
+        b.prepare()
+        b.sync()
+        t0 = time.perf_counter()
+        b.advance(-(-K // 32))
+        b.sync()
+        return time.perf_counter() - t0
+    finally:
+        b.close()
+
+
+def profile_kernels(runs, launch, chunks):
+    """Mean HIP-event durations (ms) of the E, H, DFT and advance launches on
+    a batch of ``runs`` lasting 1 + ``chunks`` chunks: one warm-up chunk, then
+    ``chunks`` timed chunks in which every launch does a full step of work."""
+    import ctypes as C
+    from oidt_amd import _lib
+    from oidt_amd.autotune import with_steps
+    from oidt_amd.solver import Batch
+    need = (1 + int(chunks)) * 32
+    s0 = runs[0][0]
+    pb = Batch(s0.scheme, s0.precision, max_slots=len(runs))
+    try:
+        pb.set_launch(launch[1], launch[2], launch[0])
+        for s, fg in runs:
+            pb.add(with_steps(s, need), fg)
+        out = (C.c_double * 4)()
+        _lib.check(_lib.lib().oidt_batch_profile(pb.h, 1, out))
+        _lib.check(_lib.lib().oidt_batch_profile(pb.h, int(chunks), out))
+        return list(out)
+    finally:
+        pb.close()
+
+
+def roofline_of(scheme, es, variant, xchunk, cells, ms, key):
+    """Roofline of the dominant stencil kernel from its HIP-event duration."""
+    ms_e, ms_h, ms_dft, ms_adv = ms
+    be, bh = algorithmic_bytes(scheme, es)
+    stem = {'tm': 'k_tmz', 'te': 'k_tez', '3d': 'k_3d'}[scheme]
+    if variant == 3:
+        kname, ms_k, bytes_k = (stem + '_step', ms_e, (be + bh) * cells)
+    elif ms_e >= ms_h:
+        kname, ms_k, bytes_k = (stem + ('_e2' if variant == 2 else '_e'), ms_e, be * cells)
+    else:
+        kname, ms_k, bytes_k = (stem + ('_h2' if variant == 2 else '_h'), ms_h, bh * cells)
+    achieved = bytes_k / (ms_k * 1e-3) / 1e9
+    frac = achieved / HBM_PEAK_GBS
+    if frac > 1.0:
+        raise SystemExit('bench.py: %s measured %.1f GB/s of algorithmic bytes, above the %.0f GB/s HBM peak: '
+                         'the kernel timing is not valid' % (kname, achieved, HBM_PEAK_GBS))
+    traffic, traffic_src = committed_traffic(kname, key, variant, xchunk)
+    return {'bound': 'hbm', 'kernel': kname, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
+            'unit': 'GB/s', 'frac': round(frac, 4), 'traffic': traffic, 'traffic_source': traffic_src,
+            'bytes_per_launch': bytes_k, 'ms_per_launch': round(ms_k, 5),
+            'ms_e': round(ms_e, 5), 'ms_h': round(ms_h, 5), 'ms_dft': round(ms_dft, 5),
+            'ms_advance': round(ms_adv, 5)}
+
+
+def s21_accuracy(descs):
+    """Per-realisation S21 / S11 of the full-length ensemble in FP32 against
+    the same ensemble in FP64 (one shared incident run each, device
+    extraction): max |dB| and max |degrees| differences over the frequencies."""
+    from oidt_amd import ensemble
     t0 = time.perf_counter()
-    for s, fg in runs:
-        sv = Solver(s, fg)
-        sv.set_launch(xchunk, grid, variant)
-        sv.run()
-        sv.close()
-    wall = time.perf_counter() - t0
+    e32 = ensemble.Ensemble(descs, precision='float32').run()
+    e64 = ensemble.Ensemble(descs, precision='float64').run()
+    rows = []
+    for r in range(len(descs)):
+        a, b = e32.s_params(r), e64.s_params(r)
+        d21 = ensemble.s_error(a['S21'], b['S21'])
+        d11 = ensemble.s_error(a['S11'], b['S11'])
+        rows.append({'r': r, 's21_db_mean': round(float(np.mean(ensemble.db(b['S21']))), 4),
+                     's21_err_db': float('%.3g' % d21[0]), 's21_err_deg': float('%.3g' % d21[1]),
+                     's11_err_db': float('%.3g' % d11[0])})
+    saved = e32.cell_updates_unshared() - e32.cell_updates
+    e32.close()
+    e64.close()
+    return {'reference': 'the same ensemble in FP64 on the device (FP64 kernels pinned to the FP64 oracle at 1e-10)',
+            'steps': int(e32.nt), 'incident_runs': len(e32.incident),
+            'cell_updates_saved_by_shared_incident': int(saved),
+            'max_s21_err_db': max(r['s21_err_db'] for r in rows),
+            'max_s21_err_deg': max(r['s21_err_deg'] for r in rows),
+            'per_realisation': rows, 'seconds': round(time.perf_counter() - t0, 1)}
+
+
+def bench_ensemble(args, descs, label):
+    """An ensemble (or a single run) as one batch: K timed steps after W
+    warm-up steps, the same runs one at a time (plain path), the dominant
+    kernel's roofline, the S21 accuracy of the full-length ensemble and the
+    CPU baseline."""
+    from oidt_amd import _lib
+    W, K = int(args.warmup), int(args.steps)
+    runs = ensemble_runs(descs, K)
+    scheme, prec = runs[0][0].scheme, runs[0][0].precision
+    es = 4 if prec == np.float32 else 8
+    log('%s: %d runs, autotuning' % (args.workload, len(runs)))
+    variant, xchunk, grid, tuned = choose_launch(args, runs)
+    launch = (variant, xchunk, grid)
+    log('%s: launch form %r; warm-up and %d timed steps' % (args.workload, launch, K))
+    # warm-up: the same runs for W steps in their own batch
+    if W > 0:
+        timed_batch(ensemble_runs(descs, W), launch, W)
+    wall = timed_batch(runs, launch, K)
     cells = sum(s.cells for s, _ in runs)
-    print(json.dumps({'metric': 'Mcell-updates/s', 'value': round(cells * K / wall / 1e6, 1),
-                      'unit': 'Mcell-updates/s', 'n_gpus': 1, 'steps': K, 'warmup': 0,
-                      'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True, 'scaling': 'weak',
-                      'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
-                      'config': {'workload': label + ', plain: one run at a time incl. create/upload',
-                                 'runs': len(runs), 'variant': variant, 'xchunk': xchunk, 'grid_cap': grid,
-                                 'autotune': tuned}}),
-          flush=True)
+    value = cells * K / wall / 1e6
+    line = {
+        'metric': 'Mcell-updates/s', 'value': round(value, 1), 'unit': 'Mcell-updates/s', 'n_gpus': 1,
+        'steps': K, 'warmup': W, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32' if es == 4 else 'f64', 'data': 'synthetic',
+        'config': {'workload': label, 'runs': len(runs), 'cells_per_step': cells, 'steps_per_run': K,
+                   'xchunk': xchunk, 'variant': variant, 'grid_cap': grid, 'parallelism': 'single GPU',
+                   'timed_region': 'K steps of every run; uploads and graph capture before it',
+                   'autotune': tuned},
+    }
+    be, bh = algorithmic_bytes(scheme, es)
+    line['hbm_roofline_frac_step'] = round(value * 1e6 * (be + bh) / (HBM_PEAK_GBS * 1e9), 4)
+    line['algorithmic_bytes_per_cell_update'] = be + bh
+    if len(runs) > 1 and not args.no_plain:
+        # plain path: each run alone, in its own batch, tuned for one run;
+        # uploads and graph capture outside the timed region as above
+        log('%s: plain leg (each run alone)' % args.workload)
+        pv, pxc, pg, ptuned = choose_launch(args, runs[:1])
+        pwall = sum(timed_batch([r], (pv, pxc, pg), K) for r in runs)
+        plain = cells * K / pwall / 1e6
+        line['plain_one_at_a_time'] = round(plain, 1)
+        line['plain_launch'] = [pv, pxc, pg]
+        line['batched_over_plain'] = round(value / plain, 3)
+    log('%s: kernel profile' % args.workload)
+    line['roofline'] = roofline_of(scheme, es, variant, xchunk, cells,
+                                   profile_kernels(runs, launch, args.profile_chunks), args.workload)
+    if args.workload in ('c3', 'c2', 'c2te') and not args.no_accuracy:
+        log('%s: full-length S21 accuracy, FP32 against FP64' % args.workload)
+        line['s21_vs_fp64'] = s21_accuracy(descs)
+    if not args.no_cpu:
+        p0 = dict(descs[0])
+        lbl = 'the %s grid of the first run' % 'x'.join(str(p0[k]) for k in ('nx', 'ny', 'nz') if k in p0)
+        log('%s: CPU baseline' % args.workload)
+        line['cpu_baseline'] = cpu_baseline_pool(p0, args.cpu_seconds, lbl)
+    return line
+
+
+def extra_ensemble(args, name, nt):
+    """Summary of another ensemble at its full step count (batched against
+    one at a time, dominant kernel roofline)."""
+    descs, label = workload(name)
+    sub = argparse.Namespace(**dict(vars(args), steps=nt, warmup=32, workload=name, no_cpu=True,
+                                    no_accuracy=True, no_plain=False))
+    t0 = time.perf_counter()
+    line = bench_ensemble(sub, descs, label)
+    keep = ('value', 'steps', 'ms_per_step', 'hbm_roofline_frac_step', 'plain_one_at_a_time',
+            'batched_over_plain', 'roofline')
+    out = {k: line[k] for k in keep if k in line}
+    out['workload'] = label
+    out['launch'] = [line['config'][k] for k in ('variant', 'xchunk', 'grid_cap')]
+    out['seconds'] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=6016)
-    ap.add_argument('--warmup', type=int, default=256)
-    ap.add_argument('--workload', default='c1',
-                    help='c0, c1, c2, c2te, c3, c4 (serving), or a study: sl_tm, sl_te, sl_3d, wi_tm, wi_te, wi_3d')
+    ap.add_argument('--steps', type=int, default=1024)
+    ap.add_argument('--warmup', type=int, default=64)
+    ap.add_argument('--workload', default='c3',
+                    help='c3 (default), c0, c1, c2, c2te, c4 (serving), or a study: sl_tm, sl_te, sl_3d, wi_tm, '
+                         'wi_te, wi_3d')
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=0,
                     help='kernel form: 1 march, 2 batched loads, 3 fused step; 0: autotuner picks form and xchunk')
@@ -385,8 +607,11 @@ def main():
     ap.add_argument('--profile-chunks', type=int, default=4)
     ap.add_argument('--cpu-seconds', type=float, default=15.0, help='length of the CPU baseline sample')
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--plain', action='store_true', help='ensembles: one run at a time instead of one batch')
-    ap.add_argument('--requests', type=int, default=48, help='serving workload: number of requests')
+    ap.add_argument('--no-plain', action='store_true', help='ensembles: skip the one-at-a-time leg')
+    ap.add_argument('--no-accuracy', action='store_true', help='ensembles: skip the full-length S21 accuracy leg')
+    ap.add_argument('--no-extras', action='store_true',
+                    help='default workload: skip the config 2 ensemble and config 4 serving summaries')
+    ap.add_argument('--requests', type=int, default=256, help='serving workload: number of requests')
     ap.add_argument('--realisations', type=int, default=1,
                     help='study workloads: roughness realisations in one batch (against the same run one at a time)')
     ap.add_argument('--copy-probe', type=float, default=0.0,
@@ -399,8 +624,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    from oidt_amd import _lib, configs, materials
-    from oidt_amd.solver import Batch
+    from oidt_amd import _lib
 
     if args.copy_probe > 0:
         nbytes = int(args.copy_probe * 2 ** 20)
@@ -409,88 +633,23 @@ def main():
         return
 
     if args.workload == 'c4':
-        bench_serving(args)
+        c4 = bench_serving(args)
+        line = {'metric': 'Mcell-updates/s', 'value': c4.pop('value'), 'unit': 'Mcell-updates/s', 'n_gpus': 1,
+                'steps': None, 'warmup': 0, 'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak',
+                'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+                'config': {'workload': c4.pop('workload'), 'parallelism': 'single GPU'}}
+        line.update(c4)
+        print(json.dumps(line), flush=True)
         return
     if args.workload in STUDIES:
         bench_study(args)
         return
     descs, label = workload(args.workload)
-    if args.plain:
-        bench_plain_ensemble(args, descs, label)
-        return
-    chunk = _lib.chunk_steps()
-    warm = -(-args.warmup // chunk) * chunk
-    K = args.steps
-    runs = []
-    for p in descs:
-        p = dict(p, nt=warm + K)
-        s, up, lo = configs.realise(p)
-        runs.append((p, s, materials.fg_mask(s, up, lo)))
-    scheme, prec = runs[0][1].scheme, runs[0][1].precision
-    variant, xchunk, grid, tuned = choose_launch(args, [(s, fg) for _, s, fg in runs])
-    b = Batch(scheme, prec, max_slots=len(runs))
-    b.set_launch(xchunk, grid, variant)
-    variant, xchunk, grid = b.launch()
-    for p, s, fg in runs:
-        b.add(s, fg)
-    b.advance(warm // chunk)
-    b.sync()
-    nch = -(-K // chunk)
-    t0 = time.perf_counter()
-    b.advance(nch)
-    b.sync()
-    wall = time.perf_counter() - t0
-    cells = sum(s.cells for _, s, _ in runs)
-    value = cells * K / wall / 1e6
-
-    # Per-kernel durations with HIP events on the batch stream (a second batch of
-    # the same runs, so the timed run above is untouched).
-    pb = Batch(scheme, prec, max_slots=len(runs))
-    pb.set_launch(xchunk, grid, variant)
-    for p, s, fg in runs:
-        pb.add(s, fg)
-    import ctypes as C
-    out = (C.c_double * 4)()
-    _lib.check(_lib.lib().oidt_batch_profile(pb.h, int(args.profile_chunks), out))
-    ms_e, ms_h, ms_dft, ms_adv = list(out)
-    pb.close()
-    es = 4 if prec == np.float32 else 8
-    be, bh = algorithmic_bytes(scheme, es)
-    stem = {'tm': 'k_tmz', 'te': 'k_tez', '3d': 'k_3d'}[scheme]
-    if variant == 3:
-        kname, ms_k, bytes_k = (stem + '_step', ms_e, (be + bh) * cells)
-    elif ms_e >= ms_h:
-        kname, ms_k, bytes_k = (stem + ('_e2' if variant == 2 else '_e'), ms_e, be * cells)
-    else:
-        kname, ms_k, bytes_k = (stem + ('_h2' if variant == 2 else '_h'), ms_h, bh * cells)
-    achieved = bytes_k / (ms_k * 1e-3) / 1e9
-    traffic, traffic_src = committed_traffic(kname, args.workload, variant, xchunk)
-    roof = {'bound': 'hbm', 'kernel': kname, 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
-            'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
-            'traffic_source': traffic_src,
-            'bytes_per_launch': bytes_k, 'ms_per_launch': round(ms_k, 5),
-            'ms_e': round(ms_e, 5), 'ms_h': round(ms_h, 5), 'ms_dft': round(ms_dft, 5), 'ms_advance': round(ms_adv, 5)}
-    line = {
-        'metric': 'Mcell-updates/s', 'value': round(value, 1), 'unit': 'Mcell-updates/s', 'n_gpus': 1,
-        'steps': K, 'warmup': warm, 'ms_per_step': round(wall / K * 1e3, 5), 'higher_is_better': True,
-        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32' if es == 4 else 'f64', 'data': 'synthetic',
-        'config': {'workload': label, 'runs': len(runs), 'cells_per_step': cells,
-                   'steps_per_run': K, 'xchunk': xchunk, 'variant': variant, 'grid_cap': grid,
-                   'parallelism': 'single GPU',
-                   'autotune': tuned},
-        'hbm_roofline_frac_step': round(value * 1e6 * (be + bh) / (HBM_PEAK_GBS * 1e9), 4),
-        'algorithmic_bytes_per_cell_update': be + bh,
-        'roofline': roof,
-    }
-    b.close()
-    if not args.no_cpu:
-        # CPU leg: the oracle on the first run's grid, bounded sample.
-        p0 = dict(descs[0])
-        rate, secs, nsteps = cpu_baseline(p0, args.cpu_seconds)
-        line['cpu_baseline'] = {'value': round(rate, 3), 'unit': 'Mcell-updates/s', 'cores': 1, 'kind': 'port',
-                                'sample': '%d steps (after 1 warm-up) of the FP64 NumPy oracle on the %dx%dx%d grid '
-                                          'of the first run, %.1f s' % (nsteps, p0['nx'], p0['ny'],
-                                                                         p0.get('nz', 1), secs)}
+    line = bench_ensemble(args, descs, label)
+    if args.workload == 'c3' and not args.no_extras:
+        # the other batched configurations, each at its own full length
+        line['c2'] = extra_ensemble(args, 'c2', 5000)
+        line['c4'] = bench_serving(args)
     print(json.dumps(line), flush=True)
 
 

@@ -209,6 +209,13 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 /* Enqueue nchunks chunks for every resident unfinished run.  Asynchronous. */
 int oidt_batch_advance(oidt_batch* b, int32_t nchunks);
 
+/* Build the launch lists and capture the chunk graph of the batch's current
+ * composition without enqueuing any step (the first advance after an add,
+ * remove or launch change does this itself).  Lets a caller keep graph
+ * capture and list uploads out of a timed or latency-critical region.
+ * No counterpart in the reference. */
+int oidt_batch_prepare(oidt_batch* b);
+
 int oidt_batch_sync(oidt_batch* b);
 
 /* Non-blocking progress: steps completed by the GPU for a slot (finished

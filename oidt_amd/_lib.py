@@ -68,6 +68,7 @@ SIGNATURES = {
     'oidt_batch_create': (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     'oidt_batch_add': (C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(RunInputs), C.POINTER(C.c_int32)]),
     'oidt_batch_advance': (C.c_int, [C.c_void_p, C.c_int32]),
+    'oidt_batch_prepare': (C.c_int, [C.c_void_p]),
     'oidt_batch_sync': (C.c_int, [C.c_void_p]),
     'oidt_batch_poll': (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     'oidt_batch_progress': (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

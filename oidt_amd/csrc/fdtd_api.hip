@@ -967,6 +967,14 @@ int oidt_batch_advance(oidt_batch* b, int32_t nchunks) {
   return OIDT_OK;
 }
 
+int oidt_batch_prepare(oidt_batch* b) {
+  if (!b) return fail(OIDT_EINVAL, "null batch");
+  if (!b->dirty) return OIDT_OK;
+  int rc;
+  if ((rc = rebuild_lists(b)) != OIDT_OK) return rc;
+  return capture_graph(b);
+}
+
 int oidt_batch_sync(oidt_batch* b) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
   HIP_TRY(hipStreamSynchronize(b->stream));

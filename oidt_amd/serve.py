@@ -44,6 +44,7 @@ class Request:
     handle: Any = field(default=None, compare=False, repr=False)
     result: Optional[dict] = field(default=None, compare=False, repr=False)
     error: Optional[str] = field(default=None, compare=False)
+    waited_for_memory: bool = field(default=False, compare=False)
 
     @property
     def key(self):
@@ -134,6 +135,7 @@ class Server:
         self.resident_bytes = 0
         self.peak_bytes = 0
         self.peak_resident = 0
+        self.deferred_for_memory = 0   # requests that waited at the head of the queue for HBM
 
     def run(self, requests, t0=None):
         """Serve ``requests`` (arrival times relative to the start); returns them
@@ -161,7 +163,12 @@ class Server:
                 progressed = True
             while waiting:
                 r = waiting[0]
-                if self.resident_bytes + r.footprint > self.budget or not self.backend.has_slot(r.key):
+                if self.resident_bytes + r.footprint > self.budget:
+                    if not r.waited_for_memory:
+                        r.waited_for_memory = True
+                        self.deferred_for_memory += 1
+                    break
+                if not self.backend.has_slot(r.key):
                     break
                 waiting.pop(0)
                 r.handle = self.backend.add(r.key, r.setup, r.fg)

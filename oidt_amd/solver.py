@@ -285,6 +285,11 @@ class Batch:
     def advance(self, nchunks):
         check(lib().oidt_batch_advance(self.h, int(nchunks)))
 
+    def prepare(self):
+        """Build the launch lists and capture the chunk graph now (the next
+        advance then only replays it)."""
+        check(lib().oidt_batch_prepare(self.h))
+
     def sync(self):
         check(lib().oidt_batch_sync(self.h))
 
