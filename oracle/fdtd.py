@@ -39,6 +39,37 @@ class _Base:
         self.w = 2 * np.pi * np.asarray(c.sparam_freqs, f)
         self.nf = len(self.w)
         self.n = 0
+        # The sign with which each psi array corrects its field, and the
+        # grading order of each slab side, follow the reference kernels.  Both
+        # are instance data so that the pins in tests/ can plant a mistake
+        # (a flipped sign, one side graded in the other side's order) and
+        # show that they catch it.
+        self.psi_sign = dict(self.PSI_SIGN)
+        self.psi_flip = set()
+
+    PSI_SIGN = {}
+
+    def grading(self, name, side, kind):
+        """(b, c) per slab depth for one side of psi array ``name``.  E psi,
+        depth d in [0, N): grading index N-1-d on the low side, d on the high
+        side (update_pml_e*, fdtd_funcs.py:80-100).  H psi, depth d in [1, N):
+        N-d low, d high (update_pml_h*, :104-124)."""
+        N = self.N
+        low = (side == 0) != ((name, side) in self.psi_flip)
+        if kind == 'e':
+            d = np.arange(N)
+            g = N - 1 - d if low else d
+            return self.be[g], self.ce[g]
+        d = np.arange(1, N)
+        g = N - d if low else d
+        return self.bh[g], self.ch[g]
+
+    def correct(self, F, idx, t, name):
+        """F[idx] += t or -= t, by the sign of psi array ``name``."""
+        if self.psi_sign[name] > 0:
+            F[idx] += t
+        else:
+            F[idx] -= t
 
     def phasors(self, n):
         # map_sfft_coeffs (fdtd_funcs.py:230-237): E at n*dt, H at (n+0.5)*dt
@@ -52,6 +83,11 @@ class _Base:
             self.step(self.n)
             self.n += 1
         return self
+
+    def step(self, n):
+        """One time step: the field updates in launch order, then the port DFT."""
+        self.update(n)
+        self.accumulate(n)
 
 
 class TMz(_Base):
@@ -76,7 +112,9 @@ class TMz(_Base):
             return c.sx, c.sx + 1, syb, sye
         return c.nx - c.sx - 1, c.nx - c.sx, syb, sye
 
-    def step(self, n):
+    PSI_SIGN = {'ezx': +1, 'ezy': -1, 'hxy': -1, 'hyx': +1}
+
+    def update(self, n):
         c, N, dx, db = self.c, self.N, self.dx, self.db
         Ez, Hx, Hy = self.Ez, self.Hx, self.Hy
         (cbz,) = self.cb
@@ -89,44 +127,44 @@ class TMz(_Base):
         x0, x1, y0, y1 = self.source_box()
         Ez[max(x0, 0):x1, max(y0, 0):y1] -= self.J[n]
         d = np.arange(N)
-        be_lo, ce_lo = self.be[N - 1 - d], self.ce[N - 1 - d]
-        be_hi, ce_hi = self.be[d], self.ce[d]
         # update_pml_ez_xinc
-        lo, hi = ib + d, ie - N + d
-        self.p_ez_xlo[:, J] = be_lo[:, None] * self.p_ez_xlo[:, J] + ce_lo[:, None] * (Hy[lo + 1, J] - Hy[lo, J]) / dx
-        Ez[lo, J] += cbz[lo, J] * self.p_ez_xlo[:, J]
-        self.p_ez_xhi[:, J] = be_hi[:, None] * self.p_ez_xhi[:, J] + ce_hi[:, None] * (Hy[hi + 1, J] - Hy[hi, J]) / dx
-        Ez[hi, J] += cbz[hi, J] * self.p_ez_xhi[:, J]
+        for side, cells, p in ((0, ib + d, self.p_ez_xlo), (1, ie - N + d, self.p_ez_xhi)):
+            b_, c_ = self.grading('ezx', side, 'e')
+            p[:, J] = b_[:, None] * p[:, J] + c_[:, None] * (Hy[cells + 1, J] - Hy[cells, J]) / dx
+            self.correct(Ez, (cells, J), cbz[cells, J] * p[:, J], 'ezx')
         # update_pml_ez_yinc
-        lo, hi = jb + d, je - N + d
-        self.p_ez_ylo[I, :] = be_lo[None, :] * self.p_ez_ylo[I, :] + ce_lo[None, :] * (Hx[I, lo + 1] - Hx[I, lo]) / dx
-        Ez[I, lo] -= cbz[I, lo] * self.p_ez_ylo[I, :]
-        self.p_ez_yhi[I, :] = be_hi[None, :] * self.p_ez_yhi[I, :] + ce_hi[None, :] * (Hx[I, hi + 1] - Hx[I, hi]) / dx
-        Ez[I, hi] -= cbz[I, hi] * self.p_ez_yhi[I, :]
+        for side, cells, p in ((0, jb + d, self.p_ez_ylo), (1, je - N + d, self.p_ez_yhi)):
+            b_, c_ = self.grading('ezy', side, 'e')
+            p[I, :] = b_[None, :] * p[I, :] + c_[None, :] * (Hx[I, cells + 1] - Hx[I, cells]) / dx
+            self.correct(Ez, (I, cells), cbz[I, cells] * p[I, :], 'ezy')
         # update_hx, update_hy
         Hx[I, jb + 1:je] += db * (Ez[I, jb:je - 1] - Ez[I, jb + 1:je]) / self.kyh[None, jb + 1:je]
         Hy[ib + 1:ie, J] += db * (Ez[ib + 1:ie, J] - Ez[ib:ie - 1, J]) / self.kxh[ib + 1:ie, None]
         dd = np.arange(1, N)
-        bh_lo, ch_lo = self.bh[N - dd], self.ch[N - dd]
-        bh_hi, ch_hi = self.bh[dd], self.ch[dd]
         # update_pml_hx_yinc
-        lo, hi = jb + dd, je - N + dd
-        self.p_hx_ylo[I, 1:] = bh_lo[None, :] * self.p_hx_ylo[I, 1:] + ch_lo[None, :] * (Ez[I, lo] - Ez[I, lo - 1]) / dx
-        Hx[I, lo] -= db * self.p_hx_ylo[I, 1:]
-        self.p_hx_yhi[I, 1:] = bh_hi[None, :] * self.p_hx_yhi[I, 1:] + ch_hi[None, :] * (Ez[I, hi] - Ez[I, hi - 1]) / dx
-        Hx[I, hi] -= db * self.p_hx_yhi[I, 1:]
+        for side, cells, p in ((0, jb + dd, self.p_hx_ylo), (1, je - N + dd, self.p_hx_yhi)):
+            b_, c_ = self.grading('hxy', side, 'h')
+            p[I, 1:] = b_[None, :] * p[I, 1:] + c_[None, :] * (Ez[I, cells] - Ez[I, cells - 1]) / dx
+            self.correct(Hx, (I, cells), db * p[I, 1:], 'hxy')
         # update_pml_hy_xinc
-        lo, hi = ib + dd, ie - N + dd
-        self.p_hy_xlo[1:, J] = bh_lo[:, None] * self.p_hy_xlo[1:, J] + ch_lo[:, None] * (Ez[lo, J] - Ez[lo - 1, J]) / dx
-        Hy[lo, J] += db * self.p_hy_xlo[1:, J]
-        self.p_hy_xhi[1:, J] = bh_hi[:, None] * self.p_hy_xhi[1:, J] + ch_hi[:, None] * (Ez[hi, J] - Ez[hi - 1, J]) / dx
-        Hy[hi, J] += db * self.p_hy_xhi[1:, J]
-        # simul_fft_efield / simul_fft_hfield at both ports, window [N, ny-N)
+        for side, cells, p in ((0, ib + dd, self.p_hy_xlo), (1, ie - N + dd, self.p_hy_xhi)):
+            b_, c_ = self.grading('hyx', side, 'h')
+            p[1:, J] = b_[:, None] * p[1:, J] + c_[:, None] * (Ez[cells, J] - Ez[cells - 1, J]) / dx
+            self.correct(Hy, (cells, J), db * p[1:, J], 'hyx')
+
+    def port_samples(self, ic):
+        """simul_fft_efield / simul_fft_hfield samples at port column ic over
+        the window [N, ny-N): Ez, and Hy averaged over the two columns ic,
+        ic+1 (fdtd_funcs.py:128-144)."""
+        W = slice(self.N, self.c.ny - self.N)
+        return {'e': self.Ez[ic, W], 'h': 0.5 * (self.Hy[ic, W] + self.Hy[ic + 1, W])}
+
+    def accumulate(self, n):
         cE, sE, cH, sH = self.phasors(n)
-        W = slice(N, c.ny - N)
+        W = slice(self.N, self.c.ny - self.N)
         for p, ic in zip('12', self.ports):
-            e = Ez[ic, W]
-            h = 0.5 * (Hy[ic, W] + Hy[ic + 1, W])
+            smp = self.port_samples(ic)
+            e, h = smp['e'], smp['h']
             a = self.acc
             a['e' + p + 'r'][:, W] += e[None, :] * cE[:, None]
             a['e' + p + 'i'][:, W] -= e[None, :] * sE[:, None]
@@ -164,7 +202,9 @@ class TEz(_Base):
             return c.sx, c.sx + 1, syb, sye
         return c.nx - c.sx - 1, c.nx - c.sx, syb, sye
 
-    def step(self, n):
+    PSI_SIGN = {'exy': +1, 'eyx': -1, 'hzx': -1, 'hzy': +1}
+
+    def update(self, n):
         c, N, dx, db = self.c, self.N, self.dx, self.db
         Ex, Ey, Hz = self.Ex, self.Ey, self.Hz
         cbx, cby = self.cb
@@ -175,20 +215,16 @@ class TEz(_Base):
         # update_ey: i in [ib,ie), j in (jb,je)
         Ey[I, J1] += cby[I, J1] * (Hz[I, J1] - Hz[ib + 1:ie + 1, J1]) / self.kxe[I, None]
         d = np.arange(N)
-        be_lo, ce_lo = self.be[N - 1 - d], self.ce[N - 1 - d]
-        be_hi, ce_hi = self.be[d], self.ce[d]
         # update_pml_ex_yinc: i in (ib,ie)
-        lo, hi = jb + d, je - N + d
-        self.p_ex_ylo[I1, :] = be_lo[None, :] * self.p_ex_ylo[I1, :] + ce_lo[None, :] * (Hz[I1, lo + 1] - Hz[I1, lo]) / dx
-        Ex[I1, lo] += cbx[I1, lo] * self.p_ex_ylo[I1, :]
-        self.p_ex_yhi[I1, :] = be_hi[None, :] * self.p_ex_yhi[I1, :] + ce_hi[None, :] * (Hz[I1, hi + 1] - Hz[I1, hi]) / dx
-        Ex[I1, hi] += cbx[I1, hi] * self.p_ex_yhi[I1, :]
+        for side, cells, p in ((0, jb + d, self.p_ex_ylo), (1, je - N + d, self.p_ex_yhi)):
+            b_, c_ = self.grading('exy', side, 'e')
+            p[I1, :] = b_[None, :] * p[I1, :] + c_[None, :] * (Hz[I1, cells + 1] - Hz[I1, cells]) / dx
+            self.correct(Ex, (I1, cells), cbx[I1, cells] * p[I1, :], 'exy')
         # update_pml_ey_xinc: j in (jb,je)
-        lo, hi = ib + d, ie - N + d
-        self.p_ey_xlo[:, J1] = be_lo[:, None] * self.p_ey_xlo[:, J1] + ce_lo[:, None] * (Hz[lo + 1, J1] - Hz[lo, J1]) / dx
-        Ey[lo, J1] -= cby[lo, J1] * self.p_ey_xlo[:, J1]
-        self.p_ey_xhi[:, J1] = be_hi[:, None] * self.p_ey_xhi[:, J1] + ce_hi[:, None] * (Hz[hi + 1, J1] - Hz[hi, J1]) / dx
-        Ey[hi, J1] -= cby[hi, J1] * self.p_ey_xhi[:, J1]
+        for side, cells, p in ((0, ib + d, self.p_ey_xlo), (1, ie - N + d, self.p_ey_xhi)):
+            b_, c_ = self.grading('eyx', side, 'e')
+            p[:, J1] = b_[:, None] * p[:, J1] + c_[:, None] * (Hz[cells + 1, J1] - Hz[cells, J1]) / dx
+            self.correct(Ey, (cells, J1), cby[cells, J1] * p[:, J1], 'eyx')
         # update_hz: i in (ib,ie), j in (jb,je)
         Hz[I1, J1] += db * ((Ex[I1, J1] - Ex[I1, jb:je - 1]) / self.kyh[None, J1]
                             - (Ey[I1, J1] - Ey[ib:ie - 1, J1]) / self.kxh[I1, None])
@@ -196,27 +232,33 @@ class TEz(_Base):
         x0, x1, y0, y1 = self.source_box()
         Hz[max(x0, 0):x1, max(y0, 0):y1] -= self.J[n]
         dd = np.arange(1, N)
-        bh_lo, ch_lo = self.bh[N - dd], self.ch[N - dd]
-        bh_hi, ch_hi = self.bh[dd], self.ch[dd]
         # update_pml_hz_xinc: j in (jb,je)
-        lo, hi = ib + dd, ie - N + dd
-        self.p_hz_xlo[1:, J1] = bh_lo[:, None] * self.p_hz_xlo[1:, J1] + ch_lo[:, None] * (Ey[lo, J1] - Ey[lo - 1, J1]) / dx
-        Hz[lo, J1] -= db * self.p_hz_xlo[1:, J1]
-        self.p_hz_xhi[1:, J1] = bh_hi[:, None] * self.p_hz_xhi[1:, J1] + ch_hi[:, None] * (Ey[hi, J1] - Ey[hi - 1, J1]) / dx
-        Hz[hi, J1] -= db * self.p_hz_xhi[1:, J1]
+        for side, cells, p in ((0, ib + dd, self.p_hz_xlo), (1, ie - N + dd, self.p_hz_xhi)):
+            b_, c_ = self.grading('hzx', side, 'h')
+            p[1:, J1] = b_[:, None] * p[1:, J1] + c_[:, None] * (Ey[cells, J1] - Ey[cells - 1, J1]) / dx
+            self.correct(Hz, (cells, J1), db * p[1:, J1], 'hzx')
         # update_pml_hz_yinc: i in (ib,ie)
-        lo, hi = jb + dd, je - N + dd
-        self.p_hz_ylo[I1, 1:] = bh_lo[None, :] * self.p_hz_ylo[I1, 1:] + ch_lo[None, :] * (Ex[I1, lo] - Ex[I1, lo - 1]) / dx
-        Hz[I1, lo] += db * self.p_hz_ylo[I1, 1:]
-        self.p_hz_yhi[I1, 1:] = bh_hi[None, :] * self.p_hz_yhi[I1, 1:] + ch_hi[None, :] * (Ex[I1, hi] - Ex[I1, hi - 1]) / dx
-        Hz[I1, hi] += db * self.p_hz_yhi[I1, 1:]
-        # DFT: Ey averaged over j, Hz over the 2x2 cell corners
+        for side, cells, p in ((0, jb + dd, self.p_hz_ylo), (1, je - N + dd, self.p_hz_yhi)):
+            b_, c_ = self.grading('hzy', side, 'h')
+            p[I1, 1:] = b_[None, :] * p[I1, 1:] + c_[None, :] * (Ex[I1, cells] - Ex[I1, cells - 1]) / dx
+            self.correct(Hz, (I1, cells), db * p[I1, 1:], 'hzy')
+
+    def port_samples(self, ic):
+        """Port samples at column ic over the window [N, ny-N): Ey averaged
+        over rows j, j+1; Hz over the 2x2 corners of columns ic, ic+1 and rows
+        j, j+1 (FDTD_2D_TE/.../fdtd_funcs.py:128-144)."""
+        N, ny = self.N, self.c.ny
+        W, W1 = slice(N, ny - N), slice(N + 1, ny - N + 1)
+        Ey, Hz = self.Ey, self.Hz
+        return {'e': 0.5 * (Ey[ic, W] + Ey[ic, W1]),
+                'h': 0.25 * (Hz[ic, W] + Hz[ic + 1, W] + Hz[ic, W1] + Hz[ic + 1, W1])}
+
+    def accumulate(self, n):
         cE, sE, cH, sH = self.phasors(n)
-        W = slice(N, c.ny - N)
-        W1 = slice(N + 1, c.ny - N + 1)
+        W = slice(self.N, self.c.ny - self.N)
         for p, ic in zip('12', self.ports):
-            e = 0.5 * (Ey[ic, W] + Ey[ic, W1])
-            h = 0.25 * (Hz[ic, W] + Hz[ic + 1, W] + Hz[ic, W1] + Hz[ic + 1, W1])
+            smp = self.port_samples(ic)
+            e, h = smp['e'], smp['h']
             a = self.acc
             a['e' + p + 'r'][:, W] += e[None, :] * cE[:, None]
             a['e' + p + 'i'][:, W] -= e[None, :] * sE[:, None]
@@ -262,7 +304,10 @@ class Yee3D(_Base):
             z0, z1 = c.nz - c.sz - 1, c.nz - c.sz
         return x0, x1, y0, y1, z0, z1
 
-    def step(self, n):
+    PSI_SIGN = {'exy': +1, 'exz': -1, 'eyx': -1, 'eyz': +1, 'ezx': +1, 'ezy': -1,
+                'hxy': -1, 'hxz': +1, 'hyx': +1, 'hyz': -1, 'hzx': -1, 'hzy': +1}
+
+    def update(self, n):
         c, N, dx, db = self.c, self.N, self.dx, self.db
         Ex, Ey, Ez, Hx, Hy, Hz = self.Ex, self.Ey, self.Ez, self.Hx, self.Hy, self.Hz
         cbx, cby, cbz = self.cb
@@ -289,39 +334,44 @@ class Yee3D(_Base):
         F = Ey if c.source_condition == 'TE' else Hy
         F[max(x0, 0):x1, max(y0, 0):y1, max(z0, 0):z1] -= self.J[n]
         d = np.arange(N)
-        bel, cel = self.be[N - 1 - d], self.ce[N - 1 - d]
-        beh, ceh = self.be[d], self.ce[d]
         ps = self.psi
-        for side, cells, b_, c_ in ((0, jb + d, bel, cel), (1, je - N + d, beh, ceh)):
+        # E CPML: each kernel's low side (cells b + d), then its high side (cells e - N + d)
+        for side, cells in ((0, jb + d), (1, je - N + d)):
             # ex_y (+), i (ib,ie), k [kb,ke)
+            b_, c_ = self.grading('exy', side, 'e')
             p = ps['exy'][side]
             p[I1, :, K] = b_[None, :, None] * p[I1, :, K] + c_[None, :, None] * (Hz[I1, cells + 1, K] - Hz[I1, cells, K]) / dx
-            Ex[I1, cells, K] += cbx[I1, cells, K] * p[I1, :, K]
-        for side, cells, b_, c_ in ((0, kb + d, bel, cel), (1, ke - N + d, beh, ceh)):
+            self.correct(Ex, (I1, cells, K), cbx[I1, cells, K] * p[I1, :, K], 'exy')
+        for side, cells in ((0, kb + d), (1, ke - N + d)):
             # ex_z (-), i (ib,ie), j [jb,je)
+            b_, c_ = self.grading('exz', side, 'e')
             p = ps['exz'][side]
             p[I1, J, :] = b_[None, None, :] * p[I1, J, :] + c_[None, None, :] * (Hy[I1, J, cells + 1] - Hy[I1, J, cells]) / dx
-            Ex[I1, J, cells] -= cbx[I1, J, cells] * p[I1, J, :]
-        for side, cells, b_, c_ in ((0, ib + d, bel, cel), (1, ie - N + d, beh, ceh)):
+            self.correct(Ex, (I1, J, cells), cbx[I1, J, cells] * p[I1, J, :], 'exz')
+        for side, cells in ((0, ib + d), (1, ie - N + d)):
             # ey_x (-), j (jb,je), k [kb,ke)
+            b_, c_ = self.grading('eyx', side, 'e')
             p = ps['eyx'][side]
             p[:, J1, K] = b_[:, None, None] * p[:, J1, K] + c_[:, None, None] * (Hz[cells + 1, J1, K] - Hz[cells, J1, K]) / dx
-            Ey[cells, J1, K] -= cby[cells, J1, K] * p[:, J1, K]
-        for side, cells, b_, c_ in ((0, kb + d, bel, cel), (1, ke - N + d, beh, ceh)):
+            self.correct(Ey, (cells, J1, K), cby[cells, J1, K] * p[:, J1, K], 'eyx')
+        for side, cells in ((0, kb + d), (1, ke - N + d)):
             # ey_z (+), i [ib,ie), j (jb,je)
+            b_, c_ = self.grading('eyz', side, 'e')
             p = ps['eyz'][side]
             p[I, J1, :] = b_[None, None, :] * p[I, J1, :] + c_[None, None, :] * (Hx[I, J1, cells + 1] - Hx[I, J1, cells]) / dx
-            Ey[I, J1, cells] += cby[I, J1, cells] * p[I, J1, :]
-        for side, cells, b_, c_ in ((0, ib + d, bel, cel), (1, ie - N + d, beh, ceh)):
+            self.correct(Ey, (I, J1, cells), cby[I, J1, cells] * p[I, J1, :], 'eyz')
+        for side, cells in ((0, ib + d), (1, ie - N + d)):
             # ez_x (+), j [jb,je), k (kb,ke)
+            b_, c_ = self.grading('ezx', side, 'e')
             p = ps['ezx'][side]
             p[:, J, K1] = b_[:, None, None] * p[:, J, K1] + c_[:, None, None] * (Hy[cells + 1, J, K1] - Hy[cells, J, K1]) / dx
-            Ez[cells, J, K1] += cbz[cells, J, K1] * p[:, J, K1]
-        for side, cells, b_, c_ in ((0, jb + d, bel, cel), (1, je - N + d, beh, ceh)):
+            self.correct(Ez, (cells, J, K1), cbz[cells, J, K1] * p[:, J, K1], 'ezx')
+        for side, cells in ((0, jb + d), (1, je - N + d)):
             # ez_y (-), i [ib,ie), k (kb,ke)
+            b_, c_ = self.grading('ezy', side, 'e')
             p = ps['ezy'][side]
             p[I, :, K1] = b_[None, :, None] * p[I, :, K1] + c_[None, :, None] * (Hx[I, cells + 1, K1] - Hx[I, cells, K1]) / dx
-            Ez[I, cells, K1] -= cbz[I, cells, K1] * p[I, :, K1]
+            self.correct(Ez, (I, cells, K1), cbz[I, cells, K1] * p[I, :, K1], 'ezy')
         # update_hx: i [ib,ie), j (jb,je), k (kb,ke)
         s = (I, J1, K1)
         Hx[s] += db * ((Ey[s] - Ey[I, J1, kb:ke - 1]) / kzh[None, None, K1]
@@ -335,53 +385,69 @@ class Yee3D(_Base):
         Hz[s] += db * ((Ex[s] - Ex[I1, jb:je - 1, K]) / kyh[None, J1, None]
                        - (Ey[s] - Ey[ib:ie - 1, J1, K]) / kxh[I1, None, None])
         dd = np.arange(1, N)
-        bhl, chl = self.bh[N - dd], self.ch[N - dd]
-        bhh, chh = self.bh[dd], self.ch[dd]
-        for side, cells, b_, c_ in ((0, jb + dd, bhl, chl), (1, je - N + dd, bhh, chh)):
+        for side, cells in ((0, jb + dd), (1, je - N + dd)):
             # hx_y (-), i [ib,ie), k (kb,ke)
+            b_, c_ = self.grading('hxy', side, 'h')
             p = ps['hxy'][side]
             p[I, 1:, K1] = b_[None, :, None] * p[I, 1:, K1] + c_[None, :, None] * (Ez[I, cells, K1] - Ez[I, cells - 1, K1]) / dx
-            Hx[I, cells, K1] -= db * p[I, 1:, K1]
-        for side, cells, b_, c_ in ((0, kb + dd, bhl, chl), (1, ke - N + dd, bhh, chh)):
+            self.correct(Hx, (I, cells, K1), db * p[I, 1:, K1], 'hxy')
+        for side, cells in ((0, kb + dd), (1, ke - N + dd)):
             # hx_z (+), i [ib,ie), j (jb,je)
+            b_, c_ = self.grading('hxz', side, 'h')
             p = ps['hxz'][side]
             p[I, J1, 1:] = b_[None, None, :] * p[I, J1, 1:] + c_[None, None, :] * (Ey[I, J1, cells] - Ey[I, J1, cells - 1]) / dx
-            Hx[I, J1, cells] += db * p[I, J1, 1:]
-        for side, cells, b_, c_ in ((0, ib + dd, bhl, chl), (1, ie - N + dd, bhh, chh)):
+            self.correct(Hx, (I, J1, cells), db * p[I, J1, 1:], 'hxz')
+        for side, cells in ((0, ib + dd), (1, ie - N + dd)):
             # hy_x (+), j [jb,je), k (kb,ke)
+            b_, c_ = self.grading('hyx', side, 'h')
             p = ps['hyx'][side]
             p[1:, J, K1] = b_[:, None, None] * p[1:, J, K1] + c_[:, None, None] * (Ez[cells, J, K1] - Ez[cells - 1, J, K1]) / dx
-            Hy[cells, J, K1] += db * p[1:, J, K1]
-        for side, cells, b_, c_ in ((0, kb + dd, bhl, chl), (1, ke - N + dd, bhh, chh)):
+            self.correct(Hy, (cells, J, K1), db * p[1:, J, K1], 'hyx')
+        for side, cells in ((0, kb + dd), (1, ke - N + dd)):
             # hy_z (-), i (ib,ie), j [jb,je)
+            b_, c_ = self.grading('hyz', side, 'h')
             p = ps['hyz'][side]
             p[I1, J, 1:] = b_[None, None, :] * p[I1, J, 1:] + c_[None, None, :] * (Ex[I1, J, cells] - Ex[I1, J, cells - 1]) / dx
-            Hy[I1, J, cells] -= db * p[I1, J, 1:]
-        for side, cells, b_, c_ in ((0, ib + dd, bhl, chl), (1, ie - N + dd, bhh, chh)):
+            self.correct(Hy, (I1, J, cells), db * p[I1, J, 1:], 'hyz')
+        for side, cells in ((0, ib + dd), (1, ie - N + dd)):
             # hz_x (-), j (jb,je), k [kb,ke)
+            b_, c_ = self.grading('hzx', side, 'h')
             p = ps['hzx'][side]
             p[1:, J1, K] = b_[:, None, None] * p[1:, J1, K] + c_[:, None, None] * (Ey[cells, J1, K] - Ey[cells - 1, J1, K]) / dx
-            Hz[cells, J1, K] -= db * p[1:, J1, K]
-        for side, cells, b_, c_ in ((0, jb + dd, bhl, chl), (1, je - N + dd, bhh, chh)):
+            self.correct(Hz, (cells, J1, K), db * p[1:, J1, K], 'hzx')
+        for side, cells in ((0, jb + dd), (1, je - N + dd)):
             # hz_y (+), i (ib,ie), k [kb,ke)
+            b_, c_ = self.grading('hzy', side, 'h')
             p = ps['hzy'][side]
             p[I1, 1:, K] = b_[None, :, None] * p[I1, 1:, K] + c_[None, :, None] * (Ex[I1, cells, K] - Ex[I1, cells - 1, K]) / dx
-            Hz[I1, cells, K] += db * p[I1, 1:, K]
-        # DFT at the two port planes, window i in [N, nx-N), j in [N, ny-N)
-        cE, sE, cH, sH = self.phasors(n)
+            self.correct(Hz, (I1, cells, K), db * p[I1, 1:, K], 'hzy')
+
+    def port_samples(self, kc):
+        """Port samples on plane k = kc over the window i in [N, nx-N), j in
+        [N, ny-N), each averaged to the cell corner (i, j, kc): Ex over i,
+        i+1; Ey over j, j+1; Hx over (j, k) pairs; Hy over (i, k) pairs
+        (simul_fft_efield / simul_fft_hfield, fdtd_funcs.py:270-292)."""
+        c, N = self.c, self.N
         wi, wj = slice(N, c.nx - N), slice(N, c.ny - N)
         wi1, wj1 = slice(N + 1, c.nx - N + 1), slice(N + 1, c.ny - N + 1)
+        Ex, Ey, Hx, Hy = self.Ex, self.Ey, self.Hx, self.Hy
+        return {'ex': 0.5 * (Ex[wi, wj, kc] + Ex[wi1, wj, kc]),
+                'ey': 0.5 * (Ey[wi, wj, kc] + Ey[wi, wj1, kc]),
+                'hx': 0.25 * (Hx[wi, wj, kc] + Hx[wi, wj1, kc] + Hx[wi, wj, kc + 1] + Hx[wi, wj1, kc + 1]),
+                'hy': 0.25 * (Hy[wi, wj, kc] + Hy[wi1, wj, kc] + Hy[wi, wj, kc + 1] + Hy[wi1, wj, kc + 1])}
+
+    def accumulate(self, n):
+        c, N = self.c, self.N
+        cE, sE, cH, sH = self.phasors(n)
+        wi, wj = slice(N, c.nx - N), slice(N, c.ny - N)
         eE = (cE - 1j * sE)[:, None, None]
         eH = (cH - 1j * sH)[:, None, None]
         for p, kc in zip((1, 2), self.ports):
-            sx_ = 0.5 * (Ex[wi, wj, kc] + Ex[wi1, wj, kc])
-            sy_ = 0.5 * (Ey[wi, wj, kc] + Ey[wi, wj1, kc])
-            hx_ = 0.25 * (Hx[wi, wj, kc] + Hx[wi, wj1, kc] + Hx[wi, wj, kc + 1] + Hx[wi, wj1, kc + 1])
-            hy_ = 0.25 * (Hy[wi, wj, kc] + Hy[wi1, wj, kc] + Hy[wi, wj, kc + 1] + Hy[wi1, wj, kc + 1])
-            self.acc[('ex', p)][:, wi, wj] += sx_[None] * eE
-            self.acc[('ey', p)][:, wi, wj] += sy_[None] * eE
-            self.acc[('hx', p)][:, wi, wj] += hx_[None] * eH
-            self.acc[('hy', p)][:, wi, wj] += hy_[None] * eH
+            smp = self.port_samples(kc)
+            for k in ('ex', 'ey'):
+                self.acc[(k, p)][:, wi, wj] += smp[k][None] * eE
+            for k in ('hx', 'hy'):
+                self.acc[(k, p)][:, wi, wj] += smp[k][None] * eH
 
     def fields(self):
         return {k.lower(): getattr(self, k) for k in ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')}

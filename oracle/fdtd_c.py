@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY, like the rest of ``oracle/``.  ``Yee3DC`` is the
 NumPy oracle's ``Yee3D`` (same arrays, coefficients and state layout) whose
 time stepping runs in C with OpenMP, for config-scale 3D checks.  Built by
-``oracle.build`` (called from ``__graft_entry__.build``) into
+``oracle.build`` (called from ``__graft_entry__.build``, or on first use) into
 ``oracle/liboracle_fdtd3d.so``.
 """
 import ctypes as C
@@ -37,7 +37,8 @@ def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError('oracle/liboracle_fdtd3d.so is not built: python -m oracle.build')
+            from . import build
+            build.build()
         L = C.CDLL(LIB_PATH)
         L.o3d_run.restype = C.c_int
         L.o3d_run.argtypes = [C.POINTER(_State), C.c_int32, C.c_int32]

