@@ -100,6 +100,7 @@ struct SlotHost {
   int64_t admit_chunk = 0;          // batch chunk counter when admitted
   bool fused = false;               // advanced by the fused step kernel (alternating field sets)
   int nplanes = 0, nq = 0, wcells = 0, pcells = 0, wyn = 0;
+  int nzp = 0;                      // padded z extent (3D)
   std::vector<MonHost> mons;
 };
 
@@ -478,6 +479,7 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   s.w = (OIDT_GLOBAL const double*)(base + L.off_w);
   sh.slot.assign(reinterpret_cast<unsigned char*>(&s), reinterpret_cast<unsigned char*>(&s) + sizeof(s));
   sh.nplanes = L.nplanes; sh.nq = L.nq; sh.wcells = L.wcells; sh.pcells = L.pcells; sh.wyn = L.wy1 - L.wy0;
+  sh.nzp = pd.nz;
   return OIDT_OK;
 }
 
@@ -503,8 +505,22 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
   return OIDT_OK;
 }
 
-// Kernels whose tiles are per wavefront (four tiles per block).
-bool march_tiles(const oidt_batch* b) { return b->variant == 3; }
+// Kernels whose tiles are per wavefront (four tiles per block): the fused
+// 2D steps.
+bool march_tiles(const oidt_batch* b) { return b->variant == 3 && b->scheme != SCHEME_3D; }
+
+// Rows of a fused 3D step block (the halo row included) for a run of padded z
+// extent nzp, or 0 when its LDS rows do not fit (see k_3d_fused).
+template <typename T>
+int fused3d_rows_t(int nzp) {
+  const int zw = (nzp + Fused3d<T>::ZSEG - 1) / Fused3d<T>::ZSEG;
+  const int R = Fused3d<T>::W / zw;
+  if (zw > Fused3d<T>::ZWMAX || R < 2 || Fused3d<T>::lds_cells(zw) > Fused3d<T>::LDS) return 0;
+  return R;
+}
+int fused3d_rows(int precision, int nzp) {
+  return precision == OIDT_F64 ? fused3d_rows_t<double>(nzp) : fused3d_rows_t<float>(nzp);
+}
 
 // Monitor table and the two work lists (after the E and after the H
 // half-step): one item per kMonCells consecutive box cells.
@@ -561,7 +577,13 @@ int rebuild_lists(oidt_batch* b) {
     const SlotHost& sh = b->slots[si];
     if (!sh.used) continue;
     const oidt_run_desc& d = sh.desc;
-    if (d.scheme == SCHEME_3D) {
+    if (d.scheme == SCHEME_3D && b->variant == 3) {
+      // fused step: one block per tile of xc planes x (rows per block - 1)
+      // rows x the whole z extent
+      const int rows = fused3d_rows(b->precision, sh.nzp) - 1;
+      for (int x0 = 0; x0 < d.nx; x0 += xc)
+        for (int y0 = 0; y0 < d.ny; y0 += rows) tiles.push_back(make_int4(si, x0, y0, 0));
+    } else if (d.scheme == SCHEME_3D) {
       for (int x0 = 0; x0 < d.nx; x0 += xc)
         for (int y0 = 0; y0 < d.ny; y0 += kBY)
           for (int z0 = 0; z0 < d.nz; z0 += kBX) tiles.push_back(make_int4(si, x0, y0, z0));
@@ -646,8 +668,9 @@ void launch_fused(oidt_batch* b, int gs, int tstep) {
     hipLaunchKernelGGL((k_tmz_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
   } else if (b->scheme == SCHEME_TEZ) {
     hipLaunchKernelGGL((k_tez_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
-  } else if constexpr (XM <= kFused3dMaxXc) {
-    hipLaunchKernelGGL((k_3d_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+  } else {
+    hipLaunchKernelGGL((k_3d_fused<T>), grd, dim3(64, Fused3d<T>::W), 0, b->stream, sl, b->d_tiles, b->d_counts,
+                       tstep, XM);
   }
 }
 
@@ -885,6 +908,10 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
   if (rc != OIDT_OK) return rc;
   if (desc->scheme != b->scheme || desc->precision != b->precision)
     return fail(OIDT_EINVAL, "run scheme/precision differs from the batch");
+  if (b->variant == 3 && b->scheme == SCHEME_3D) {
+    const int nzp = b->precision == OIDT_F64 ? padded<double>(*desc).nz : padded<float>(*desc).nz;
+    if (fused3d_rows(b->precision, nzp) == 0) return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64)");
+  }
   int si = -1;
   for (int i = 0; i < b->max_slots; ++i)
     if (!b->slots[i].used) { si = i; break; }
@@ -921,8 +948,10 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (!pow2 && !(fused_only && variant == 3))
     return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32 (also 12, 20 or 24 for the fused step)");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
-  if (variant == 3 && b->scheme == SCHEME_3D && (xchunk ? xchunk : b->xchunk) > kFused3dMaxXc)
-    return fail(OIDT_EINVAL, "the fused 3D step takes xchunk 1, 2, 4, 8 or 16");
+  if (variant == 3 && b->scheme == SCHEME_3D)
+    for (const auto& sh : b->slots)
+      if (sh.used && fused3d_rows(b->precision, sh.nzp) == 0)
+        return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64)");
   if (variant == 3 && b->scheme == SCHEME_TMZ_LOSSY)
     return fail(OIDT_EINVAL, "the lossy TMz form has no fused step (variant 1 or 2)");
   if (variant == 3 && b->has_monitors())

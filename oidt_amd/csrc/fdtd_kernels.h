@@ -18,6 +18,8 @@
 // the E-CPML; the host rejects source boxes that the E-CPML stencil reads, so
 // the results are the same.
 #pragma once
+#include <type_traits>
+
 #include "fdtd_slot.h"
 
 namespace oidt {
@@ -2103,504 +2105,560 @@ __global__ __launch_bounds__(kThreads) void k_tez_step(const Slot<T>* __restrict
 }
 
 // ------------------------------------------------------- fused 3D step ---
-// One launch per time step for 3D, reading field/psi set (n & 1) and writing
-// the other, so no tile races another.  Each wavefront owns a tile of 64
-// z-cells (lanes) x 4 y-rows x XM x-planes and walks it as one sequence of
-// cell items: first the row below the tile (j0-1, all XM planes: new E only,
-// not stored), then for each of its 4 rows the plane below the tile (i0-1,
-// not stored) followed by the row's XM planes.  For the H update at (i, j, k)
-//   * the j-1 neighbour (new Ex, Ez) is the previous row's result at the same
-//     plane, kept per plane in the wave's own LDS row buffer,
-//   * the i-1 neighbour (new Ey, Ez) is the previous item of the same row,
-//   * the k-1 neighbour (new Ex, Ey) comes from the next-lower lane; lane 0
-//     takes it from the column below the tile (4 x XM cells, computed once
-//     per tile, one per lane).
-// The loads of the next item are issued before the stores of the current one,
-// and waves never wait for each other.  Halo cells go through the same load
-// and update code as their owners, so they are bit-identical to them.  The
-// run's descriptor fields are copied to registers per tile (View3): stores
-// through the field pointers could otherwise alias them and force reloads.
+// One launch per time step for 3D: both half-steps, reading field/psi set
+// (n & 1) and writing the other, so no block races another.  A block of W
+// waves owns R rows (y) x the whole padded z extent x XM planes (x): each row
+// is covered by ZW = ceil(nz / 128) waves of 64 lanes, a lane holding 2
+// consecutive z-cells (one 8-byte access per field in FP32), so R = W / ZW.
+// Row 0 of the block is the row below the tile (j0 - 1): it computes new E
+// only (not stored), which the first tile row's H update needs; rows 1..R-1
+// are the tile's own.  The block marches along x; per plane i:
+//   1. every wave writes its old H (plane i) to the LDS row buffer (row R:
+//      the row above the tile, Hx and Hz, for the last row's E update);
+//   2. barrier;
+//   3. new E of plane i: j+1 neighbours from the LDS rows, k+1 from the
+//      lane's next cell (the next lane's first cell from LDS), x+1 from the
+//      old H of plane i+1 held in registers;
+//   4. new E to the LDS row buffer, then the loads of plane i+1 are issued;
+//   5. barrier;
+//   6. new H of the tile rows: j-1 and k-1 neighbours from the LDS rows and
+//      the lane's previous cell, x-1 from the previous plane's new E in
+//      registers; E and H of the tile rows stored to the other set.
+// Before the first plane the block computes new E of plane i0 - 1 (not
+// stored) the same way.  Halo cells (row j0 - 1, plane i0 - 1) go through the
+// same per-cell code as their owners, so they are bit-identical to them.
+// The per-cell arithmetic and the update ranges are those of k_3d_e and
+// k_3d_h (reference order: curl, source, psi of the first axis, psi of the
+// second axis; the TM source on Hy at the start of the H half-step).
+// Field traffic per cell and step: each component read once and written
+// once (24 + 24 B in FP32) plus the halo row, the material byte and psi.
 
+// Waves per block of the fused 3D step: 16 in FP32, 8 in FP64; z-cells per
+// lane: 2.  The LDS rows of a block take (6 R + 3) x ZW x 128 cells, at most
+// 61 KB for a padded z extent up to 1024 (FP32) or 512 (FP64).
 template <typename T>
-struct View3 {
-  int nx, ny, nz, N, ib, ie, jb, je, kb, ke, src_field, sx0, sx1, sy0, sy1, sz0, sz1;
-  T db;
-  T cbv[3][kMatMax];
-  const T* ike[3];
-  const T* ikh[3];
-  const T *be, *ceo, *bh, *cho;
-  const uint8_t* mat;
-};
-
-template <typename T>
-__device__ __forceinline__ View3<T> view3(const Slot<T>& s) {
-  View3<T> v;
-  v.nx = s.nx; v.ny = s.ny; v.nz = s.nz; v.N = s.N;
-  v.ib = s.ib; v.ie = s.ie; v.jb = s.jb; v.je = s.je; v.kb = s.kb; v.ke = s.ke;
-  v.src_field = s.src_field;
-  v.sx0 = s.sx0; v.sx1 = s.sx1; v.sy0 = s.sy0; v.sy1 = s.sy1; v.sz0 = s.sz0; v.sz1 = s.sz1;
-  v.db = s.db;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int q = 0; q < kMatMax; ++q) v.cbv[c][q] = s.cbv[c][q];
-    v.ike[c] = s.ike[c];
-    v.ikh[c] = s.ikh[c];
+struct Fused3d {
+  static constexpr int W = sizeof(T) == 4 ? 16 : 8;
+  static constexpr int C = 2;                       // z-cells per lane
+  static constexpr int ZSEG = 64 * C;               // z-cells per wave
+  static constexpr int ZWMAX = sizeof(T) == 4 ? 8 : 4;
+  static constexpr int lds_cells(int zw) { return (6 * (W / zw) + 3) * zw * ZSEG; }
+  static constexpr int max_cells(int zw) {
+    return zw > ZWMAX ? 0 : (lds_cells(zw) > max_cells(zw + 1) ? lds_cells(zw) : max_cells(zw + 1));
   }
-  v.be = s.be; v.ceo = s.ceo; v.bh = s.bh; v.cho = s.cho;
-  v.mat = s.mat;
-  return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T cbsel3(const View3<T>& v, uint8_t m, int c) {
-  const int q = (m >> (2 * c)) & 3;
-  return q == 0 ? v.cbv[c][0] : (q == 1 ? v.cbv[c][1] : (q == 2 ? v.cbv[c][2] : v.cbv[c][3]));
-}
-
-// Where one cell's E components are updated (3D index ranges of
-// fdtd_funcs.py:44-80, FDTD_3D; CPML slabs :82-126).
-struct E3Where {
-  bool bx, by, bz, src;
-  int si, sj, sk, di, dj, dk, ci, cj, ck;
+  static constexpr int LDS = max_cells(1);
 };
 
+// Two consecutive z-cells of one field.
 template <typename T>
-__device__ __forceinline__ E3Where e3_where(const View3<T>& v, int i, int j, int k) {
-  E3Where f;
-  const bool in = i >= 0 && j >= 0 && k >= 0 && i < v.nx && j < v.ny && k < v.nz;
-  const bool iin = in && i >= v.ib && i < v.ie, iin1 = in && i > v.ib && i < v.ie;
-  const bool jin = in && j >= v.jb && j < v.je, jin1 = in && j > v.jb && j < v.je;
-  const bool kin = in && k >= v.kb && k < v.ke, kin1 = in && k > v.kb && k < v.ke;
-  f.bx = iin1 && jin && kin;
-  f.by = iin && jin1 && kin;
-  f.bz = iin && jin && kin1;
-  f.src = in && v.src_field == F_EY && i >= v.sx0 && i < v.sx1 && j >= v.sy0 && j < v.sy1 && k >= v.sz0 &&
-          k < v.sz1;
-  f.di = f.dj = f.dk = f.ci = f.cj = f.ck = 0;
-  f.si = in ? slab_e(i, v.ib, v.ie, v.N, f.di, f.ci) : -1;
-  f.sj = in ? slab_e(j, v.jb, v.je, v.N, f.dj, f.cj) : -1;
-  f.sk = in ? slab_e(k, v.kb, v.ke, v.N, f.dk, f.ck) : -1;
-  return f;
-}
-
-// Everything one cell's new E reads from the old set (indexes clamped into
-// the grid; psi only inside its slab).
-template <typename T>
-struct E3In {
-  T ex, ey, ez, hx, hy, hz, hzj, hyk, hxk, hzi, hyi, hxj;
-  T qxy, qxz, qyx, qyz, qzx, qzy;
-  uint8_t m;
+struct Q4 {
+  T v[2];
 };
 
+typedef float oidt_f32x2 __attribute__((ext_vector_type(2)));
+typedef double oidt_f64x2 __attribute__((ext_vector_type(2)));
 template <typename T>
-__device__ __forceinline__ E3In<T> e3_load(const View3<T>& v, int i, int j, int k, const E3Where& f,
-                                           const T* __restrict__ Ex, const T* __restrict__ Ey,
-                                           const T* __restrict__ Ez, const T* __restrict__ Hx,
-                                           const T* __restrict__ Hy, const T* __restrict__ Hz,
-                                           OIDT_GLOBAL T* const* psiA) {
-  E3In<T> a;
-  const int nx = v.nx, ny = v.ny, nz = v.nz, N = v.N;
-  const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
-  const unsigned pl = (unsigned)ny * nz;
-  const unsigned idx = (unsigned)ic * pl + (unsigned)jc * nz + kc;
-  const unsigned oi = ic + 1 < nx ? pl : 0u, oj = jc + 1 < ny ? (unsigned)nz : 0u, ok = kc + 1 < nz ? 1u : 0u;
-  const size_t iy = ((size_t)ic * N + f.dj) * nz + kc, iz = ((size_t)ic * ny + jc) * N + f.dk;
-  const size_t ix = ((size_t)f.di * ny + jc) * nz + kc;
-  a.ex = Ex[idx];
-  a.ey = Ey[idx];
-  a.ez = Ez[idx];
-  a.hx = Hx[idx];
-  a.hy = Hy[idx];
-  a.hz = Hz[idx];
-  a.hzj = Hz[idx + oj];
-  a.hyk = Hy[idx + ok];
-  a.hxk = Hx[idx + ok];
-  a.hzi = Hz[idx + oi];
-  a.hyi = Hy[idx + oi];
-  a.hxj = Hx[idx + oj];
-  a.m = v.mat[idx];
-  a.qxy = f.bx && f.sj >= 0 ? psiA[psi_id(F_EX, 1, f.sj)][iy] : T(0);
-  a.qxz = f.bx && f.sk >= 0 ? psiA[psi_id(F_EX, 2, f.sk)][iz] : T(0);
-  a.qyx = f.by && f.si >= 0 ? psiA[psi_id(F_EY, 0, f.si)][ix] : T(0);
-  a.qyz = f.by && f.sk >= 0 ? psiA[psi_id(F_EY, 2, f.sk)][iz] : T(0);
-  a.qzx = f.bz && f.si >= 0 ? psiA[psi_id(F_EZ, 0, f.si)][ix] : T(0);
-  a.qzy = f.bz && f.sj >= 0 ? psiA[psi_id(F_EZ, 1, f.sj)][iy] : T(0);
-  return a;
-}
-
-// Coefficients of one cell's E update (reciprocal kappa per axis, CPML b and
-// c/dx of the slab the cell lies in along each axis).
-template <typename T>
-struct E3C {
-  T ikx, iky, ikz, bei, cei, bej, cej, bek, cek;
-};
+using vec2_t = typename std::conditional<sizeof(T) == 4, oidt_f32x2, oidt_f64x2>::type;
 
 template <typename T>
-__device__ __forceinline__ E3C<T> e3_coef(const View3<T>& v, const E3Where& f, int i, int j, int k) {
-  E3C<T> c;
-  const int ic = min(max(i, 0), v.nx - 1), jc = min(max(j, 0), v.ny - 1), kc = min(max(k, 0), v.nz - 1);
-  c.ikx = v.ike[0][ic];
-  c.iky = v.ike[1][jc];
-  c.ikz = v.ike[2][kc];
-  c.bei = f.si >= 0 ? v.be[f.ci] : T(0);
-  c.cei = f.si >= 0 ? v.ceo[f.ci] : T(0);
-  c.bej = f.sj >= 0 ? v.be[f.cj] : T(0);
-  c.cej = f.sj >= 0 ? v.ceo[f.cj] : T(0);
-  c.bek = f.sk >= 0 ? v.be[f.ck] : T(0);
-  c.cek = f.sk >= 0 ? v.ceo[f.ck] : T(0);
-  return c;
-}
-
-template <typename T>
-struct E3Out {
-  T ex, ey, ez, pxy, pxz, pyx, pyz, pzx, pzy;
-};
-
-// The E update of one cell in the reference order: bulk curl, source, psi of
-// the first then the second transverse axis (fdtd_funcs.py:44-126, FDTD_3D).
-template <typename T>
-__device__ __forceinline__ E3Out<T> e3_update(const View3<T>& v, const E3Where& f, const E3In<T>& a,
-                                              const E3C<T>& c, T jv) {
-  E3Out<T> r;
-  r.ex = a.ex;
-  r.ey = a.ey;
-  r.ez = a.ez;
-  r.pxy = r.pxz = r.pyx = r.pyz = r.pzx = r.pzy = T(0);
-  if (f.bx) {
-    const T cb = cbsel3(v, a.m, 0);
-    const T dHz = a.hzj - a.hz;
-    const T dHy = a.hyk - a.hy;
-    T e = a.ex + cb * (dHz * c.iky - dHy * c.ikz);
-    if (f.sj >= 0) { r.pxy = c.bej * a.qxy + c.cej * dHz; e += cb * r.pxy; }
-    if (f.sk >= 0) { r.pxz = c.bek * a.qxz + c.cek * dHy; e -= cb * r.pxz; }
-    r.ex = e;
-  }
-  if (f.by) {
-    const T cb = cbsel3(v, a.m, 1);
-    const T dHx = a.hxk - a.hx;
-    const T dHz = a.hzi - a.hz;
-    T e = a.ey + cb * (dHx * c.ikz - dHz * c.ikx);
-    if (f.src) e -= jv;
-    if (f.si >= 0) { r.pyx = c.bei * a.qyx + c.cei * dHz; e -= cb * r.pyx; }
-    if (f.sk >= 0) { r.pyz = c.bek * a.qyz + c.cek * dHx; e += cb * r.pyz; }
-    r.ey = e;
-  } else if (f.src) {
-    r.ey = a.ey - jv;
-  }
-  if (f.bz) {
-    const T cb = cbsel3(v, a.m, 2);
-    const T dHy = a.hyi - a.hy;
-    const T dHx = a.hxj - a.hx;
-    T e = a.ez + cb * (dHy * c.ikx - dHx * c.iky);
-    if (f.si >= 0) { r.pzx = c.bei * a.qzx + c.cei * dHy; e += cb * r.pzx; }
-    if (f.sj >= 0) { r.pzy = c.bej * a.qzy + c.cej * dHx; e -= cb * r.pzy; }
-    r.ez = e;
-  }
-  return r;
-}
-
-// Where one cell's H components are updated (fdtd_funcs.py:128-266, FDTD_3D)
-// and the old psi its H-CPML terms read.
-struct H3Where {
-  bool bx, by, bz, src;
-  int si, sj, sk, di, dj, dk, ci, cj, ck;
-};
-
-template <typename T>
-__device__ __forceinline__ H3Where h3_where(const View3<T>& v, int i, int j, int k) {
-  H3Where h;
-  const bool in = i >= 0 && j >= 0 && k >= 0 && i < v.nx && j < v.ny && k < v.nz;
-  const bool iin = in && i >= v.ib && i < v.ie, iin1 = in && i > v.ib && i < v.ie;
-  const bool jin = in && j >= v.jb && j < v.je, jin1 = in && j > v.jb && j < v.je;
-  const bool kin = in && k >= v.kb && k < v.ke, kin1 = in && k > v.kb && k < v.ke;
-  h.bx = iin && jin1 && kin1;
-  h.by = iin1 && jin && kin1;
-  h.bz = iin1 && jin1 && kin;
-  h.src = in && v.src_field == F_HY && i >= v.sx0 && i < v.sx1 && j >= v.sy0 && j < v.sy1 && k >= v.sz0 &&
-          k < v.sz1;
-  h.di = h.dj = h.dk = h.ci = h.cj = h.ck = 0;
-  h.si = in ? slab_h(i, v.ib, v.ie, v.N, h.di, h.ci) : -1;
-  h.sj = jin1 ? slab_h(j, v.jb, v.je, v.N, h.dj, h.cj) : -1;
-  h.sk = kin1 ? slab_h(k, v.kb, v.ke, v.N, h.dk, h.ck) : -1;
-  return h;
-}
-
-template <typename T>
-struct H3In {
-  T qxy, qxz, qyx, qyz, qzx, qzy;
-};
-
-template <typename T>
-__device__ __forceinline__ H3In<T> h3_load(const View3<T>& v, int i, int j, int k, const H3Where& h,
-                                           OIDT_GLOBAL T* const* psiA) {
-  H3In<T> q;
-  const int ny = v.ny, nz = v.nz, N = v.N;
-  const int ic = min(max(i, 0), v.nx - 1), jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
-  const size_t iy = ((size_t)ic * N + h.dj) * nz + kc, iz = ((size_t)ic * ny + jc) * N + h.dk;
-  const size_t ix = ((size_t)h.di * ny + jc) * nz + kc;
-  q.qxy = h.bx && h.sj >= 0 ? psiA[psi_id(F_HX, 1, h.sj)][iy] : T(0);
-  q.qxz = h.bx && h.sk >= 0 ? psiA[psi_id(F_HX, 2, h.sk)][iz] : T(0);
-  q.qyx = h.by && h.si >= 0 ? psiA[psi_id(F_HY, 0, h.si)][ix] : T(0);
-  q.qyz = h.by && h.sk >= 0 ? psiA[psi_id(F_HY, 2, h.sk)][iz] : T(0);
-  q.qzx = h.bz && h.si >= 0 ? psiA[psi_id(F_HZ, 0, h.si)][ix] : T(0);
-  q.qzy = h.bz && h.sj >= 0 ? psiA[psi_id(F_HZ, 1, h.sj)][iy] : T(0);
+__device__ __forceinline__ Q4<T> ld4(const T* base, unsigned boff) {
+  const vec2_t<T> a = *(const OIDT_GLOBAL vec2_t<T>*)(reinterpret_cast<const char*>(base) + boff);
+  Q4<T> q;
+  q.v[0] = a.x;
+  q.v[1] = a.y;
   return q;
 }
 
-template <typename T, int XM>
-__global__ __launch_bounds__(kThreads) void k_3d_step(const Slot<T>* __restrict__ slots,
-                                                      const int4* __restrict__ tiles,
-                                                      const int* __restrict__ counts, int tstep) {
-  constexpr int R = kBY;                        // y-rows of a wave tile
-  constexpr int NITEM = XM + R * (XM + 1);      // halo row, then (plane halo + XM planes) per row
-  static_assert(R * XM <= kBX, "the k0-1 halo column takes one cell per lane");
-  __shared__ T rowbuf[kBY][2][XM][kBX];         // per wave: new Ex, Ez of the previous row, per plane
+// Non-temporal store of the lane's cells (nothing in the same launch reads them).
+template <typename T>
+__device__ __forceinline__ void st4_nt(T* base, unsigned boff, const Q4<T>& q) {
+  const vec2_t<T> a = {q.v[0], q.v[1]};
+  __builtin_nontemporal_store(a, (OIDT_GLOBAL vec2_t<T>*)(reinterpret_cast<char*>(base) + boff));
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_put4(T* p, const Q4<T>& q) {
+  *reinterpret_cast<vec2_t<T>*>(p) = vec2_t<T>{q.v[0], q.v[1]};
+}
+
+template <typename T>
+__device__ __forceinline__ Q4<T> lds_get4(const T* p) {
+  const vec2_t<T> a = *reinterpret_cast<const vec2_t<T>*>(p);
+  Q4<T> q;
+  q.v[0] = a.x;
+  q.v[1] = a.y;
+  return q;
+}
+
+// Old values a cell group loads for one plane (set A): E, the material bytes
+// and the psi of the E update (x and y slabs contiguous in z; z slab per
+// cell), and separately the psi of the H update.
+template <typename T>
+struct E3Old {
+  Q4<T> ex, ey, ez;
+  unsigned m;                 // material bytes of the lane's cells
+  Q4<T> pyx, pzx, pxy, pzy, pxz, pyz;   // psi ey_x, ez_x, ex_y, ez_y, ex_z, ey_z
+};
+template <typename T>
+struct H3Old {
+  Q4<T> pyx, pzx, pxy, pzy, pxz, pyz;   // psi hy_x, hz_x, hx_y, hz_y, hx_z, hy_z
+};
+
+template <typename T>
+__device__ __forceinline__ Q4<T> q4_zero() {
+  Q4<T> q;
+  q.v[0] = q.v[1] = T(0);
+  return q;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* __restrict__ slots,
+                                                                 const int4* __restrict__ tiles,
+                                                                 const int* __restrict__ counts, int tstep,
+                                                                 int xm) {
+  constexpr int W = Fused3d<T>::W;
+  constexpr int C = Fused3d<T>::C;
+  constexpr int ZSEG = Fused3d<T>::ZSEG;
+  constexpr unsigned es = sizeof(T);
+  __shared__ T lds[Fused3d<T>::LDS];
   const int ntiles = counts[0];
   const int lane = threadIdx.x;
   const int w = wave_id();
-  const int stride = gridDim.x * kBY;
-  for (int tt = xcd_block() * kBY + w; tt < ntiles; tt += stride) {
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
     const int4 tl = tiles[tt];
     const Slot<T>& s = slots[tl.x];
     const int n = s.base + tstep;
-    if (!s.active || n >= s.nt) continue;
-    const View3<T> v = view3(s);
+    if (!s.active || n >= s.nt) continue;     // uniform over the block: every wave skips together
+    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+    const int zw = (nz + ZSEG - 1) / ZSEG;     // waves per row (nz: padded extent)
+    const int R = W / zw;                      // rows of the block, the halo row first
+    const int NZS = zw * ZSEG;                 // LDS row stride (cells)
+    const int r = w / zw, zi = w - r * zw;
+    const int i0 = tl.y, j0 = tl.z;
+    const int iend = min(i0 + xm, nx);
+    const int j = j0 - 1 + r;
+    const int k = zi * ZSEG + C * lane;        // first of the lane's cells
+    const bool inrow = r < R;                  // waves past R * zw (when zw does not divide W) idle
+    const bool live = inrow && j >= 0 && j < ny && k < nz;
+    const bool owner = live && r >= 1;         // stores E and H (the halo row only computes E)
+    const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
     const bool odd = (n & 1) != 0;
     OIDT_GLOBAL T* const* FA = odd ? s.f2 : s.f;
     OIDT_GLOBAL T* const* FB = odd ? s.f : s.f2;
-    const T* __restrict__ Ex = FA[F_EX];
-    const T* __restrict__ Ey = FA[F_EY];
-    const T* __restrict__ Ez = FA[F_EZ];
-    const T* __restrict__ Hx = FA[F_HX];
-    const T* __restrict__ Hy = FA[F_HY];
-    const T* __restrict__ Hz = FA[F_HZ];
+    const T* __restrict__ ExA = FA[F_EX];
+    const T* __restrict__ EyA = FA[F_EY];
+    const T* __restrict__ EzA = FA[F_EZ];
+    const T* __restrict__ HxA = FA[F_HX];
+    const T* __restrict__ HyA = FA[F_HY];
+    const T* __restrict__ HzA = FA[F_HZ];
+    T* __restrict__ ExB = FB[F_EX];
+    T* __restrict__ EyB = FB[F_EY];
+    T* __restrict__ EzB = FB[F_EZ];
+    T* __restrict__ HxB = FB[F_HX];
+    T* __restrict__ HyB = FB[F_HY];
+    T* __restrict__ HzB = FB[F_HZ];
     OIDT_GLOBAL T* const* psiA = odd ? s.psi2 : s.psi;
     OIDT_GLOBAL T* const* psiB = odd ? s.psi : s.psi2;
-    const T jvE = v.src_field == F_EY ? s.jsrc[n] : T(0);
-    const T jvH = v.src_field == F_HY ? s.jsrc[n] : T(0);
-    const int nx = v.nx, ny = v.ny, nz = v.nz, N = v.N;
-    const int i0 = tl.y, j0 = tl.z, k0 = tl.w;
-    const int k = k0 + lane;
-    const bool kok = k < nz;
-    const int kc = min(k, nz - 1);
+    const uint8_t* __restrict__ mat = s.mat;
+    const T db = s.db;
     const unsigned pl = (unsigned)ny * nz;
-    const int port0 = s.port[0], port1 = s.port[1];
-    const int wx0 = s.wx0, wx1 = s.wx1, wy0 = s.wy0, wy1 = s.wy1, prow = s.prow;
-    T* const ring = s.ring;
-    const int nplanes = s.nplanes, pcells = s.pcells;
-    const bool kcapE = kok && (k == port0 || k == port1);
-    const bool kcapH = kok && (k == port0 || k == port0 + 1 || k == port1 || k == port1 + 1);
-    // ---- coefficients, loaded once per tile: lane l holds plane i0-1+l (l <= XM)
-    // and row j0-1+l (l <= R); z coefficients of the lane's own column
-    T lp_ike = T(0), lp_be = T(0), lp_ce = T(0), lp_ikh = T(0), lp_bh = T(0), lp_ch = T(0);
-    if (lane <= XM) {
-      const int ip = i0 - 1 + lane;
-      if (ip >= 0 && ip < nx) {
-        int d = 0, ci = 0;
-        lp_ike = v.ike[0][ip];
-        if (slab_e(ip, v.ib, v.ie, N, d, ci) >= 0) { lp_be = v.be[ci]; lp_ce = v.ceo[ci]; }
-        lp_ikh = v.ikh[0][ip];
-        if (slab_h(ip, v.ib, v.ie, N, d, ci) >= 0) { lp_bh = v.bh[ci]; lp_ch = v.cho[ci]; }
-      }
+    const int jc = min(max(j, 0), ny - 1);
+    const unsigned kq = (unsigned)min(k, nz - C);
+    const unsigned jk = (unsigned)jc * nz + kq;
+    T* const sH = lds;                         // old H: 3 components x (R + 1) rows
+    T* const sE = lds + 3 * (R + 1) * NZS;     // new E: 3 components x R rows
+    // ---- row (wave-uniform) coefficients
+    const bool jinE = j >= jb && j < je, jin1 = j > jb && j < je;
+    int dje = 0, cje = 0, djh = 0, cjh = 0;
+    const int sjE = jinE ? slab_e(j, jb, je, N, dje, cje) : -1;
+    const int sjH = jin1 ? slab_h(j, jb, je, N, djh, cjh) : -1;
+    const T ikyE = jinE ? s.ike[1][j] : T(0);
+    const T ikyH = jin1 ? s.ikh[1][j] : T(0);
+    const T bej = sjE >= 0 ? s.be[cje] : T(0), cej = sjE >= 0 ? s.ceo[cje] : T(0);
+    const T bhj = sjH >= 0 ? s.bh[cjh] : T(0), chj = sjH >= 0 ? s.cho[cjh] : T(0);
+    // ---- z coefficients of the lane's cells (E: k in [kb, ke); H: k in (kb, ke))
+    T ikzE[C], ikzH[C];
+    bool zslab = false;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int kk = k + c;
+      const bool kin = kk >= kb && kk < ke, kin1 = kk > kb && kk < ke;
+      ikzE[c] = live && kin ? s.ike[2][kk] : T(0);
+      ikzH[c] = live && kin1 ? s.ikh[2][kk] : T(0);
+      zslab = zslab || (kk >= kb && kk < kb + N) || (kk >= ke - N && kk < ke);
     }
-    T lr_ike = T(0), lr_be = T(0), lr_ce = T(0), lr_ikh = T(0), lr_bh = T(0), lr_ch = T(0);
-    if (lane <= R) {
-      const int jp = j0 - 1 + lane;
-      if (jp >= 0 && jp < ny) {
-        int d = 0, cj = 0;
-        lr_ike = v.ike[1][jp];
-        if (slab_e(jp, v.jb, v.je, N, d, cj) >= 0) { lr_be = v.be[cj]; lr_ce = v.ceo[cj]; }
-        const bool jin1 = jp > v.jb && jp < v.je;
-        lr_ikh = jin1 ? v.ikh[1][jp] : T(0);
-        if (jin1 && slab_h(jp, v.jb, v.je, N, d, cj) >= 0) { lr_bh = v.bh[cj]; lr_ch = v.cho[cj]; }
-      }
-    }
-    T kz_ike = v.ike[2][kc], kz_be = T(0), kz_ce = T(0);
+    // ---- x coefficients: lane u holds plane i0 - 1 + u
+    T lx_ikE = T(0), lx_bE = T(0), lx_cE = T(0), lx_ikH = T(0), lx_bH = T(0), lx_cH = T(0);
     {
-      int d = 0, ck = 0;
-      if (kok && slab_e(k, v.kb, v.ke, N, d, ck) >= 0) { kz_be = v.be[ck]; kz_ce = v.ceo[ck]; }
+      const int ip = i0 - 1 + lane;
+      if (lane <= xm && ip >= 0 && ip < nx) {
+        int d = 0, ci = 0;
+        lx_ikE = s.ike[0][ip];
+        if (slab_e(ip, ib, ie, N, d, ci) >= 0) { lx_bE = s.be[ci]; lx_cE = s.ceo[ci]; }
+        lx_ikH = s.ikh[0][ip];
+        if (slab_h(ip, ib, ie, N, d, ci) >= 0) { lx_bH = s.bh[ci]; lx_cH = s.cho[ci]; }
+      }
     }
-    const bool kin1 = kok && k > v.kb && k < v.ke;
-    int dkh = 0, ckh = 0;
-    const int skh = kin1 ? slab_h(k, v.kb, v.ke, N, dkh, ckh) : -1;
-    const T ikzh = kin1 ? v.ikh[2][k] : T(0);
-    const T bhk = skh >= 0 ? v.bh[ckh] : T(0), chk = skh >= 0 ? v.cho[ckh] : T(0);
-    // ---- column k0-1 (halo, not stored): lane l holds cell (i0 + l / R, j0 + l % R, k0 - 1)
-    T colEx = T(0), colEy = T(0);
-    if (lane < R * XM) {
-      const int ih = i0 + lane / R, jh = j0 + lane % R;
-      const E3Where f = e3_where(v, ih, jh, k0 - 1);
-      const E3Out<T> o = e3_update(v, f, e3_load(v, ih, jh, k0 - 1, f, Ex, Ey, Ez, Hx, Hy, Hz, psiA),
-                                   e3_coef(v, f, ih, jh, k0 - 1), jvE);
-      colEx = o.ex;
-      colEy = o.ey;
-    }
-    // item t -> cell: t < XM: (i0 + t, j0 - 1), the row below the tile;
-    // otherwise row r = (t - XM) / (XM + 1), position q: q = 0 the plane
-    // i0 - 1 of the row, q >= 1 the owned plane i0 + q - 1.
-    auto cell_of = [&](int t, int& i, int& j) {
-      if (t < XM) {
-        i = i0 + t;
-        j = j0 - 1;
-      } else {
-        const int r = (t - XM) / (XM + 1), q = (t - XM) - r * (XM + 1);
-        i = i0 - 1 + q;
-        j = j0 + r;
+    const T cx0 = s.cbv[0][0], cx1 = s.cbv[0][1], cx2 = s.cbv[0][2], cx3 = s.cbv[0][3];
+    const T cy0 = s.cbv[1][0], cy1 = s.cbv[1][1], cy2 = s.cbv[1][2], cy3 = s.cbv[1][3];
+    const T cz0 = s.cbv[2][0], cz1 = s.cbv[2][1], cz2 = s.cbv[2][2], cz3 = s.cbv[2][3];
+    // sources: TE on Ey (E half-step), TM on Hy (start of the H half-step)
+    const bool jsrcE = s.src_field == F_EY && j >= s.sy0 && j < s.sy1;
+    const bool jsrcH = s.src_field == F_HY && j >= s.sy0 && j < s.sy1;
+    const T jv = (jsrcE || jsrcH) ? s.jsrc[n] : T(0);
+    const int sz0 = s.sz0, sz1 = s.sz1, sx0 = s.sx0, sx1 = s.sx1;
+    // capture window (rows and planes inclusive of wy1 / wx1) and port planes
+    const bool jcap = owner && j >= s.wy0 && j <= s.wy1;
+    const int wx0 = s.wx0, wx1 = s.wx1, prow = s.prow, pcells = s.pcells, nplanes = s.nplanes, wy0 = s.wy0;
+    const int port0 = s.port[0], port1 = s.port[1];
+    T* const ring = s.ring;
+    const unsigned xhi = s.psi_hi[0], zhi = s.psi_hi[2];
+    const T* const beT = s.be;
+    const T* const ceT = s.ceo;
+    const T* const bhT = s.bh;
+    const T* const chT = s.cho;
+
+    // x slab of plane i: side (-1: none), offset of the lane's cells in the slab arrays
+    auto xslab = [&](int i, unsigned& ix) {
+      int di = 0, ci = 0;
+      const int si = slab_e(i, ib, ie, N, di, ci);    // E and H psi share the cell index d
+      ix = (((unsigned)di * ny + jc) * nz + kq) * es + (si == 1 ? xhi : 0u);
+      return si;
+    };
+    auto zoff = [&](int i, int kk, bool hside, int& sk, int& ck) {
+      int dk = 0;
+      ck = 0;
+      sk = hside ? ((kk > kb && kk < ke) ? slab_h(kk, kb, ke, N, dk, ck) : -1)
+                 : ((kk >= kb && kk < ke) ? slab_e(kk, kb, ke, N, dk, ck) : -1);
+      return (((unsigned)i * ny + jc) * N + dk) * es + (sk == 1 ? zhi : 0u);
+    };
+    // E-update values of plane i (set A)
+    auto loadE = [&](int i, E3Old<T>& p) {
+      const unsigned idx = (unsigned)i * pl + jk;
+      const unsigned bo = idx * es;
+      p.ex = ld4<T>(ExA, bo);
+      p.ey = ld4<T>(EyA, bo);
+      p.ez = ld4<T>(EzA, bo);
+      p.m = *(const OIDT_GLOBAL unsigned short*)(reinterpret_cast<const char*>(mat) + idx);
+      unsigned ix;
+      if (xslab(i, ix) >= 0) {
+        p.pyx = ld4<T>(psiA[psi_id(F_EY, 0, 0)], ix);
+        p.pzx = ld4<T>(psiA[psi_id(F_EZ, 0, 0)], ix);
+      }
+      if (sjE >= 0) {
+        const unsigned iy = (((unsigned)i * N + dje) * nz + kq) * es;
+        p.pxy = ld4<T>(psiA[psi_id(F_EX, 1, sjE)], iy);
+        p.pzy = ld4<T>(psiA[psi_id(F_EZ, 1, sjE)], iy);
+      }
+      if (zslab) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          int sk, ck;
+          const unsigned iz = zoff(i, k + c, false, sk, ck);
+          if (sk >= 0) {
+            p.pxz.v[c] = ldg<T>(psiA[psi_id(F_EX, 2, 0)], iz);
+            p.pyz.v[c] = ldg<T>(psiA[psi_id(F_EY, 2, 0)], iz);
+          }
+        }
       }
     };
-    int ic, jc;
-    cell_of(0, ic, jc);
-    E3Where fc = e3_where(v, ic, jc, k);
-    E3In<T> ac = e3_load(v, ic, jc, k, fc, Ex, Ey, Ez, Hx, Hy, Hz, psiA);
-    H3In<T> qc = {};
-    T pey = T(0), pez = T(0);   // new Ey, Ez of the previous item of the row (i-1)
-#pragma unroll 1
-    for (int t = 0; t < NITEM; ++t) {
-      int i, j;
-      cell_of(t, i, j);
-      // prefetch the next item (its E inputs and, for owned cells, its H psi)
-      E3Where fn = fc;
-      E3In<T> an = ac;
-      H3In<T> qn = qc;
-      if (t + 1 < NITEM) {
-        int i2, j2;
-        cell_of(t + 1, i2, j2);
-        fn = e3_where(v, i2, j2, k);
-        an = e3_load(v, i2, j2, k, fn, Ex, Ey, Ez, Hx, Hy, Hz, psiA);
-        if (j2 >= j0 && i2 >= i0) qn = h3_load(v, i2, j2, k, h3_where(v, i2, j2, k), psiA);
+    // H-update psi of plane i (set A)
+    auto loadHpsi = [&](int i, H3Old<T>& p) {
+      unsigned ix;
+      if (xslab(i, ix) >= 0) {
+        p.pyx = ld4<T>(psiA[psi_id(F_HY, 0, 0)], ix);
+        p.pzx = ld4<T>(psiA[psi_id(F_HZ, 0, 0)], ix);
       }
-      // coefficients of this item from the per-tile vectors
-      const int pi = i - (i0 - 1), rj = j - (j0 - 1);
-      E3C<T> cf;
-      cf.ikx = lane_get(lp_ike, pi);
-      cf.bei = lane_get(lp_be, pi);
-      cf.cei = lane_get(lp_ce, pi);
-      cf.iky = lane_get(lr_ike, rj);
-      cf.bej = lane_get(lr_be, rj);
-      cf.cej = lane_get(lr_ce, rj);
-      cf.ikz = kz_ike;
-      cf.bek = kz_be;
-      cf.cek = kz_ce;
-      const E3Out<T> e = e3_update(v, fc, ac, cf, jvE);
-      if (t < XM) {
-        rowbuf[w][0][t][lane] = e.ex;
-        rowbuf[w][1][t][lane] = e.ez;
-      } else {
-        const int r = (t - XM) / (XM + 1), q = (t - XM) - r * (XM + 1);
-        if (q > 0 && i < nx && j < ny) {
-          const int u = q - 1;
-          const unsigned idx = (unsigned)i * pl + (unsigned)j * nz + kc;
-          // ---- E stores of (i, j, k)
-          if (fc.bx) {
-            FB[F_EX][idx] = e.ex;
-            if (fc.sj >= 0) psiB[psi_id(F_EX, 1, fc.sj)][((size_t)i * N + fc.dj) * nz + k] = e.pxy;
-            if (fc.sk >= 0) psiB[psi_id(F_EX, 2, fc.sk)][((size_t)i * ny + j) * N + fc.dk] = e.pxz;
+      if (sjH >= 0) {
+        const unsigned iy = (((unsigned)i * N + djh) * nz + kq) * es;
+        p.pxy = ld4<T>(psiA[psi_id(F_HX, 1, sjH)], iy);
+        p.pzy = ld4<T>(psiA[psi_id(F_HZ, 1, sjH)], iy);
+      }
+      if (zslab) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          int sk, ck;
+          const unsigned iz = zoff(i, k + c, true, sk, ck);
+          if (sk >= 0) {
+            p.pxz.v[c] = ldg<T>(psiA[psi_id(F_HX, 2, 0)], iz);
+            p.pyz.v[c] = ldg<T>(psiA[psi_id(F_HY, 2, 0)], iz);
           }
-          if (fc.by) {
-            FB[F_EY][idx] = e.ey;
-            if (fc.si >= 0) psiB[psi_id(F_EY, 0, fc.si)][((size_t)fc.di * ny + j) * nz + k] = e.pyx;
-            if (fc.sk >= 0) psiB[psi_id(F_EY, 2, fc.sk)][((size_t)i * ny + j) * N + fc.dk] = e.pyz;
-          } else if (fc.src) {
-            FB[F_EY][idx] = e.ey;
-          }
-          if (fc.bz) {
-            FB[F_EZ][idx] = e.ez;
-            if (fc.si >= 0) psiB[psi_id(F_EZ, 0, fc.si)][((size_t)fc.di * ny + j) * nz + k] = e.pzx;
-            if (fc.sj >= 0) psiB[psi_id(F_EZ, 1, fc.sj)][((size_t)i * N + fc.dj) * nz + k] = e.pzy;
-          }
-          const bool icap = i >= wx0 && i <= wx1, jcap = j >= wy0 && j <= wy1;
-          if (kcapE && jcap && icap) {
-            const size_t c = (size_t)((i - wx0) * prow + (j - wy0));
-            const size_t base = (size_t)tstep * nplanes;
-            const int cap = k == port0 ? 0 : 1;
-            ring[(base + cap * 6 + 0) * pcells + c] = e.ex;
-            ring[(base + cap * 6 + 1) * pcells + c] = e.ey;
-            if (port0 == port1) {
-              ring[(base + 6) * pcells + c] = e.ex;
-              ring[(base + 7) * pcells + c] = e.ey;
-            }
-          }
-          // ---- H update of (i, j, k): new E of the i-1, j-1 and k-1 neighbours
-          const T exk = lane_up1(e.ex, lane_get(colEx, u * R + r));
-          const T eyk = lane_up1(e.ey, lane_get(colEy, u * R + r));
-          const T exj = rowbuf[w][0][u][lane], ezj = rowbuf[w][1][u][lane];
-          const T eyi = pey, ezi = pez;
-          const H3Where h = h3_where(v, i, j, k);
-          const T ikxh = lane_get(lp_ikh, pi), bhi = lane_get(lp_bh, pi), chi = lane_get(lp_ch, pi);
-          const T ikyh = lane_get(lr_ikh, rj), bhj = lane_get(lr_bh, rj), chj = lane_get(lr_ch, rj);
-          const T db = v.db;
-          const size_t iy = ((size_t)i * N + h.dj) * nz + kc, iz = ((size_t)i * ny + j) * N + h.dk;
-          const size_t ix = ((size_t)h.di * ny + j) * nz + kc;
-          T vx = ac.hx;
-          if (h.bx) {
-            const T dEy = e.ey - eyk;
-            const T dEz = e.ez - ezj;
-            vx = vx + db * (dEy * ikzh - dEz * ikyh);
-            if (h.sj >= 0) {
-              const T p = bhj * qc.qxy + chj * dEz;
-              psiB[psi_id(F_HX, 1, h.sj)][iy] = p;
-              vx -= db * p;
-            }
-            if (h.sk >= 0) {
-              const T p = bhk * qc.qxz + chk * dEy;
-              psiB[psi_id(F_HX, 2, h.sk)][iz] = p;
-              vx += db * p;
-            }
-            FB[F_HX][idx] = vx;
-          }
-          T vy = ac.hy;
-          if (h.src) vy -= jvH;
-          if (h.by) {
-            const T dEz = e.ez - ezi;
-            const T dEx = e.ex - exk;
-            vy = vy + db * (dEz * ikxh - dEx * ikzh);
-            if (h.si >= 0) {
-              const T p = bhi * qc.qyx + chi * dEz;
-              psiB[psi_id(F_HY, 0, h.si)][ix] = p;
-              vy += db * p;
-            }
-            if (h.sk >= 0) {
-              const T p = bhk * qc.qyz + chk * dEx;
-              psiB[psi_id(F_HY, 2, h.sk)][iz] = p;
-              vy -= db * p;
-            }
-          }
-          if (h.by || h.src) FB[F_HY][idx] = vy;
-          if (h.bz) {
-            const T dEx = e.ex - exj;
-            const T dEy = e.ey - eyi;
-            T vz = ac.hz + db * (dEx * ikyh - dEy * ikxh);
-            if (h.si >= 0) {
-              const T p = bhi * qc.qzx + chi * dEy;
-              psiB[psi_id(F_HZ, 0, h.si)][ix] = p;
-              vz -= db * p;
-            }
-            if (h.sj >= 0) {
-              const T p = bhj * qc.qzy + chj * dEx;
-              psiB[psi_id(F_HZ, 1, h.sj)][iy] = p;
-              vz += db * p;
-            }
-            FB[F_HZ][idx] = vz;
-          }
-          if (kcapH && jcap && icap) {
-            const size_t c = (size_t)((i - wx0) * prow + (j - wy0));
-            const size_t base = (size_t)tstep * nplanes;
-            if (k == port0) { ring[(base + 2) * pcells + c] = vx; ring[(base + 4) * pcells + c] = vy; }
-            if (k == port0 + 1) { ring[(base + 3) * pcells + c] = vx; ring[(base + 5) * pcells + c] = vy; }
-            if (k == port1) { ring[(base + 8) * pcells + c] = vx; ring[(base + 10) * pcells + c] = vy; }
-            if (k == port1 + 1) { ring[(base + 9) * pcells + c] = vx; ring[(base + 11) * pcells + c] = vy; }
-          }
-          rowbuf[w][0][u][lane] = e.ex;
-          rowbuf[w][1][u][lane] = e.ez;
         }
-        pey = e.ey;
-        pez = e.ez;
       }
-      fc = fn;
-      ac = an;
-      qc = qn;
+    };
+    auto loadH = [&](int i, Q4<T>& hx, Q4<T>& hy, Q4<T>& hz) {
+      const unsigned bo = ((unsigned)i * pl + jk) * es;
+      hx = ld4<T>(HxA, bo);
+      hy = ld4<T>(HyA, bo);
+      hz = ld4<T>(HzA, bo);
+    };
+    // the row above the tile (row R of the block): old Hx, Hz of plane i,
+    // loaded by the halo row's waves
+    const int jtop = j0 - 1 + R;
+    const bool toprow = r == 0 && jtop < ny && k < nz;   // (jtop >= 0: R >= 2)
+    auto loadTop = [&](int i, Q4<T>& hx, Q4<T>& hz) {
+      const unsigned bo = ((unsigned)i * pl + (unsigned)jtop * nz + kq) * es;
+      hx = ld4<T>(HxA, bo);
+      hz = ld4<T>(HzA, bo);
+    };
+
+    // new E of plane i for the lane's cells; `store`: write its psi to set B
+    auto updE = [&](int i, const E3Old<T>& p, const Q4<T>& hx, const Q4<T>& hy, const Q4<T>& hz,
+                    const Q4<T>& hy1, const Q4<T>& hz1, bool store, Q4<T>& ex, Q4<T>& ey, Q4<T>& ez) {
+      const int u = i - (i0 - 1);
+      const T ikx = lane_get(lx_ikE, u), bex = lane_get(lx_bE, u), cex = lane_get(lx_cE, u);
+      unsigned ix;
+      const int si = xslab(i, ix);
+      const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
+      const bool isrc = i >= sx0 && i < sx1;
+      // neighbours j+1 (LDS row r + 1) and k+1 (next cell; the next lane's first from LDS)
+      const Q4<T> hxj = lds_get4(sH + (0 * (R + 1) + r + 1) * NZS + k);
+      const Q4<T> hzj = lds_get4(sH + (2 * (R + 1) + r + 1) * NZS + k);
+      const int kn = min(k + C, NZS - 1);
+      const T hxk4 = sH[(0 * (R + 1) + r) * NZS + kn], hyk4 = sH[(1 * (R + 1) + r) * NZS + kn];
+      Q4<T> qyx = p.pyx, qzx = p.pzx, qxy = p.pxy, qzy = p.pzy;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int kk = k + c;
+        const bool kin = kk >= kb && kk < ke, kin1 = kk > kb && kk < ke;
+        const bool bx = iin1 && jinE && kin;
+        const bool by = iin && jin1 && kin;
+        const bool bz = iin && jinE && kin1;
+        const bool src = jsrcE && isrc && kk >= sz0 && kk < sz1;
+        const unsigned mc = (p.m >> (8 * c)) & 0xffu;
+        const T hxk = c < C - 1 ? hx.v[c + 1] : hxk4;
+        const T hyk = c < C - 1 ? hy.v[c + 1] : hyk4;
+        const T hz0 = hz.v[c], hy0 = hy.v[c];
+        const T ikz = ikzE[c];
+        int sk = -1, ck = 0;
+        unsigned iz = 0;
+        if (zslab) iz = zoff(i, kk, false, sk, ck);
+        T vx = p.ex.v[c], vy = p.ey.v[c], vz = p.ez.v[c];
+        // update_ex: i (ib,ie), j [jb,je), k [kb,ke); psi ex_y (+), ex_z (-)
+        if (bx) {
+          const T cb = cb_of(cx0, cx1, cx2, cx3, mc, 0);
+          const T dHz = hzj.v[c] - hz0;
+          const T dHy = hyk - hy0;
+          vx = vx + cb * (dHz * ikyE - dHy * ikz);
+          if (sjE >= 0) { const T q = bej * p.pxy.v[c] + cej * dHz; qxy.v[c] = q; vx += cb * q; }
+          if (sk >= 0) {
+            const T q = beT[ck] * p.pxz.v[c] + ceT[ck] * dHy;
+            if (store) stg<T>(psiB[psi_id(F_EX, 2, 0)], iz, q);
+            vx -= cb * q;
+          }
+        }
+        // update_ey: i [ib,ie), j (jb,je), k [kb,ke); source (TE); psi ey_x (-), ey_z (+)
+        if (by) {
+          const T cb = cb_of(cy0, cy1, cy2, cy3, mc, 1);
+          const T dHx = hxk - hx.v[c];
+          const T dHz = hz1.v[c] - hz0;
+          vy = vy + cb * (dHx * ikz - dHz * ikx);
+          if (src) vy -= jv;
+          if (si >= 0) { const T q = bex * p.pyx.v[c] + cex * dHz; qyx.v[c] = q; vy -= cb * q; }
+          if (sk >= 0) {
+            const T q = beT[ck] * p.pyz.v[c] + ceT[ck] * dHx;
+            if (store) stg<T>(psiB[psi_id(F_EY, 2, 0)], iz, q);
+            vy += cb * q;
+          }
+        } else if (src) {
+          vy -= jv;
+        }
+        // update_ez: i [ib,ie), j [jb,je), k (kb,ke); psi ez_x (+), ez_y (-)
+        if (bz) {
+          const T cb = cb_of(cz0, cz1, cz2, cz3, mc, 2);
+          const T dHy = hy1.v[c] - hy0;
+          const T dHx = hxj.v[c] - hx.v[c];
+          vz = vz + cb * (dHy * ikx - dHx * ikyE);
+          if (si >= 0) { const T q = bex * p.pzx.v[c] + cex * dHy; qzx.v[c] = q; vz += cb * q; }
+          if (sjE >= 0) { const T q = bej * p.pzy.v[c] + cej * dHx; qzy.v[c] = q; vz -= cb * q; }
+        }
+        ex.v[c] = vx;
+        ey.v[c] = vy;
+        ez.v[c] = vz;
+      }
+      if (store) {
+        // x- and y-slab psi of the lane's cells: cells outside a component's
+        // range keep their value (0: never updated)
+        if (si >= 0) {
+          st4_nt<T>(psiB[psi_id(F_EY, 0, 0)], ix, qyx);
+          st4_nt<T>(psiB[psi_id(F_EZ, 0, 0)], ix, qzx);
+        }
+        if (sjE >= 0) {
+          const unsigned iy = (((unsigned)i * N + dje) * nz + kq) * es;
+          st4_nt<T>(psiB[psi_id(F_EX, 1, sjE)], iy, qxy);
+          st4_nt<T>(psiB[psi_id(F_EZ, 1, sjE)], iy, qzy);
+        }
+      }
+    };
+
+    Q4<T> hxc, hyc, hzc;        // old H of the plane being updated
+    Q4<T> hxn, hyn, hzn;        // old H of the next plane
+    Q4<T> tpx, tpz;             // old Hx, Hz of the row above the tile, plane being updated
+    Q4<T> eyp, ezp;             // new Ey, Ez of the previous plane
+    E3Old<T> pc{};
+    hxc = hyc = hzc = hxn = hyn = hzn = tpx = tpz = eyp = ezp = q4_zero<T>();
+    const int ifirst = i0 > 0 ? i0 - 1 : i0;   // the plane below the tile: new E only
+    if (live) {
+      loadE(ifirst, pc);
+      loadH(ifirst, hxc, hyc, hzc);
+      loadH(min(ifirst + 1, nx - 1), hxn, hyn, hzn);
     }
+    if (toprow) loadTop(ifirst, tpx, tpz);
+    for (int i = ifirst; i < iend; ++i) {
+      const bool pro = i < i0;
+      // 1. old H of plane i to the LDS rows
+      if (inrow) {
+        lds_put4(sH + (0 * (R + 1) + r) * NZS + k, hxc);
+        lds_put4(sH + (1 * (R + 1) + r) * NZS + k, hyc);
+        lds_put4(sH + (2 * (R + 1) + r) * NZS + k, hzc);
+        if (r == 0) {
+          lds_put4(sH + (0 * (R + 1) + R) * NZS + k, tpx);
+          lds_put4(sH + (2 * (R + 1) + R) * NZS + k, tpz);
+        }
+      }
+      __syncthreads();
+      // 2. new E of plane i
+      Q4<T> exn, eyn, ezn;
+      if (inrow) updE(i, pc, hxc, hyc, hzc, hyn, hzn, owner && !pro, exn, eyn, ezn);
+      // 3. new E to the LDS rows; H psi of plane i and the loads of plane i + 1
+      if (inrow) {
+        lds_put4(sE + (0 * R + r) * NZS + k, exn);
+        lds_put4(sE + (1 * R + r) * NZS + k, eyn);
+        lds_put4(sE + (2 * R + r) * NZS + k, ezn);
+      }
+      H3Old<T> ph{};
+      const bool hrow_ = owner && !pro;
+      if (hrow_) loadHpsi(i, ph);
+      const bool more = i + 1 < iend;
+      E3Old<T> pn{};
+      Q4<T> hxn2 = hxn, hyn2 = hyn, hzn2 = hzn, tpxn = tpx, tpzn = tpz;
+      if (more) {
+        if (live) {
+          loadE(i + 1, pn);
+          loadH(min(i + 2, nx - 1), hxn2, hyn2, hzn2);
+        }
+        if (toprow) loadTop(i + 1, tpxn, tpzn);
+      }
+      __syncthreads();
+      if (hrow_) {
+        // 4. new H of plane i, tile rows
+        const int u = i - (i0 - 1);
+        const T ikx = lane_get(lx_ikH, u), bhx = lane_get(lx_bH, u), chx = lane_get(lx_cH, u);
+        int di = 0, ci = 0;
+        const int si = slab_h(i, ib, ie, N, di, ci);
+        unsigned ix;
+        (void)xslab(i, ix);
+        const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
+        const bool isrc = i >= sx0 && i < sx1;
+        const Q4<T> exj = lds_get4(sE + (0 * R + r - 1) * NZS + k);
+        const Q4<T> ezj = lds_get4(sE + (2 * R + r - 1) * NZS + k);
+        const int kp = max(k - 1, 0);
+        const T exk0 = sE[(0 * R + r) * NZS + kp], eyk0 = sE[(1 * R + r) * NZS + kp];
+        Q4<T> hxo, hyo, hzo;
+        Q4<T> qyx = ph.pyx, qzx = ph.pzx, qxy = ph.pxy, qzy = ph.pzy;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const int kk = k + c;
+          const bool kin = kk >= kb && kk < ke, kin1 = kk > kb && kk < ke;
+          const bool bx = iin && jin1 && kin1;
+          const bool by = iin1 && jinE && kin1;
+          const bool bz = iin1 && jin1 && kin;
+          const bool src = jsrcH && isrc && kk >= sz0 && kk < sz1;
+          const T ikz = ikzH[c];
+          const T ex = exn.v[c], ey = eyn.v[c], ez = ezn.v[c];
+          const T exk = c > 0 ? exn.v[c - 1] : exk0;
+          const T eyk = c > 0 ? eyn.v[c - 1] : eyk0;
+          int sk = -1, ck = 0;
+          unsigned iz = 0;
+          if (zslab) iz = zoff(i, kk, true, sk, ck);
+          // update_hx: i [ib,ie), j (jb,je), k (kb,ke); psi hx_y (-), hx_z (+)
+          T vx = hxc.v[c];
+          if (bx) {
+            const T dEy = ey - eyk;
+            const T dEz = ez - ezj.v[c];
+            vx = vx + db * (dEy * ikz - dEz * ikyH);
+            if (sjH >= 0) { const T q = bhj * ph.pxy.v[c] + chj * dEz; qxy.v[c] = q; vx -= db * q; }
+            if (sk >= 0) {
+              const T q = bhT[ck] * ph.pxz.v[c] + chT[ck] * dEy;
+              stg<T>(psiB[psi_id(F_HX, 2, 0)], iz, q);
+              vx += db * q;
+            }
+          }
+          // update_hy: i (ib,ie), j [jb,je), k (kb,ke); TM source first; psi hy_x (+), hy_z (-)
+          T vy = hyc.v[c];
+          if (src) vy -= jv;
+          if (by) {
+            const T dEz = ez - ezp.v[c];
+            const T dEx = ex - exk;
+            vy = vy + db * (dEz * ikx - dEx * ikz);
+            if (si >= 0) { const T q = bhx * ph.pyx.v[c] + chx * dEz; qyx.v[c] = q; vy += db * q; }
+            if (sk >= 0) {
+              const T q = bhT[ck] * ph.pyz.v[c] + chT[ck] * dEx;
+              stg<T>(psiB[psi_id(F_HY, 2, 0)], iz, q);
+              vy -= db * q;
+            }
+          }
+          // update_hz: i (ib,ie), j (jb,je), k [kb,ke); psi hz_x (-), hz_y (+)
+          T vz = hzc.v[c];
+          if (bz) {
+            const T dEx = ex - exj.v[c];
+            const T dEy = ey - eyp.v[c];
+            vz = vz + db * (dEx * ikyH - dEy * ikx);
+            if (si >= 0) { const T q = bhx * ph.pzx.v[c] + chx * dEy; qzx.v[c] = q; vz -= db * q; }
+            if (sjH >= 0) { const T q = bhj * ph.pzy.v[c] + chj * dEx; qzy.v[c] = q; vz += db * q; }
+          }
+          hxo.v[c] = vx;
+          hyo.v[c] = vy;
+          hzo.v[c] = vz;
+        }
+        const unsigned bo = ((unsigned)i * pl + jk) * es;
+        st4_nt<T>(ExB, bo, exn);
+        st4_nt<T>(EyB, bo, eyn);
+        st4_nt<T>(EzB, bo, ezn);
+        st4_nt<T>(HxB, bo, hxo);
+        st4_nt<T>(HyB, bo, hyo);
+        st4_nt<T>(HzB, bo, hzo);
+        if (si >= 0) {
+          st4_nt<T>(psiB[psi_id(F_HY, 0, 0)], ix, qyx);
+          st4_nt<T>(psiB[psi_id(F_HZ, 0, 0)], ix, qzx);
+        }
+        if (sjH >= 0) {
+          const unsigned iy = (((unsigned)i * N + djh) * nz + kq) * es;
+          st4_nt<T>(psiB[psi_id(F_HX, 1, sjH)], iy, qxy);
+          st4_nt<T>(psiB[psi_id(F_HZ, 1, sjH)], iy, qzy);
+        }
+        // port-plane capture: E at k = port, H at k = port and port + 1
+        if (jcap && i >= wx0 && i <= wx1) {
+          const size_t cc = (size_t)((i - wx0) * prow + (j - wy0));
+          const size_t base = (size_t)tstep * nplanes;
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            const int kk = k + c;
+            if (kk == port0 || kk == port1) {
+              const int cap = kk == port0 ? 0 : 1;
+              ring[(base + cap * 6 + 0) * pcells + cc] = exn.v[c];
+              ring[(base + cap * 6 + 1) * pcells + cc] = eyn.v[c];
+              if (port0 == port1) {
+                ring[(base + 6) * pcells + cc] = exn.v[c];
+                ring[(base + 7) * pcells + cc] = eyn.v[c];
+              }
+            }
+            if (kk == port0) { ring[(base + 2) * pcells + cc] = hxo.v[c]; ring[(base + 4) * pcells + cc] = hyo.v[c]; }
+            if (kk == port0 + 1) { ring[(base + 3) * pcells + cc] = hxo.v[c]; ring[(base + 5) * pcells + cc] = hyo.v[c]; }
+            if (kk == port1) { ring[(base + 8) * pcells + cc] = hxo.v[c]; ring[(base + 10) * pcells + cc] = hyo.v[c]; }
+            if (kk == port1 + 1) { ring[(base + 9) * pcells + cc] = hxo.v[c]; ring[(base + 11) * pcells + cc] = hyo.v[c]; }
+          }
+        }
+      }
+      // 5. advance the march
+      eyp = eyn;
+      ezp = ezn;
+      hxc = hxn; hyc = hyn; hzc = hzn;
+      hxn = hxn2; hyn = hyn2; hzn = hzn2;
+      tpx = tpxn; tpz = tpzn;
+      pc = pn;
+    }
+    __syncthreads();   // the next tile reuses the LDS rows
   }
 }
 

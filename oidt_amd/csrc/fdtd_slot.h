@@ -34,7 +34,6 @@ constexpr int kFB = 16;
 constexpr int kMatMax = 4;
 // Vacuum permeability of the reference configurations (mu = 4e-7 pi).
 constexpr double kMu0 = 4.0e-7 * 3.14159265358979323846;
-constexpr int kFused3dMaxXc = 16;  // x-planes of a fused 3D step tile (4 rows x 16 planes: one halo cell per lane)
 
 // SCHEME_TMZ_LOSSY: the 2D TMz update form of the loss and wave-impedance
 // study drivers (lossy E update, CPML over the whole border).

@@ -82,13 +82,19 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
     ('float64', 1, 4, 'f', {'sz': 70}),                        # the march form
     ('float64', 1, 8, 'b', {'source_condition': 'TM'}),
     ('float32', 1, 16, 'f', {'p1': 63, 'p2': 127}),
+    ('float64', 3, 4, 'f', {'sz': 70}),                        # the fused step: two waves per row
+    ('float64', 3, 16, 'b', {'source_condition': 'TM'}),
+    ('float32', 3, 8, 'f', {'p1': 63, 'p2': 127}),             # H samples at the z-wave boundary 128
+    ('float32', 3, 12, 'b', {'nz': 300, 'p2': 270}),           # three waves per row, one wave idle
+    ('float64', 3, 2, 'f', {'nz': 300, 'p2': 270}),            # FP64: two rows of three waves, two idle
 ])
 def test_3d_long_z_runs(precision, variant, xc, direction, extra):
     """A guide long enough in z (nz = 250) for four 64-cell z runs per row:
     runs wholly outside the z-CPML slabs, runs that hold a source slab or a
     port plane (E samples plane p, H samples p and p+1, here on both sides of
     a run boundary), and the partial last run all give the oracle's fields and
-    port DFT in the two-kernel 3D forms."""
+    port DFT in the two-kernel 3D forms and the fused step (whose rows span
+    two or three waves here, 128 z-cells each)."""
     p = configs.strip3d(250, nx=36, ny=34, nt=200, nf=6, precision=precision, width=12, height=8, port_len=10,
                         buffer_z=6, source_type=1, gauss_pulse_deg=-1.0)
     p.update(extra)
@@ -140,11 +146,24 @@ def test_launch_form_validation():
         t.close()
     d = Batch('3d', np.float32, max_slots=1)
     try:
+        d.set_launch(20, 0, 3)
         with pytest.raises(OidtError):
-            d.set_launch(20, 0, 3)
+            d.set_launch(0, 0, 1)       # keeps xchunk 20, which the two-kernel forms do not take
         d.set_launch(12, 0, 3)
     finally:
         d.close()
+    # the fused 3D step holds whole z rows in LDS: runs longer than its limit
+    # (FP64: 512 cells) are refused before anything is allocated
+    q = Batch('3d', np.float64, max_slots=1)
+    try:
+        q.set_launch(4, 0, 3)
+        p = configs.strip3d(520, nx=36, ny=34, nt=64, nf=2, precision='float64', width=12, height=8, port_len=10,
+                            buffer_z=6)
+        s, up, lo, fg = device_inputs(p)
+        with pytest.raises(OidtError):
+            q.add(s, fg)
+    finally:
+        q.close()
 
 
 @pytest.mark.gpu
