@@ -241,6 +241,24 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
 int oidt_batch_port_power(oidt_batch* b, int32_t slot, int32_t port, int32_t sub_slot, int32_t sub_port,
                           int32_t row0, int32_t nrows, double* re, double* im);
 
+/* Largest frequency count oidt_batch_port_power_noalloc takes. */
+#define OIDT_NOALLOC_MAX_NF 16384
+
+/* oidt_batch_port_power without any allocation: no hipMalloc, hipFree,
+ * stream creation or host heap use on its success path.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  The
+ * per-frequency sums pass through a fixed device array of the library
+ * (2 x OIDT_NOALLOC_MAX_NF doubles = 256 KiB per device, held until the
+ * process ends); runs with more frequencies are refused (OIDT_EINVAL).  The
+ * reduction is enqueued on the batch's own stream, after the work already
+ * enqueued there, and the call returns once re / im are written.  Every call
+ * on one device shares that array: calls that overlap on one device (from
+ * several host threads) race and must be serialised by the caller.
+ * Arguments and results as oidt_batch_port_power (identical values). */
+int oidt_batch_port_power_noalloc(oidt_batch* b, int32_t slot, int32_t port, int32_t sub_slot, int32_t sub_port,
+                                  int32_t row0, int32_t nrows, double* re, double* im);
+
 /* Attach a field monitor (oidt_monitor_desc) to a run that has not started;
  * *monitor receives its index within the run.  Monitors need a two-kernel
  * form (set_launch variant 1 or 2): the fused step has no point between the

@@ -81,6 +81,8 @@ SIGNATURES = {
     'oidt_batch_set_launch': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     'oidt_batch_port_power': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                         C.c_int32, C.c_void_p, C.c_void_p]),
+    'oidt_batch_port_power_noalloc': (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                                C.c_int32, C.c_void_p, C.c_void_p]),
     'oidt_batch_get_launch': (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     'oidt_batch_destroy': (C.c_int, [C.c_void_p]),
     'oidt_batch_profile': (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),

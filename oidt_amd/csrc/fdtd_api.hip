@@ -13,6 +13,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -284,6 +285,7 @@ uint64_t footprint(const oidt_run_desc& d) {
 
 struct oidt_batch {
   int scheme = 0, precision = 0, max_slots = 0;
+  int device = 0;              // device current at creation; the batch's stream and memory live there
   hipStream_t stream = nullptr;
   void* d_slots = nullptr;
   int* d_counts = nullptr;  // [0] stencil tiles, [1] DFT work items
@@ -835,6 +837,35 @@ int slot_ok(oidt_batch* b, int slot) {
   return OIDT_OK;
 }
 
+// Fixed device array for oidt_batch_port_power_noalloc: the per-frequency
+// (re, im) port powers of one call.  Each device holds its own copy for the
+// life of the process; calls on one device share it.
+constexpr int kNoallocMaxNf = OIDT_NOALLOC_MAX_NF;
+__device__ __align__(256) double scratch[2 * kNoallocMaxNf];
+
+// Device address of `scratch` per device, taken once with that device
+// current (the host-side name is not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for(int device, double** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(OIDT_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) HIP_TRY(hipSetDevice(device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess) return fail(OIDT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return OIDT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -882,6 +913,10 @@ int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt
   b->scheme = scheme;
   b->precision = precision;
   b->max_slots = max_slots;
+  if (hipGetDevice(&b->device) != hipSuccess) {
+    delete b;
+    return fail(OIDT_EHIP, "hipGetDevice failed");
+  }
   // measured defaults on MI355X: the prefetching march over 4-plane tiles in
   // 3D, the batched-load form with four cells per thread in 2D
   b->variant = scheme == SCHEME_3D ? 1 : 2;
@@ -1105,6 +1140,57 @@ int oidt_batch_port_power(oidt_batch* b, int32_t slot, int32_t port, int32_t sub
     re[f] = buf[2 * f];
     im[f] = buf[2 * f + 1];
   }
+  return OIDT_OK;
+}
+
+int oidt_batch_port_power_noalloc(oidt_batch* b, int32_t slot, int32_t port, int32_t sub_slot, int32_t sub_port,
+                                  int32_t row0, int32_t nrows, double* re, double* im) {
+  int rc = slot_ok(b, slot);
+  if (rc != OIDT_OK) return rc;
+  const SlotHost& sh = b->slots[slot];
+  const oidt_run_desc& d = sh.desc;
+  if (port < 0 || port > 1) return fail(OIDT_EINVAL, "bad port");
+  if (!re || !im) return fail(OIDT_EINVAL, "null output");
+  if (d.scheme == SCHEME_TMZ_LOSSY) return fail(OIDT_EINVAL, "the lossy TMz form has no port DFT");
+  if (d.nf > kNoallocMaxNf)
+    return fail(OIDT_EINVAL, "oidt_batch_port_power_noalloc takes at most " + std::to_string(kNoallocMaxNf) +
+                                 " frequencies; the run has " + std::to_string(d.nf));
+  const SlotHost* ss = nullptr;
+  if (sub_slot >= 0) {
+    if ((rc = slot_ok(b, sub_slot)) != OIDT_OK) return rc;
+    ss = &b->slots[sub_slot];
+    if (sub_port < 0 || sub_port > 1) return fail(OIDT_EINVAL, "bad port of the subtracted run");
+    if (ss->desc.nf != d.nf || ss->wcells != sh.wcells || ss->wyn != sh.wyn)
+      return fail(OIDT_EINVAL, "the subtracted run has other port planes or frequencies");
+  }
+  const int rowlen = d.scheme == SCHEME_3D ? sh.wyn : 1;
+  const int rows = rowlen > 0 ? sh.wcells / rowlen : 0;
+  if (row0 < 0 || nrows < 0 || row0 + nrows > rows) return fail(OIDT_EINVAL, "port window outside the DFT window");
+  if (d.nf == 0) return OIDT_OK;
+  double* out = nullptr;
+  if ((rc = scratch_for(b->device, &out)) != OIDT_OK) return rc;
+  const size_t off = d.precision == OIDT_F64 ? layout<double>(d).off_acc : layout<float>(d).off_acc;
+  const double* a = reinterpret_cast<const double*>(static_cast<char*>(sh.dmem) + off);
+  const double* sa = nullptr;
+  if (ss) {
+    const size_t soff = ss->desc.precision == OIDT_F64 ? layout<double>(ss->desc).off_acc
+                                                      : layout<float>(ss->desc).off_acc;
+    sa = reinterpret_cast<const double*>(static_cast<char*>(ss->dmem) + soff);
+  }
+  const int ncomp = sh.nq / 2;
+  // On the batch stream, after the work enqueued there: the accumulators are
+  // final once the runs' chunks have completed.  k_port_power writes every
+  // (re, im) pair of frequencies [0, nf) (one block per frequency), so
+  // nothing in `scratch` needs clearing first.
+  hipLaunchKernelGGL(k_port_power, dim3(d.nf), dim3(kThreads), 0, b->stream, a, sa, d.scheme, port * ncomp,
+                     ss ? sub_port * ncomp : 0, d.nf, sh.wcells, row0 * rowlen, (row0 + nrows) * rowlen, out);
+  HIP_TRY(hipGetLastError());
+  // split the interleaved (re, im) pairs straight into the caller's arrays
+  HIP_TRY(hipMemcpy2DAsync(re, sizeof(double), out, 2 * sizeof(double), sizeof(double), d.nf,
+                           hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipMemcpy2DAsync(im, sizeof(double), out + 1, 2 * sizeof(double), sizeof(double), d.nf,
+                           hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return OIDT_OK;
 }
 
