@@ -325,6 +325,11 @@ def bench_serving(args, count=None):
     total_cu = sum(r.cell_updates for r in reqs)
     plain_rate = total_cu / plain_wall / 1e6
     plain_rps = len(reqs) / plain_wall
+    # the plain leg's time per scheme (requests run back to back from t = 0)
+    by_scheme, prev = {}, 0.0
+    for r in sorted(reqs, key=lambda q: q.t_done):
+        by_scheme[r.setup.scheme] = by_scheme.get(r.setup.scheme, 0.0) + (r.t_done - prev)
+        prev = r.t_done
     # HBM budget: a quarter of the summed footprints of all requests, so that
     # admission waits for memory while many runs stay resident
     budget = int(sum(r.footprint for r in reqs) / 4)
@@ -379,6 +384,11 @@ def bench_serving(args, count=None):
         'plain_mcell_updates_per_s': round(plain_rate, 1), 'plain_requests_per_s': round(plain_rps, 3),
         'plain_wall_s': round(plain_wall, 3),
         'served_over_plain': round(sat['mcell_updates_per_s'] / plain_rate, 3),
+        # a 3D request of this mix already fills the GPU alone, so batching
+        # gains little on it: the ratio if the 2D requests cost nothing and
+        # the 3D ones ran as in the plain leg bounds what serving can reach
+        'plain_seconds_by_scheme': {k: round(v, 3) for k, v in by_scheme.items()},
+        'served_over_plain_bound_3d_as_plain': round(plain_wall / by_scheme.get('3d', plain_wall), 3),
         'saturated': dict(rnd(sat), admissions_deferred_for_memory=sat_deferred,
                           peak_resident_runs=srv.peak_resident, peak_resident_bytes=srv.peak_bytes),
         'poisson': dict(rnd(pois), rate_per_s=round(rate, 3),

@@ -945,7 +945,8 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
     return fail(OIDT_EINVAL, "run scheme/precision differs from the batch");
   if (b->variant == 3 && b->scheme == SCHEME_3D) {
     const int nzp = b->precision == OIDT_F64 ? padded<double>(*desc).nz : padded<float>(*desc).nz;
-    if (fused3d_rows(b->precision, nzp) == 0) return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64)");
+    if (fused3d_rows(b->precision, nzp) == 0 || desc->num_cpml > kCpmlMax)
+      return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64) and num_cpml <= 64");
   }
   int si = -1;
   for (int i = 0; i < b->max_slots; ++i)
@@ -985,8 +986,8 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
   if (variant == 3 && b->scheme == SCHEME_3D)
     for (const auto& sh : b->slots)
-      if (sh.used && fused3d_rows(b->precision, sh.nzp) == 0)
-        return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64)");
+      if (sh.used && (fused3d_rows(b->precision, sh.nzp) == 0 || sh.desc.num_cpml > kCpmlMax))
+        return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64) and num_cpml <= 64");
   if (variant == 3 && b->scheme == SCHEME_TMZ_LOSSY)
     return fail(OIDT_EINVAL, "the lossy TMz form has no fused step (variant 1 or 2)");
   if (variant == 3 && b->has_monitors())

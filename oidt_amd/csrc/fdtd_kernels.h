@@ -2149,44 +2149,76 @@ struct Fused3d {
   static constexpr int LDS = max_cells(1);
 };
 
-// Two consecutive z-cells of one field.
+// The lane's consecutive z-cells of one field.
 template <typename T>
 struct Q4 {
-  T v[2];
+  T v[Fused3d<T>::C];
 };
 
 typedef float oidt_f32x2 __attribute__((ext_vector_type(2)));
+typedef float oidt_f32x4 __attribute__((ext_vector_type(4)));
 typedef double oidt_f64x2 __attribute__((ext_vector_type(2)));
-template <typename T>
-using vec2_t = typename std::conditional<sizeof(T) == 4, oidt_f32x2, oidt_f64x2>::type;
+
+// C cells as 8- or 16-byte vector accesses (2 x 16 bytes for 4 doubles).
+template <typename T, typename F>
+__device__ __forceinline__ void q4_chunks(F&& f) {
+  constexpr int C = Fused3d<T>::C;
+  if constexpr (C * sizeof(T) == 8) f(oidt_f32x2{}, 0);
+  else if constexpr (sizeof(T) == 4) f(oidt_f32x4{}, 0);
+  else {
+    f(oidt_f64x2{}, 0);
+    if constexpr (C == 4) f(oidt_f64x2{}, 2);
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ Q4<T> ld4(const T* base, unsigned boff) {
-  const vec2_t<T> a = *(const OIDT_GLOBAL vec2_t<T>*)(reinterpret_cast<const char*>(base) + boff);
   Q4<T> q;
-  q.v[0] = a.x;
-  q.v[1] = a.y;
+  q4_chunks<T>([&](auto v, int c0) {
+    using V = decltype(v);
+    constexpr int n = sizeof(V) / sizeof(T);
+    const V a = *(const OIDT_GLOBAL V*)(reinterpret_cast<const char*>(base) + boff + c0 * sizeof(T));
+#pragma unroll
+    for (int c = 0; c < n; ++c) q.v[c0 + c] = a[c];
+  });
   return q;
 }
 
 // Non-temporal store of the lane's cells (nothing in the same launch reads them).
 template <typename T>
 __device__ __forceinline__ void st4_nt(T* base, unsigned boff, const Q4<T>& q) {
-  const vec2_t<T> a = {q.v[0], q.v[1]};
-  __builtin_nontemporal_store(a, (OIDT_GLOBAL vec2_t<T>*)(reinterpret_cast<char*>(base) + boff));
+  q4_chunks<T>([&](auto v, int c0) {
+    using V = decltype(v);
+    constexpr int n = sizeof(V) / sizeof(T);
+    V a;
+#pragma unroll
+    for (int c = 0; c < n; ++c) a[c] = q.v[c0 + c];
+    __builtin_nontemporal_store(a, (OIDT_GLOBAL V*)(reinterpret_cast<char*>(base) + boff + c0 * sizeof(T)));
+  });
 }
 
 template <typename T>
 __device__ __forceinline__ void lds_put4(T* p, const Q4<T>& q) {
-  *reinterpret_cast<vec2_t<T>*>(p) = vec2_t<T>{q.v[0], q.v[1]};
+  q4_chunks<T>([&](auto v, int c0) {
+    using V = decltype(v);
+    constexpr int n = sizeof(V) / sizeof(T);
+    V a;
+#pragma unroll
+    for (int c = 0; c < n; ++c) a[c] = q.v[c0 + c];
+    *reinterpret_cast<V*>(p + c0) = a;
+  });
 }
 
 template <typename T>
 __device__ __forceinline__ Q4<T> lds_get4(const T* p) {
-  const vec2_t<T> a = *reinterpret_cast<const vec2_t<T>*>(p);
   Q4<T> q;
-  q.v[0] = a.x;
-  q.v[1] = a.y;
+  q4_chunks<T>([&](auto v, int c0) {
+    using V = decltype(v);
+    constexpr int n = sizeof(V) / sizeof(T);
+    const V a = *reinterpret_cast<const V*>(p + c0);
+#pragma unroll
+    for (int c = 0; c < n; ++c) q.v[c0 + c] = a[c];
+  });
   return q;
 }
 
@@ -2207,8 +2239,22 @@ struct H3Old {
 template <typename T>
 __device__ __forceinline__ Q4<T> q4_zero() {
   Q4<T> q;
-  q.v[0] = q.v[1] = T(0);
+#pragma unroll
+  for (int c = 0; c < Fused3d<T>::C; ++c) q.v[c] = T(0);
   return q;
+}
+
+// Workgroup barrier for the LDS row buffers only: the fences order this
+// wave's LDS accesses around the barrier (a wait on the LDS counter), not its
+// global loads and stores.  No wave of a fused 3D block reads what another
+// writes to global memory (reads come from field set A, writes go to set B),
+// so the next plane's loads stay in flight across the barrier instead of
+// being waited for as a full __syncthreads() would (its release fence waits
+// for every outstanding global access of the wave).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <typename T>
@@ -2221,6 +2267,10 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
   constexpr int ZSEG = Fused3d<T>::ZSEG;
   constexpr unsigned es = sizeof(T);
   __shared__ T lds[Fused3d<T>::LDS];
+  // the run's CPML coefficients (be, ce/dx, bh, ch/dx), staged per tile: the
+  // z-slab cells read them per cell, and an LDS read waits only on the LDS
+  // counter, not on the plane loads in flight
+  __shared__ T cpml[4][kCpmlMax];
   const int ntiles = counts[0];
   const int lane = threadIdx.x;
   const int w = wave_id();
@@ -2313,10 +2363,16 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
     const int port0 = s.port[0], port1 = s.port[1];
     T* const ring = s.ring;
     const unsigned xhi = s.psi_hi[0], zhi = s.psi_hi[2];
-    const T* const beT = s.be;
-    const T* const ceT = s.ceo;
-    const T* const bhT = s.bh;
-    const T* const chT = s.cho;
+    if (w == 0 && lane < N) {
+      cpml[0][lane] = s.be[lane];
+      cpml[1][lane] = s.ceo[lane];
+      cpml[2][lane] = s.bh[lane];
+      cpml[3][lane] = s.cho[lane];
+    }
+    const T* const beT = cpml[0];
+    const T* const ceT = cpml[1];
+    const T* const bhT = cpml[2];
+    const T* const chT = cpml[3];
 
     // x slab of plane i: side (-1: none), offset of the lane's cells in the slab arrays
     auto xslab = [&](int i, unsigned& ix) {
@@ -2339,7 +2395,8 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
       p.ex = ld4<T>(ExA, bo);
       p.ey = ld4<T>(EyA, bo);
       p.ez = ld4<T>(EzA, bo);
-      p.m = *(const OIDT_GLOBAL unsigned short*)(reinterpret_cast<const char*>(mat) + idx);
+      if constexpr (C == 2) p.m = *(const OIDT_GLOBAL unsigned short*)(reinterpret_cast<const char*>(mat) + idx);
+      else p.m = *(const OIDT_GLOBAL unsigned*)(reinterpret_cast<const char*>(mat) + idx);
       unsigned ix;
       if (xslab(i, ix) >= 0) {
         p.pyx = ld4<T>(psiA[psi_id(F_EY, 0, 0)], ix);
@@ -2403,8 +2460,12 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
     };
 
     // new E of plane i for the lane's cells; `store`: write its psi to set B
-    auto updE = [&](int i, const E3Old<T>& p, const Q4<T>& hx, const Q4<T>& hy, const Q4<T>& hz,
-                    const Q4<T>& hy1, const Q4<T>& hz1, bool store, Q4<T>& ex, Q4<T>& ey, Q4<T>& ez) {
+    auto updE = [&](int i, const E3Old<T>& p, const Q4<T>& hy1, const Q4<T>& hz1, bool store, Q4<T>& ex, Q4<T>& ey,
+                    Q4<T>& ez) {
+      // the lane's own old H, from the LDS row it wrote
+      const Q4<T> hx = lds_get4(sH + (0 * (R + 1) + r) * NZS + k);
+      const Q4<T> hy = lds_get4(sH + (1 * (R + 1) + r) * NZS + k);
+      const Q4<T> hz = lds_get4(sH + (2 * (R + 1) + r) * NZS + k);
       const int u = i - (i0 - 1);
       const T ikx = lane_get(lx_ikE, u), bex = lane_get(lx_bE, u), cex = lane_get(lx_cE, u);
       unsigned ix;
@@ -2491,7 +2552,7 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
       }
     };
 
-    Q4<T> hxc, hyc, hzc;        // old H of the plane being updated
+    Q4<T> hxc, hyc, hzc;        // old H of the plane being updated (to the LDS rows)
     Q4<T> hxn, hyn, hzn;        // old H of the next plane
     Q4<T> tpx, tpz;             // old Hx, Hz of the row above the tile, plane being updated
     Q4<T> eyp, ezp;             // new Ey, Ez of the previous plane
@@ -2516,10 +2577,10 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
           lds_put4(sH + (2 * (R + 1) + R) * NZS + k, tpz);
         }
       }
-      __syncthreads();
+      lds_barrier();
       // 2. new E of plane i
       Q4<T> exn, eyn, ezn;
-      if (inrow) updE(i, pc, hxc, hyc, hzc, hyn, hzn, owner && !pro, exn, eyn, ezn);
+      if (inrow) updE(i, pc, hyn, hzn, owner && !pro, exn, eyn, ezn);
       // 3. new E to the LDS rows; H psi of plane i and the loads of plane i + 1
       if (inrow) {
         lds_put4(sE + (0 * R + r) * NZS + k, exn);
@@ -2539,7 +2600,7 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
         }
         if (toprow) loadTop(i + 1, tpxn, tpzn);
       }
-      __syncthreads();
+      lds_barrier();
       if (hrow_) {
         // 4. new H of plane i, tile rows
         const int u = i - (i0 - 1);
@@ -2556,6 +2617,10 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
         const T exk0 = sE[(0 * R + r) * NZS + kp], eyk0 = sE[(1 * R + r) * NZS + kp];
         Q4<T> hxo, hyo, hzo;
         Q4<T> qyx = ph.pyx, qzx = ph.pzx, qxy = ph.pxy, qzy = ph.pzy;
+        // the lane's own old H, from the LDS row it wrote
+        const Q4<T> hxc = lds_get4(sH + (0 * (R + 1) + r) * NZS + k);
+        const Q4<T> hyc = lds_get4(sH + (1 * (R + 1) + r) * NZS + k);
+        const Q4<T> hzc = lds_get4(sH + (2 * (R + 1) + r) * NZS + k);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const int kk = k + c;
@@ -2658,7 +2723,7 @@ __global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* 
       tpx = tpxn; tpz = tpzn;
       pc = pn;
     }
-    __syncthreads();   // the next tile reuses the LDS rows
+    lds_barrier();   // the next tile reuses the LDS rows
   }
 }
 
