@@ -284,8 +284,10 @@ int oidt_batch_remove(oidt_batch* b, int32_t slot);
  * (0 keeps the current one; 1 = per-plane march, the 3D default, which in 3D
  * issues the next plane's loads before updating the current one; 2 = batched
  * loads per x-chunk, the 2D default; 3 = fused Yee step, one launch per step,
- * all schemes, xchunk also 12, 20 or 24, at most 16 in 3D).  The kernel form of a run cannot change between the fused
- * and the two-kernel forms once the run has started (OIDT_ESTATE).
+ * all schemes, xchunk also 12, 20 or 24, at most 16 in 3D; 4 = temporally
+ * blocked step, TMz only, xchunk = steps per launch, 2, 4 or 8).  Once a run
+ * has started, its form cannot change between two-kernel, fused and blocked,
+ * nor the blocked step's depth (OIDT_ESTATE).
  * No counterpart in the reference (fixed launch shapes, fdtd_main.py:263-273). */
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant);
 

@@ -24,10 +24,9 @@ from .solver import Batch
 # (kernel form, x-chunk depth, grid cap; 0 = library default)
 CANDIDATES = {
     'tm': [(2, 4, 0), (2, 4, 65536), (2, 8, 0), (3, 1, 0), (3, 2, 0), (3, 8, 0), (3, 12, 0), (3, 16, 0),
-           (3, 20, 0), (3, 24, 0), (3, 32, 0)],
+           (3, 20, 0), (3, 24, 0), (3, 32, 0), (4, 2, 0), (4, 4, 0), (4, 8, 0)],
     'te': [(2, 2, 0), (2, 4, 0), (2, 4, 65536), (2, 8, 0), (1, 4, 0), (3, 1, 0), (3, 2, 0), (3, 8, 0), (3, 16, 0)],
-    '3d': [(1, 2, 0), (1, 4, 0), (1, 4, 65536), (1, 8, 0), (2, 1, 0), (2, 1, 65536), (3, 8, 0), (3, 16, 0),
-           (3, 32, 0)],
+    '3d': [(1, 2, 0), (1, 4, 0), (1, 4, 65536), (1, 8, 0), (2, 1, 0), (2, 1, 65536), (3, 16, 0)],
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'oidt_amd', 'autotune.json')

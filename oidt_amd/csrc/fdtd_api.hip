@@ -99,7 +99,11 @@ struct SlotHost {
   size_t bytes = 0;
   std::vector<unsigned char> slot;  // host image of Slot<T>
   int64_t admit_chunk = 0;          // batch chunk counter when admitted
-  bool fused = false;               // advanced by the fused step kernel (alternating field sets)
+  // Steps between swaps of the run's two field sets: 0 for the two-kernel
+  // forms (one set), 1 for the fused step, the launch depth for the
+  // temporally blocked step.  After `steps` completed steps the state is in
+  // set B when ceil(steps / tb) is odd.
+  int tb = 0;
   int nplanes = 0, nq = 0, wcells = 0, pcells = 0, wyn = 0;
   int nzp = 0;                      // padded z extent (3D)
   std::vector<MonHost> mons;
@@ -507,9 +511,17 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
   return OIDT_OK;
 }
 
+// SlotHost::tb of a run stepped by a launch form.
+int run_tb(int variant, int xchunk) { return variant == 4 ? xchunk : (variant >= 3 ? 1 : 0); }
+
 // Kernels whose tiles are per wavefront (four tiles per block): the fused
 // 2D steps.
-bool march_tiles(const oidt_batch* b) { return b->variant == 3 && b->scheme != SCHEME_3D; }
+bool march_tiles(const oidt_batch* b) {
+  return (b->variant == 3 && b->scheme != SCHEME_3D) || b->variant == 5;
+}
+
+// z-cells a wave of the 3D halo march owns (its lane 0 is the cell below).
+constexpr int kHaloZ = kBX - 1;
 
 // Rows of a fused 3D step block (the halo row included) for a run of padded z
 // extent nzp, or 0 when its LDS rows do not fit (see k_3d_fused).
@@ -585,6 +597,16 @@ int rebuild_lists(oidt_batch* b) {
       const int rows = fused3d_rows(b->precision, sh.nzp) - 1;
       for (int x0 = 0; x0 < d.nx; x0 += xc)
         for (int y0 = 0; y0 < d.ny; y0 += rows) tiles.push_back(make_int4(si, x0, y0, 0));
+    } else if (b->variant == 5) {
+      // 3D halo march: one wavefront per (row, 63 z-cells) x xc planes
+      for (int x0 = 0; x0 < d.nx; x0 += xc)
+        for (int y0 = 0; y0 < d.ny; ++y0)
+          for (int z0 = 0; z0 < d.nz; z0 += kHaloZ) tiles.push_back(make_int4(si, x0, y0, z0));
+    } else if (b->variant == 4) {
+      // blocked step: one block per tile of its inner cells
+      const int tx = kTbW * kTbK - 2 * xc, ty = 64 - 2 * xc;
+      for (int x0 = 0; x0 < d.nx; x0 += tx)
+        for (int y0 = 0; y0 < d.ny; y0 += ty) tiles.push_back(make_int4(si, x0, y0, 0));
     } else if (d.scheme == SCHEME_3D) {
       for (int x0 = 0; x0 < d.nx; x0 += xc)
         for (int y0 = 0; y0 < d.ny; y0 += kBY)
@@ -676,8 +698,45 @@ void launch_fused(oidt_batch* b, int gs, int tstep) {
   }
 }
 
+// The temporally blocked TMz step: xchunk steps per launch.
+template <typename T>
+void launch_blocked(oidt_batch* b, int gs, int tstep) {
+  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
+  const dim3 blk(64, kTbW), grd(gs);
+  switch (b->xchunk) {
+    case 2: hipLaunchKernelGGL((k_tmz_tb<T, 2>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+    case 4: hipLaunchKernelGGL((k_tmz_tb<T, 4>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+    default: hipLaunchKernelGGL((k_tmz_tb<T, 8>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+  }
+}
+
+// Steps one stencil launch advances: the blocked step's depth, else 1.
+int launch_steps(const oidt_batch* b) { return b->variant == 4 ? b->xchunk : 1; }
+
+// The 3D halo march: the whole step in one launch, xchunk planes per wave.
+template <typename T>
+void launch_halo(oidt_batch* b, int gs, int tstep) {
+  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
+  const dim3 blk(kBX, kBY), grd(gs);
+  switch (b->xchunk) {
+    case 1: hipLaunchKernelGGL((k_3d_halo<T, 1>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+    case 2: hipLaunchKernelGGL((k_3d_halo<T, 2>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+    case 4: hipLaunchKernelGGL((k_3d_halo<T, 4>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+    case 16: hipLaunchKernelGGL((k_3d_halo<T, 16>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+    default: hipLaunchKernelGGL((k_3d_halo<T, 8>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
+  }
+}
+
 template <typename T>
 void launch_stencil(oidt_batch* b, int gs, int tstep, bool e_half) {
+  if (b->variant == 4) {
+    if (e_half) launch_blocked<T>(b, gs, tstep);
+    return;
+  }
+  if (b->variant == 5) {
+    if (e_half) launch_halo<T>(b, gs, tstep);
+    return;
+  }
   if (b->variant == 3) {
     if (!e_half) return;
     switch (b->xchunk) {
@@ -737,10 +796,10 @@ void launch_monitors(oidt_batch* b, int t, int h) {
 template <typename T>
 int enqueue_chunk(oidt_batch* b) {
   const int gs = stencil_grid(b), gd = dft_grid(b);
-  for (int t = 0; t < kChunk; ++t) {
+  for (int t = 0; t < kChunk; t += launch_steps(b)) {
     launch_stencil<T>(b, gs, t, true);
     launch_monitors<T>(b, t, 0);
-    if (b->variant != 3) launch_stencil<T>(b, gs, t, false);
+    if (b->variant < 3) launch_stencil<T>(b, gs, t, false);
     launch_monitors<T>(b, t, 1);
   }
   launch_dft<T>(b, gd);
@@ -767,9 +826,9 @@ int profile_chunks(oidt_batch* b, int nchunks, double* out) {
   };
   int rc;
   for (int c = 0; c < nchunks; ++c) {
-    for (int t = 0; t < kChunk; ++t)
+    for (int t = 0; t < kChunk; t += launch_steps(b))
       for (int h = 0; h < 2; ++h) {
-        if (!(b->variant == 3 && h == 1) &&
+        if (!(b->variant >= 3 && h == 1) &&
             (rc = timed(h, [&] { launch_stencil<T>(b, gs, t, h == 0); })) != OIDT_OK) return rc;
         if (b->nmwork[h] > 0 && (rc = timed(4 + h, [&] { launch_monitors<T>(b, t, h); })) != OIDT_OK) return rc;
       }
@@ -965,7 +1024,7 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
   if (rc != OIDT_OK) return rc;
   sh.used = true;
   sh.admit_chunk = b->chunks_enqueued;
-  sh.fused = b->variant == 3;
+  sh.tb = run_tb(b->variant, b->xchunk);
   b->dirty = true;
   *slot = si;
   return OIDT_OK;
@@ -973,12 +1032,21 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
-  if (variant < 0 || variant > 3) return fail(OIDT_EINVAL, "variant must be 0 (keep), 1, 2 or 3");
+  if (variant < 0 || variant > 5) return fail(OIDT_EINVAL, "variant must be 0 (keep), 1, 2, 3, 4 or 5");
   if (variant == 0) variant = b->variant;
-  for (const auto& sh : b->slots)
-    if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.fused != (variant == 3))
-      return fail(OIDT_ESTATE, "cannot switch between fused and two-kernel forms while a run is in progress");
   const int xc = xchunk ? xchunk : b->xchunk;   // the depth this launch form will use
+  if (variant == 4) {
+    if (b->scheme != SCHEME_TMZ) return fail(OIDT_EINVAL, "the blocked step (variant 4) is for TMz runs");
+    if (xc != 2 && xc != 4 && xc != 8) return fail(OIDT_EINVAL, "the blocked step takes 2, 4 or 8 steps per launch");
+  }
+  if (variant == 5) {
+    if (b->scheme != SCHEME_3D) return fail(OIDT_EINVAL, "the halo march (variant 5) is for 3D runs");
+    if (xc != 1 && xc != 2 && xc != 4 && xc != 8 && xc != 16)
+      return fail(OIDT_EINVAL, "the halo march takes 1, 2, 4, 8 or 16 planes per wave");
+  }
+  for (const auto& sh : b->slots)
+    if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.tb != run_tb(variant, xc))
+      return fail(OIDT_ESTATE, "cannot change how a run in progress swaps field sets (two-kernel, fused or blocked form and depth)");
   const bool pow2 = xc == 1 || xc == 2 || xc == 4 || xc == 8 || xc == 16 || xc == 32;
   const bool fused_only = xc == 12 || xc == 20 || xc == 24;
   if (!pow2 && !(fused_only && variant == 3))
@@ -990,11 +1058,11 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
         return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64) and num_cpml <= 64");
   if (variant == 3 && b->scheme == SCHEME_TMZ_LOSSY)
     return fail(OIDT_EINVAL, "the lossy TMz form has no fused step (variant 1 or 2)");
-  if (variant == 3 && b->has_monitors())
+  if (variant >= 3 && b->has_monitors())
     return fail(OIDT_ESTATE, "field monitors need a two-kernel form (variant 1 or 2)");
   b->variant = variant;
   for (auto& sh : b->slots)
-    if (sh.used && b->chunks_enqueued == sh.admit_chunk) sh.fused = variant == 3;
+    if (sh.used && b->chunks_enqueued == sh.admit_chunk) sh.tb = run_tb(variant, xc);
   if (xchunk) b->xchunk = xchunk;
   b->grid_override = grid_blocks;
   b->dirty = true;
@@ -1206,10 +1274,9 @@ int oidt_batch_read_field(oidt_batch* b, int32_t slot, int32_t field, void* host
   if (!host || nbytes != bytes) return fail(OIDT_EINVAL, "host buffer size mismatch");
   HIP_TRY(hipStreamSynchronize(b->stream));
   reap_marks(b);
-  // The fused step alternates field sets: after an odd number of completed
-  // steps the state is in set B.
+  // The fused and blocked steps alternate field sets (see SlotHost::tb).
   const int64_t steps = std::min<int64_t>(d.nt, std::max<int64_t>(0, b->chunks_done - sh.admit_chunk) * kChunk);
-  const bool in_b = sh.fused && (steps & 1);
+  const bool in_b = sh.tb > 0 && (((steps + sh.tb - 1) / sh.tb) & 1);
   const size_t off = d.precision == OIDT_F64 ? (in_b ? layout<double>(d).off_f2[field] : layout<double>(d).off_f[field])
                                              : (in_b ? layout<float>(d).off_f2[field] : layout<float>(d).off_f[field]);
   const bool is3d = d.scheme == SCHEME_3D;
@@ -1245,7 +1312,7 @@ int oidt_batch_add_monitor(oidt_batch* b, int32_t slot, const oidt_monitor_desc*
   SlotHost& sh = b->slots[slot];
   const oidt_run_desc& d = sh.desc;
   if (b->chunks_enqueued != sh.admit_chunk) return fail(OIDT_ESTATE, "monitors are attached before the run starts");
-  if (b->variant == 3) return fail(OIDT_ESTATE, "field monitors need a two-kernel form (variant 1 or 2)");
+  if (b->variant >= 3) return fail(OIDT_ESTATE, "field monitors need a two-kernel form (variant 1 or 2)");
   if (m->field < 0 || m->field > 5 || !field_used(d.scheme, m->field))
     return fail(OIDT_EINVAL, "monitor field not present in this scheme");
   if ((m->after_h != 0 && m->after_h != 1) || (m->mode != OIDT_MON_DFT && m->mode != OIDT_MON_CAPTURE))

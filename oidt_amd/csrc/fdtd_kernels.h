@@ -2104,6 +2104,614 @@ __global__ __launch_bounds__(kThreads) void k_tez_step(const Slot<T>* __restrict
   }
 }
 
+// Workgroup barrier for LDS exchanges only: the fences order this wave's
+// LDS accesses around the barrier (a wait on the LDS counter), not its global
+// loads and stores.  In the kernels that use it no wave reads what another
+// writes to global memory in the same launch (reads come from field set A,
+// writes go to set B), so loads stay in flight across the barrier instead of
+// being waited for as a full __syncthreads() would (its release fence waits
+// for every outstanding global access of the wave).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// ------------------------------------------------ temporally blocked TMz ---
+// TB Yee steps per launch.  A block of kTbW waves holds an extended tile of
+// kTbW * kTbK planes (x) by 64 rows (y, one per lane) in registers: wave w
+// owns planes ib0 + w*kTbK .. + kTbK - 1, every lane its row of those
+// planes (Ez, Hx, Hy, the four psi values and the material byte of each).
+// The tile's own cells are the inner (kTbW*kTbK - 2 TB) x (64 - 2 TB); the
+// TB-cell ghost frame on every side is loaded and stepped like the rest and
+// discarded: each step (E then H) moves the region that depends on values
+// outside the tile one cell inwards from each side, so after TB steps the
+// inner cells are exact.  Per step the only exchanges are the y neighbours
+// (DPP lane shifts) and, between waves, one plane of old Hy (for the E
+// update of the next-lower wave's last plane) and one plane of new Ez (for
+// the H update of the next-higher wave's first plane) through LDS, each
+// behind a barrier.  The per-cell arithmetic is k_tmz_step's (tmz_e, then
+// the Hx and Hy updates in its order), so results are bit-identical to TB
+// launches of the one-step form.  The launch reads field/psi set
+// ((n0 / TB) & 1) and writes the inner cells' state after its steps to the
+// other set.
+// HBM traffic per cell and launch: every field and psi value read once
+// (ghost frame included) and written once, for TB steps.
+constexpr int kTbW = 8;    // waves per block
+constexpr int kTbK = 4;    // planes per wave
+
+__device__ __forceinline__ float lane_down1(float v, float hi) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(hi), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_down1(double v, double hi) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(hi);
+  const int blo = __builtin_amdgcn_update_dpp((int)o, (int)b, 0x130, 0xf, 0xf, false);
+  const int bhi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), 0x130, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)bhi << 32) | (unsigned)blo);
+}
+
+template <typename T, int TB>
+__global__ __launch_bounds__(64 * kTbW) void k_tmz_tb(const Slot<T>* __restrict__ slots,
+                                                      const int4* __restrict__ tiles,
+                                                      const int* __restrict__ counts, int tstep) {
+  constexpr int W = kTbW, K = kTbK;
+  constexpr int TX = W * K - 2 * TB, TY = 64 - 2 * TB;   // inner cells of a tile
+  static_assert(TX > 0 && TY > 0 && kChunk % TB == 0, "blocking depth");
+  constexpr unsigned es = sizeof(T);
+  __shared__ T xhy[W][64];   // old Hy of each wave's first plane
+  __shared__ T xez[W][64];   // new Ez of each wave's last plane
+  const int ntiles = counts[0];
+  const int lane = threadIdx.x;
+  const int w = wave_id();
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n0 = s.base + tstep;
+    if (!s.active || n0 >= s.nt) continue;      // uniform over the block
+    const int nsteps = min(TB, s.nt - n0);
+    const TmzView<T> v = tmz_view(s);
+    const bool odd = ((n0 / TB) & 1) != 0;
+    const T* __restrict__ EzA = odd ? s.f2[F_EZ] : s.f[F_EZ];
+    const T* __restrict__ HxA = odd ? s.f2[F_HX] : s.f[F_HX];
+    const T* __restrict__ HyA = odd ? s.f2[F_HY] : s.f[F_HY];
+    T* __restrict__ EzB = odd ? s.f[F_EZ] : s.f2[F_EZ];
+    T* __restrict__ HxB = odd ? s.f[F_HX] : s.f2[F_HX];
+    T* __restrict__ HyB = odd ? s.f[F_HY] : s.f2[F_HY];
+    OIDT_GLOBAL T* const* psiAll = odd ? s.psi2 : s.psi;
+    OIDT_GLOBAL T* const* psiBll = odd ? s.psi : s.psi2;
+    const int nx = v.nx, ny = v.ny, N = v.N;
+    const int i0 = tl.y, j0 = tl.z;               // first inner cell
+    const int ib0 = i0 - TB + w * K;              // this wave's first plane
+    const int j = j0 - TB + lane;
+    const bool jok = j >= 0 && j < ny;
+    const int jc = min(max(j, 0), ny - 1);
+    const bool jown = jok && j >= j0 && j < j0 + TY;
+    // row (y-only) constants of this lane
+    const bool jbulk = jok && j >= v.jb && j < v.je;          // Ez and Hy rows
+    const bool jin1 = jok && j > v.jb && j < v.je;            // Hx rows
+    const bool jsrc = jok && v.src_ez && j >= v.sy0 && j < v.sy1;
+    int dye = 0, cye = 0, dyh = 0, cyh = 0;
+    const int sye = jbulk ? slab_e(j, v.jb, v.je, N, dye, cye) : -1;
+    const int syh = jin1 ? slab_h(j, v.jb, v.je, N, dyh, cyh) : -1;
+    TmzJ<T> jcf;
+    jcf.iky = jbulk ? v.ike1[j] : T(0);
+    jcf.bey = sye >= 0 ? v.be[cye] : T(0);
+    jcf.cey = sye >= 0 ? v.ceo[cye] : T(0);
+    const T ikyh = jin1 ? v.ikh1[j] : T(0);
+    const T bhy = syh >= 0 ? v.bh[cyh] : T(0), chy = syh >= 0 ? v.cho[cyh] : T(0);
+    const bool jcap = jown && j >= v.wy0 && j <= v.wy1;
+    const T* pzyA = sye >= 0 ? psiAll[psi_id(F_EZ, 1, sye)] : nullptr;
+    T* pzyB = sye >= 0 ? psiBll[psi_id(F_EZ, 1, sye)] : nullptr;
+    const T* pxyA = syh >= 0 ? psiAll[psi_id(F_HX, 1, syh)] : nullptr;
+    T* pxyB = syh >= 0 ? psiBll[psi_id(F_HX, 1, syh)] : nullptr;
+    // ---- the wave's cells, from set A (outside the grid: zero, never updated)
+    T ez[K], hx[K], hy[K], pzx[K], pzy[K], qxy[K], qyx[K];
+    unsigned mm[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = ib0 + q;
+      const bool ok = jok && i >= 0 && i < nx;
+      const TmzX x = tmz_x(v, i);
+      const unsigned idx = (unsigned)min(max(i, 0), nx - 1) * ny + jc;
+      ez[q] = ok ? ldg<T>(EzA, idx * es) : T(0);
+      hx[q] = ok ? ldg<T>(HxA, idx * es) : T(0);
+      hy[q] = ok ? ldg<T>(HyA, idx * es) : T(0);
+      mm[q] = ok ? (unsigned)ldg<uint8_t>(v.mat, idx) : 0u;
+      pzx[q] = ok && x.ibulk && jbulk && x.sxe >= 0 ? psiAll[psi_id(F_EZ, 0, x.sxe)][(size_t)x.dxe * ny + jc] : T(0);
+      pzy[q] = ok && x.ibulk && sye >= 0 ? pzyA[(size_t)i * N + dye] : T(0);
+      qxy[q] = ok && x.ibulk && syh >= 0 ? pxyA[(size_t)i * N + dyh] : T(0);
+      qyx[q] = ok && x.hbulk && jbulk && x.sxh >= 0 ? psiAll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + jc] : T(0);
+    }
+    // ---- per-plane constants, fixed over the launch: flags (wave-uniform)
+    // and coefficients (lane q holds plane ib0 + q); the cb of each cell;
+    // the source waveform (lane t holds step n0 + t)
+    unsigned pf[K];
+    T cbz[K];
+    T lk_ikx = T(0), lk_bex = T(0), lk_cex = T(0), lk_ikh = T(0), lk_bhx = T(0), lk_chx = T(0);
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = ib0 + q;
+      const TmzX x = tmz_x(v, i);
+      const bool iok = i >= 0 && i < nx;
+      const bool inner = iok && i >= i0 && i < i0 + TX;
+      pf[q] = (x.ibulk ? 1u : 0u) | (x.hbulk ? 2u : 0u) | (iok && x.isrc ? 4u : 0u) | (x.sxe >= 0 ? 8u : 0u) |
+              (x.sxh >= 0 ? 16u : 0u) | (inner && i == v.port0 ? 32u : 0u) | (inner && i == v.port1 ? 64u : 0u) |
+              (inner && i == v.port0 + 1 ? 128u : 0u) | (inner && i == v.port1 + 1 ? 256u : 0u);
+      if (lane == q) {
+        lk_ikx = x.ibulk ? v.ike0[i] : T(0);
+        lk_bex = x.sxe >= 0 ? v.be[x.cxe] : T(0);
+        lk_cex = x.sxe >= 0 ? v.ceo[x.cxe] : T(0);
+        lk_ikh = x.hbulk ? v.ikh0[i] : T(0);
+        lk_bhx = x.sxh >= 0 ? v.bh[x.cxh] : T(0);
+        lk_chx = x.sxh >= 0 ? v.cho[x.cxh] : T(0);
+      }
+      const int qm = (mm[q] >> 4) & 3;
+      cbz[q] = qm == 0 ? v.cb[0] : (qm == 1 ? v.cb[1] : (qm == 2 ? v.cb[2] : v.cb[3]));
+    }
+    const T lk_jv = v.src_ez && lane < nsteps ? s.jsrc[n0 + lane] : T(0);
+    const T db = v.db;
+    T* const ring = v.ring;
+    const int pcells = v.pcells;
+    const size_t cw = (size_t)(j - v.wy0);
+    for (int t = 0; t < nsteps; ++t) {
+      const T jv = lane_get(lk_jv, t);
+      const size_t rbase = (size_t)(tstep + t) * v.nplanes;
+      // ---- E half-step (tmz_e per cell): Hy(i+1) of the last plane from the next wave
+      xhy[w][lane] = hy[0];
+      lds_barrier();
+      const T hyup = w + 1 < W ? xhy[w + 1][lane] : hy[K - 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const unsigned f = pf[q];
+        const bool bulk = (f & 1u) && jbulk;
+        const bool src = (f & 4u) && jsrc;
+        const T hxj = lane_down1(hx[q], hx[q]);
+        const T hyi = q + 1 < K ? hy[q + 1] : hyup;
+        T e = ez[q];
+        if (bulk) {
+          const T cb = cbz[q];
+          const T dHy = hyi - hy[q];
+          const T dHx = hxj - hx[q];
+          e = e + cb * (dHy * lane_get(lk_ikx, q) - dHx * jcf.iky);
+          if (src) e -= jv;
+          if (f & 8u) {
+            const T qv = lane_get(lk_bex, q) * pzx[q] + lane_get(lk_cex, q) * dHy;
+            pzx[q] = qv;
+            e += cb * qv;
+          }
+          if (sye >= 0) {
+            const T qv = jcf.bey * pzy[q] + jcf.cey * dHx;
+            pzy[q] = qv;
+            e -= cb * qv;
+          }
+        } else if (src) {
+          e = e - jv;
+        }
+        ez[q] = e;
+        if (jcap && (f & 96u)) {
+          if (f & 32u) ring[(rbase + 0) * pcells + cw] = e;
+          if (f & 64u) ring[(rbase + 3) * pcells + cw] = e;
+        }
+      }
+      // ---- H half-step: Ez(i-1) of the first plane from the previous wave
+      xez[w][lane] = ez[K - 1];
+      lds_barrier();
+      const T ezdn = w > 0 ? xez[w - 1][lane] : ez[0];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const unsigned f = pf[q];
+        const T ezj = lane_up1(ez[q], ez[q]);
+        const T prevE = q > 0 ? ez[q - 1] : ezdn;
+        if ((f & 1u) && jin1) {
+          const T dEz = ez[q] - ezj;
+          T hv = hx[q] + db * (-dEz) * ikyh;
+          if (syh >= 0) {
+            const T qv = bhy * qxy[q] + chy * dEz;
+            qxy[q] = qv;
+            hv -= db * qv;
+          }
+          hx[q] = hv;
+        }
+        T hv = hy[q];
+        if ((f & 2u) && jbulk) {
+          const T dEz = ez[q] - prevE;
+          hv = hv + db * dEz * lane_get(lk_ikh, q);
+          if (f & 16u) {
+            const T qv = lane_get(lk_bhx, q) * qyx[q] + lane_get(lk_chx, q) * dEz;
+            qyx[q] = qv;
+            hv += db * qv;
+          }
+          hy[q] = hv;
+        }
+        if (jcap && (f & 480u)) {
+          if (f & 32u) ring[(rbase + 1) * pcells + cw] = hv;
+          if (f & 128u) ring[(rbase + 2) * pcells + cw] = hv;
+          if (f & 64u) ring[(rbase + 4) * pcells + cw] = hv;
+          if (f & 256u) ring[(rbase + 5) * pcells + cw] = hv;
+        }
+      }
+    }
+    // ---- the inner cells to set B (fields unconditionally: cells no step
+    // updates hold the same value in both sets)
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = ib0 + q;
+      if (!jown || i < i0 || i >= i0 + TX || i >= nx) continue;
+      const TmzX x = tmz_x(v, i);
+      const unsigned idx = (unsigned)i * ny + j;
+      stg_nt<T>(EzB, idx * es, ez[q]);
+      stg_nt<T>(HxB, idx * es, hx[q]);
+      stg_nt<T>(HyB, idx * es, hy[q]);
+      if (x.ibulk && jbulk && x.sxe >= 0) psiBll[psi_id(F_EZ, 0, x.sxe)][(size_t)x.dxe * ny + j] = pzx[q];
+      if (x.ibulk && jbulk && sye >= 0) pzyB[(size_t)i * N + dye] = pzy[q];
+      if (x.ibulk && jin1 && syh >= 0) pxyB[(size_t)i * N + dyh] = qxy[q];
+      if (x.hbulk && jbulk && x.sxh >= 0) psiBll[psi_id(F_HY, 0, x.sxh)][(size_t)x.dxh * ny + j] = qyx[q];
+    }
+    lds_barrier();   // the next tile reuses the exchange planes
+  }
+}
+
+// ----------------------------------------------- fused 3D step, halo march ---
+// One launch per 3D time step with no LDS and no barriers.  Each wavefront
+// owns one row j and 63 z-cells of XM planes: lane l holds z-cell k0 - 1 + l,
+// lane 0 being the cell below the tile, which only computes.  It reads
+// field/psi set (n & 1), writes the other, and marches along x; per plane i:
+//   - new E of row j, all three components (lane 0 included);
+//   - new Ex and Ez of row j - 1, not stored: the j-1 neighbours of the H
+//     update, recomputed from set A with the same per-cell code as their
+//     owner computes them;
+//   - new H of row j: the k-1 neighbours from the next-lower lane (DPP), the
+//     x-1 neighbours (Ey, Ez of plane i-1) from the previous plane's
+//     registers.
+// Before the first plane it computes new E of plane i0 - 1 the same way.
+// The k+1 neighbours of the E update come from the next-higher lane, the top
+// lane loading its own.  Every load of plane i+1 is issued before plane i is
+// updated.  The per-cell arithmetic and the update ranges are those of k_3d_e
+// and k_3d_h.  HBM traffic per cell and step: each field read once and
+// written once (48 B in FP32) plus psi and the material byte; the re-read
+// row j-1 and z-cell k0 - 1 come from the caches.
+
+// One plane's loads: row j (own), the top lane's k+1 values, row j-1.
+template <typename T>
+struct HaloPlane {
+  T ex, ey, ez, hx, hy, hz, hxj, hzj, hy1, hz1;   // row j; hxj/hzj at j+1, hy1/hz1 at i+1
+  T hxt, hyt;                                     // Hx, Hy at k+1 (top lane only)
+  T qxy, qzy, qxz, qyz, qyx, qzx;                 // E psi ex_y, ez_y, ex_z, ey_z, ey_x, ez_x
+  T rxy, rzy, rxz, ryz, ryx, rzx;                 // H psi hx_y, hz_y, hx_z, hy_z, hy_x, hz_x
+  T ex_, ez_, hx_, hy_, hz_, hy1_, hyt_;          // row j-1
+  T pxy_, pzy_, pxz_, pzx_;                       // its psi ex_y, ez_y, ex_z, ez_x
+  unsigned m, m_;                                 // material bytes, rows j and j-1
+};
+
+template <typename T, int XM>
+__global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restrict__ slots,
+                                                          const int4* __restrict__ tiles,
+                                                          const int* __restrict__ counts, int tstep) {
+  static_assert(XM + 1 <= kBX, "one plane's x-only coefficients per lane");
+  constexpr unsigned es = sizeof(T);
+  const int ntiles = counts[0];
+  const int lane = threadIdx.x;
+  const int stride = gridDim.x * kBY;
+  for (int tt = xcd_block() * kBY + wave_id(); tt < ntiles; tt += stride) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n = s.base + tstep;
+    if (!s.active || n >= s.nt) continue;
+    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+    const int i0 = tl.y, j = tl.z, k0 = tl.w;
+    const int k = k0 - 1 + lane;
+    const int kc = min(max(k, 0), nz - 1);
+    const bool owner = lane >= 1 && k < nz;
+    const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
+    const bool odd = (n & 1) != 0;
+    OIDT_GLOBAL T* const* FA = odd ? s.f2 : s.f;
+    OIDT_GLOBAL T* const* FB = odd ? s.f : s.f2;
+    const T* __restrict__ ExA = FA[F_EX];
+    const T* __restrict__ EyA = FA[F_EY];
+    const T* __restrict__ EzA = FA[F_EZ];
+    const T* __restrict__ HxA = FA[F_HX];
+    const T* __restrict__ HyA = FA[F_HY];
+    const T* __restrict__ HzA = FA[F_HZ];
+    T* __restrict__ ExB = FB[F_EX];
+    T* __restrict__ EyB = FB[F_EY];
+    T* __restrict__ EzB = FB[F_EZ];
+    T* __restrict__ HxB = FB[F_HX];
+    T* __restrict__ HyB = FB[F_HY];
+    T* __restrict__ HzB = FB[F_HZ];
+    OIDT_GLOBAL T* const* psiA = odd ? s.psi2 : s.psi;
+    OIDT_GLOBAL T* const* psiB = odd ? s.psi : s.psi2;
+    const uint8_t* __restrict__ mat = s.mat;
+    const T db = s.db;
+    const unsigned pl = (unsigned)ny * nz;
+    const int jm = max(j - 1, 0);
+    const unsigned jk = (unsigned)j * nz + kc, jmk = (unsigned)jm * nz + kc;
+    const unsigned offk = kc + 1 < nz ? es : 0u;                 // top lane's k+1 (clamped as k_3d_e)
+    const unsigned offj = j + 1 < ny ? (unsigned)nz * es : 0u;
+    // ---- row (wave-uniform) coefficients: E rows j and j-1, H row j
+    const bool jin = j >= jb && j < je, jin1 = j > jb && j < je;
+    const bool jinm = j - 1 >= jb && j - 1 < je;
+    int dj = 0, cj = 0, djm = 0, cjm = 0, djh = 0, cjh = 0;
+    const int sj = jin ? slab_e(j, jb, je, N, dj, cj) : -1;
+    const int sjm = jinm ? slab_e(j - 1, jb, je, N, djm, cjm) : -1;
+    const int sjh = jin1 ? slab_h(j, jb, je, N, djh, cjh) : -1;
+    const T iky = jin ? s.ike[1][j] : T(0), ikym = jinm ? s.ike[1][j - 1] : T(0);
+    const T ikyh = jin1 ? s.ikh[1][j] : T(0);
+    const T bej = sj >= 0 ? s.be[cj] : T(0), cej = sj >= 0 ? s.ceo[cj] : T(0);
+    const T bejm = sjm >= 0 ? s.be[cjm] : T(0), cejm = sjm >= 0 ? s.ceo[cjm] : T(0);
+    const T bhj = sjh >= 0 ? s.bh[cjh] : T(0), chj = sjh >= 0 ? s.cho[cjh] : T(0);
+    // ---- lane (z) coefficients
+    const bool kin = k >= kb && k < ke, kin1 = k > kb && k < ke;
+    int dk = 0, ck = 0, dkh = 0, ckh = 0;
+    const int sk = kin ? slab_e(k, kb, ke, N, dk, ck) : -1;
+    const int skh = kin1 ? slab_h(k, kb, ke, N, dkh, ckh) : -1;
+    const T ikz = kin ? s.ike[2][kc] : T(0), ikzh = kin1 ? s.ikh[2][kc] : T(0);
+    const T bek = sk >= 0 ? s.be[ck] : T(0), cek = sk >= 0 ? s.ceo[ck] : T(0);
+    const T bhk = skh >= 0 ? s.bh[ckh] : T(0), chk = skh >= 0 ? s.cho[ckh] : T(0);
+    const unsigned zside = sk == 1 ? s.psi_hi[2] : 0u, zsideh = skh == 1 ? s.psi_hi[2] : 0u, xhi = s.psi_hi[0];
+    // sources: TE on Ey (E half-step), TM on Hy (start of the H half-step)
+    const bool ksrc = k >= s.sz0 && k < s.sz1 && j >= s.sy0 && j < s.sy1;
+    const bool srcE = s.src_field == F_EY && ksrc, srcH = s.src_field == F_HY && ksrc;
+    const T jv = (srcE || srcH) ? s.jsrc[n] : T(0);
+    const int sx0 = s.sx0, sx1 = s.sx1;
+    // capture: E at k = port (planes cap*6 + 0/1; 6/7 when the ports coincide),
+    // Hx/Hy at k = port (2/4, 8/10) and port + 1 (3/5, 9/11)
+    const bool jcap = owner && j >= s.wy0 && j <= s.wy1;
+    const int port0 = s.port[0], port1 = s.port[1];
+    const int capE = !jcap ? -1 : (k == port0 ? 0 : (k == port1 ? 1 : -1));
+    const bool capE2 = jcap && port0 == port1 && k == port0;
+    const bool capA = jcap && k == port0, capB = jcap && k == port0 + 1;
+    const bool capC = jcap && k == port1, capD = jcap && k == port1 + 1;
+    const int wx0 = s.wx0, wx1 = s.wx1, prow = s.prow, pcells = s.pcells, wy0 = s.wy0;
+    const size_t rbase = (size_t)tstep * s.nplanes;
+    T* __restrict__ ring = s.ring;
+    const T cx0 = s.cbv[0][0], cx1 = s.cbv[0][1], cx2 = s.cbv[0][2], cx3 = s.cbv[0][3];
+    const T cy0 = s.cbv[1][0], cy1 = s.cbv[1][1], cy2 = s.cbv[1][2], cy3 = s.cbv[1][3];
+    const T cz0 = s.cbv[2][0], cz1 = s.cbv[2][1], cz2 = s.cbv[2][2], cz3 = s.cbv[2][3];
+    const int iend = min(i0 + XM, nx);
+    // ---- x-only coefficients: lane u holds plane i0 - 1 + u
+    T lx_ik = T(0), lx_be = T(0), lx_ce = T(0), lx_ikh = T(0), lx_bh = T(0), lx_ch = T(0);
+    {
+      const int ip = i0 - 1 + lane;
+      if (lane <= XM && ip >= 0 && ip < nx) {
+        int d = 0, ci = 0;
+        lx_ik = s.ike[0][ip];
+        if (slab_e(ip, ib, ie, N, d, ci) >= 0) { lx_be = s.be[ci]; lx_ce = s.ceo[ci]; }
+        lx_ikh = s.ikh[0][ip];
+        if (slab_h(ip, ib, ie, N, d, ci) >= 0) { lx_bh = s.bh[ci]; lx_ch = s.cho[ci]; }
+      }
+    }
+    T* const pEyx = psiA[psi_id(F_EY, 0, 0)];
+    T* const pEzx = psiA[psi_id(F_EZ, 0, 0)];
+    T* const pExz = psiA[psi_id(F_EX, 2, 0)];
+    T* const pEyz = psiA[psi_id(F_EY, 2, 0)];
+    T* const pHyx = psiA[psi_id(F_HY, 0, 0)];
+    T* const pHzx = psiA[psi_id(F_HZ, 0, 0)];
+    T* const pHxz = psiA[psi_id(F_HX, 2, 0)];
+    T* const pHyz = psiA[psi_id(F_HY, 2, 0)];
+    T* const pExy = sj >= 0 ? psiA[psi_id(F_EX, 1, sj)] : nullptr;
+    T* const pEzy = sj >= 0 ? psiA[psi_id(F_EZ, 1, sj)] : nullptr;
+    T* const pExym = sjm >= 0 ? psiA[psi_id(F_EX, 1, sjm)] : nullptr;
+    T* const pEzym = sjm >= 0 ? psiA[psi_id(F_EZ, 1, sjm)] : nullptr;
+    T* const pHxy = sjh >= 0 ? psiA[psi_id(F_HX, 1, sjh)] : nullptr;
+    T* const pHzy = sjh >= 0 ? psiA[psi_id(F_HZ, 1, sjh)] : nullptr;
+
+    // plane i's loads; `full`: also row j-1 and the H psi (not needed for
+    // the plane below the tile)
+    auto load = [&](int i, bool full) {
+      HaloPlane<T> p;
+      const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
+      const unsigned bo1 = (i + 1 < nx ? idx + pl : idx) * es;
+      p.ex = ldg<T>(ExA, bo);
+      p.ey = ldg<T>(EyA, bo);
+      p.ez = ldg<T>(EzA, bo);
+      p.hx = ldg<T>(HxA, bo);
+      p.hy = ldg<T>(HyA, bo);
+      p.hz = ldg<T>(HzA, bo);
+      p.hxj = ldg<T>(HxA, bo + offj);
+      p.hzj = ldg<T>(HzA, bo + offj);
+      p.hy1 = ldg<T>(HyA, bo1);
+      p.hz1 = ldg<T>(HzA, bo1);
+      p.hxt = lane == 63 ? ldg<T>(HxA, bo + offk) : T(0);
+      p.hyt = lane == 63 ? ldg<T>(HyA, bo + offk) : T(0);
+      p.m = ldg<uint8_t>(mat, idx);
+      int di = 0, ci = 0;
+      const int si = slab_e(i, ib, ie, N, di, ci);
+      const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
+      const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es;
+      const unsigned iz = (((unsigned)i * ny + j) * N + dk) * es + zside;
+      p.qxy = sj >= 0 ? ldg<T>(pExy, iy) : T(0);
+      p.qzy = sj >= 0 ? ldg<T>(pEzy, iy) : T(0);
+      p.qxz = sk >= 0 ? ldg<T>(pExz, iz) : T(0);
+      p.qyz = sk >= 0 ? ldg<T>(pEyz, iz) : T(0);
+      p.qyx = si >= 0 ? ldg<T>(pEyx, ix) : T(0);
+      p.qzx = si >= 0 ? ldg<T>(pEzx, ix) : T(0);
+      p.rxy = p.rzy = p.rxz = p.ryz = p.ryx = p.rzx = T(0);
+      p.ex_ = p.ez_ = p.hx_ = p.hy_ = p.hz_ = p.hy1_ = p.hyt_ = T(0);
+      p.pxy_ = p.pzy_ = p.pxz_ = p.pzx_ = T(0);
+      p.m_ = 0u;
+      if (full) {
+        int dih = 0, cih = 0;
+        const int sih = slab_h(i, ib, ie, N, dih, cih);
+        const unsigned ixh = (((unsigned)dih * ny + j) * nz + kc) * es + (sih == 1 ? xhi : 0u);
+        const unsigned iyh = (((unsigned)i * N + djh) * nz + kc) * es;
+        const unsigned izh = (((unsigned)i * ny + j) * N + dkh) * es + zsideh;
+        p.rxy = sjh >= 0 ? ldg<T>(pHxy, iyh) : T(0);
+        p.rzy = sjh >= 0 ? ldg<T>(pHzy, iyh) : T(0);
+        p.rxz = skh >= 0 ? ldg<T>(pHxz, izh) : T(0);
+        p.ryz = skh >= 0 ? ldg<T>(pHyz, izh) : T(0);
+        p.ryx = sih >= 0 ? ldg<T>(pHyx, ixh) : T(0);
+        p.rzx = sih >= 0 ? ldg<T>(pHzx, ixh) : T(0);
+        if (j > 0) {
+          const unsigned idm = (unsigned)i * pl + jmk, bm = idm * es;
+          const unsigned bm1 = (i + 1 < nx ? idm + pl : idm) * es;
+          p.ex_ = ldg<T>(ExA, bm);
+          p.ez_ = ldg<T>(EzA, bm);
+          p.hx_ = ldg<T>(HxA, bm);
+          p.hy_ = ldg<T>(HyA, bm);
+          p.hz_ = ldg<T>(HzA, bm);
+          p.hy1_ = ldg<T>(HyA, bm1);
+          p.hyt_ = lane == 63 ? ldg<T>(HyA, bm + offk) : T(0);
+          p.m_ = ldg<uint8_t>(mat, idm);
+          const unsigned ixm = (((unsigned)di * ny + jm) * nz + kc) * es + (si == 1 ? xhi : 0u);
+          const unsigned iym = (((unsigned)i * N + djm) * nz + kc) * es;
+          const unsigned izm = (((unsigned)i * ny + jm) * N + dk) * es + zside;
+          p.pxy_ = sjm >= 0 ? ldg<T>(pExym, iym) : T(0);
+          p.pzy_ = sjm >= 0 ? ldg<T>(pEzym, iym) : T(0);
+          p.pxz_ = sk >= 0 ? ldg<T>(pExz, izm) : T(0);
+          p.pzx_ = si >= 0 ? ldg<T>(pEzx, ixm) : T(0);
+        }
+      }
+      return p;
+    };
+
+    T eyp = T(0), ezp = T(0);   // new Ey, Ez of plane i-1, row j
+    const int ifirst = i0 > 0 ? i0 - 1 : i0;
+    HaloPlane<T> pc = load(ifirst, ifirst >= i0);
+#pragma unroll
+    for (int u = 0; u <= XM; ++u) {
+      const int i = i0 - 1 + u;
+      if (i < ifirst) continue;
+      if (i >= iend) break;
+      HaloPlane<T> pn;
+      if (u < XM && i + 1 < iend) pn = load(i + 1, true);
+      const bool own = i >= i0;               // plane i0 - 1: new E of row j only
+      const int ux = i - (i0 - 1);
+      const T ikx = lane_get(lx_ik, ux), bex = lane_get(lx_be, ux), cex = lane_get(lx_ce, ux);
+      const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
+      int di = 0, ci = 0;
+      const int si = slab_e(i, ib, ie, N, di, ci);
+      const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
+      const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
+      const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es;
+      const unsigned iz = (((unsigned)i * ny + j) * N + dk) * es + zside;
+      const bool st = own && owner;
+      // ---- new E, row j (k_3d_e per cell)
+      const T hxk = lane_down1(pc.hx, pc.hxt);
+      const T hyk = lane_down1(pc.hy, pc.hyt);
+      T vx = pc.ex, vy = pc.ey, vz = pc.ez;
+      {
+        const bool bx = iin1 && jin && kin;
+        const bool by = iin && jin1 && kin;
+        const bool bz = iin && jin && kin1;
+        const bool src = srcE && i >= sx0 && i < sx1;
+        if (bx) {
+          const T cb = cb_of(cx0, cx1, cx2, cx3, pc.m, 0);
+          const T dHz = pc.hzj - pc.hz;
+          const T dHy = hyk - pc.hy;
+          vx = vx + cb * (dHz * iky - dHy * ikz);
+          if (sj >= 0) { const T q = bej * pc.qxy + cej * dHz; if (st) stg<T>(psiB[psi_id(F_EX, 1, sj)], iy, q); vx += cb * q; }
+          if (sk >= 0) { const T q = bek * pc.qxz + cek * dHy; if (st) stg<T>(psiB[psi_id(F_EX, 2, 0)], iz, q); vx -= cb * q; }
+        }
+        if (by) {
+          const T cb = cb_of(cy0, cy1, cy2, cy3, pc.m, 1);
+          const T dHx = hxk - pc.hx;
+          const T dHz = pc.hz1 - pc.hz;
+          vy = vy + cb * (dHx * ikz - dHz * ikx);
+          if (src) vy -= jv;
+          if (si >= 0) { const T q = bex * pc.qyx + cex * dHz; if (st) stg<T>(psiB[psi_id(F_EY, 0, 0)], ix, q); vy -= cb * q; }
+          if (sk >= 0) { const T q = bek * pc.qyz + cek * dHx; if (st) stg<T>(psiB[psi_id(F_EY, 2, 0)], iz, q); vy += cb * q; }
+        } else if (src) {
+          vy -= jv;
+        }
+        if (bz) {
+          const T cb = cb_of(cz0, cz1, cz2, cz3, pc.m, 2);
+          const T dHy = pc.hy1 - pc.hy;
+          const T dHx = pc.hxj - pc.hx;
+          vz = vz + cb * (dHy * ikx - dHx * iky);
+          if (si >= 0) { const T q = bex * pc.qzx + cex * dHy; if (st) stg<T>(psiB[psi_id(F_EZ, 0, 0)], ix, q); vz += cb * q; }
+          if (sj >= 0) { const T q = bej * pc.qzy + cej * dHx; if (st) stg<T>(psiB[psi_id(F_EZ, 1, sj)], iy, q); vz -= cb * q; }
+        }
+      }
+      if (own) {
+        // ---- new Ex, Ez of row j-1 (its owner's per-cell code, not stored)
+        T vxm = pc.ex_, vzm = pc.ez_;
+        {
+          const T hykm = lane_down1(pc.hy_, pc.hyt_);
+          if (iin1 && jinm && kin) {
+            const T cb = cb_of(cx0, cx1, cx2, cx3, pc.m_, 0);
+            const T dHz = pc.hz - pc.hz_;
+            const T dHy = hykm - pc.hy_;
+            vxm = vxm + cb * (dHz * ikym - dHy * ikz);
+            if (sjm >= 0) { const T q = bejm * pc.pxy_ + cejm * dHz; vxm += cb * q; }
+            if (sk >= 0) { const T q = bek * pc.pxz_ + cek * dHy; vxm -= cb * q; }
+          }
+          if (iin && jinm && kin1) {
+            const T cb = cb_of(cz0, cz1, cz2, cz3, pc.m_, 2);
+            const T dHy = pc.hy1_ - pc.hy_;
+            const T dHx = pc.hx - pc.hx_;
+            vzm = vzm + cb * (dHy * ikx - dHx * ikym);
+            if (si >= 0) { const T q = bex * pc.pzx_ + cex * dHy; vzm += cb * q; }
+            if (sjm >= 0) { const T q = bejm * pc.pzy_ + cejm * dHx; vzm -= cb * q; }
+          }
+        }
+        // ---- new H, row j (k_3d_h per cell)
+        const T exk = lane_up1(vx, vx), eyk = lane_up1(vy, vy);
+        const T ikxh = lane_get(lx_ikh, ux), bhx = lane_get(lx_bh, ux), chx = lane_get(lx_ch, ux);
+        int dih = 0, cih = 0;
+        const int sih = slab_h(i, ib, ie, N, dih, cih);
+        const unsigned ixh = (((unsigned)dih * ny + j) * nz + kc) * es + (sih == 1 ? xhi : 0u);
+        const unsigned iyh = (((unsigned)i * N + djh) * nz + kc) * es;
+        const unsigned izh = (((unsigned)i * ny + j) * N + dkh) * es + zsideh;
+        const bool bx = iin && jin1 && kin1;
+        const bool by = iin1 && jin && kin1;
+        const bool bz = iin1 && jin1 && kin;
+        const bool src = srcH && i >= sx0 && i < sx1;
+        T hx = pc.hx;
+        if (bx) {
+          const T dEy = vy - eyk;
+          const T dEz = vz - vzm;
+          hx = hx + db * (dEy * ikzh - dEz * ikyh);
+          if (sjh >= 0) { const T q = bhj * pc.rxy + chj * dEz; if (owner) stg<T>(psiB[psi_id(F_HX, 1, sjh)], iyh, q); hx -= db * q; }
+          if (skh >= 0) { const T q = bhk * pc.rxz + chk * dEy; if (owner) stg<T>(psiB[psi_id(F_HX, 2, 0)], izh, q); hx += db * q; }
+        }
+        T hy = pc.hy;
+        if (src) hy -= jv;
+        if (by) {
+          const T dEz = vz - ezp;
+          const T dEx = vx - exk;
+          hy = hy + db * (dEz * ikxh - dEx * ikzh);
+          if (sih >= 0) { const T q = bhx * pc.ryx + chx * dEz; if (owner) stg<T>(psiB[psi_id(F_HY, 0, 0)], ixh, q); hy += db * q; }
+          if (skh >= 0) { const T q = bhk * pc.ryz + chk * dEx; if (owner) stg<T>(psiB[psi_id(F_HY, 2, 0)], izh, q); hy -= db * q; }
+        }
+        T hz = pc.hz;
+        if (bz) {
+          const T dEx = vx - vxm;
+          const T dEy = vy - eyp;
+          hz = hz + db * (dEx * ikyh - dEy * ikxh);
+          if (sih >= 0) { const T q = bhx * pc.rzx + chx * dEy; if (owner) stg<T>(psiB[psi_id(F_HZ, 0, 0)], ixh, q); hz -= db * q; }
+          if (sjh >= 0) { const T q = bhj * pc.rzy + chj * dEx; if (owner) stg<T>(psiB[psi_id(F_HZ, 1, sjh)], iyh, q); hz += db * q; }
+        }
+        if (owner) {
+          stg_nt<T>(ExB, bo, vx);
+          stg_nt<T>(EyB, bo, vy);
+          stg_nt<T>(EzB, bo, vz);
+          stg_nt<T>(HxB, bo, hx);
+          stg_nt<T>(HyB, bo, hy);
+          stg_nt<T>(HzB, bo, hz);
+          if (jcap && i >= wx0 && i <= wx1) {
+            const size_t c = (size_t)((i - wx0) * prow + (j - wy0));
+            if (capE >= 0) {
+              ring[(rbase + capE * 6 + 0) * pcells + c] = vx;
+              ring[(rbase + capE * 6 + 1) * pcells + c] = vy;
+            }
+            if (capE2) {
+              ring[(rbase + 6) * pcells + c] = vx;
+              ring[(rbase + 7) * pcells + c] = vy;
+            }
+            if (capA) { ring[(rbase + 2) * pcells + c] = hx; ring[(rbase + 4) * pcells + c] = hy; }
+            if (capB) { ring[(rbase + 3) * pcells + c] = hx; ring[(rbase + 5) * pcells + c] = hy; }
+            if (capC) { ring[(rbase + 8) * pcells + c] = hx; ring[(rbase + 10) * pcells + c] = hy; }
+            if (capD) { ring[(rbase + 9) * pcells + c] = hx; ring[(rbase + 11) * pcells + c] = hy; }
+          }
+        }
+      }
+      eyp = vy;
+      ezp = vz;
+      if (u < XM && i + 1 < iend) pc = pn;
+    }
+  }
+}
+
 // ------------------------------------------------------- fused 3D step ---
 // One launch per time step for 3D: both half-steps, reading field/psi set
 // (n & 1) and writing the other, so no block races another.  A block of W
@@ -2242,19 +2850,6 @@ __device__ __forceinline__ Q4<T> q4_zero() {
 #pragma unroll
   for (int c = 0; c < Fused3d<T>::C; ++c) q.v[c] = T(0);
   return q;
-}
-
-// Workgroup barrier for the LDS row buffers only: the fences order this
-// wave's LDS accesses around the barrier (a wait on the LDS counter), not its
-// global loads and stores.  No wave of a fused 3D block reads what another
-// writes to global memory (reads come from field set A, writes go to set B),
-// so the next plane's loads stay in flight across the barrier instead of
-// being waited for as a full __syncthreads() would (its release fence waits
-// for every outstanding global access of the wave).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <typename T>

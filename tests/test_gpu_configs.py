@@ -69,8 +69,10 @@ def config0_oracle():
     return p, out, S
 
 
-def device_protocol(p, precision):
+def device_protocol(p, precision, launch=None):
     b = Batch(p['scheme'], np.float32 if precision == 'float32' else np.float64, max_slots=4)
+    if launch:
+        b.set_launch(launch[1], launch[2], launch[0])
     slots, setups, fgs = {}, {}, {}
     for (d, k), q in protocol(dict(p, precision=precision)).items():
         s, up, lo = configs.realise(q)
@@ -83,11 +85,12 @@ def device_protocol(p, precision):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('precision,tol', [('float64', 1e-10), ('float32', None)])
-def test_config0_protocol_full_length(config0_oracle, precision, tol):
+@pytest.mark.parametrize('precision,tol,launch', [('float64', 1e-10, None), ('float32', None, None),
+                                                 ('float64', 1e-10, (4, 8, 0)), ('float32', None, (4, 4, 0))])
+def test_config0_protocol_full_length(config0_oracle, precision, tol, launch):
     p, ora, S_ora = config0_oracle
-    print('case config 0', precision, flush=True)
-    b, slots, setups, fgs = device_protocol(p, precision)
+    print('case config 0', precision, 'launch', launch, flush=True)
+    b, slots, setups, fgs = device_protocol(p, precision, launch)
     try:
         for key in RUNS:
             assert np.array_equal(fgs[key], ora[key][2]), key
@@ -224,7 +227,7 @@ def batch_vs_alone(descs, nt, launch=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('launch', [None, (3, 16, 0), (2, 4, 0)])
+@pytest.mark.parametrize('launch', [None, (3, 16, 0), (2, 4, 0), (4, 4, 0)])
 def test_config2_batch_members_identical_alone(launch):
     print('case config 2, 64 runs, launch', launch, flush=True)
     batch_vs_alone(configs.config2(), 160, launch)

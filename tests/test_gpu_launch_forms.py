@@ -10,7 +10,9 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
-FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32)], '3d': [(3, 12), (3, 16)], 'te': [(3, 16), (3, 32)]}
+FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32), (4, 2), (4, 4), (4, 8)],
+              '3d': [(3, 12), (3, 16), (5, 1), (5, 2), (5, 4), (5, 8), (5, 16)],
+              'te': [(3, 16), (3, 32)]}   # TMz also the blocked step (variant 4), 3D the halo march (5)
 
 
 @pytest.mark.gpu
@@ -73,6 +75,68 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('tb,direction,extra', [(2, 'b', {'nt': 101}), (4, 'f', {'nt': 101}), (8, 'f', {}),
+                                                (8, 'b', {'nt': 99})])
+def test_blocked_step_fp64(tb, direction, extra):
+    """The temporally blocked TMz step (tb steps per launch) in FP64 equals
+    the oracle to 1e-10, also when the step count is not a multiple of tb (the
+    last launch takes fewer steps and the state ends in the other field set)."""
+    p = configs.small('tm', 'float64', **extra)
+    p.update(rough_std=15e-9, rough_acl=40e-9, seed=5)
+    s, up, lo, fg = device_inputs(p)
+    o, _ = oracle_run(p, up, lo, direction)
+    print('case blocked tm fp64 tb', tb, direction, extra, flush=True)
+    b = Batch('tm', np.float64, max_slots=1)
+    b.set_launch(tb, 0, 4)
+    slot = b.add(s, fg, direction)
+    b.advance(-(-s.nt // 32))
+    b.sync()
+    assert b.poll(slot) == s.nt
+    for name, ref in o.fields().items():
+        err = rel_l2(b.field(slot, name), ref)
+        assert err < 1e-10, (name, err)
+    w = window('tm', o.c)
+    od = o.dft()
+    for key, z in b.dft(slot).items():
+        err = rel_l2(z[w], od[key][w])
+        assert err < 1e-10, (key, err)
+    b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('precision', ['float32', 'float64'])
+@pytest.mark.parametrize('tb', [2, 4, 8])
+def test_blocked_step_matches_fused(precision, tb):
+    """The blocked step applies the fused step's per-cell arithmetic: on an
+    uneven pair of runs in one batch (tile edges fall at different cells) its
+    fields and port DFT equal the one-step fused form's to rounding (the two
+    kernels may contract multiply-adds differently): relative L2 1e-12 in
+    FP64, 2e-6 in FP32."""
+    tol = 1e-12 if precision == 'float64' else 2e-6
+    runs = []
+    for nt, nx, seed in ((101, 140, 5), (96, 181, 6)):
+        p = configs.small('tm', precision, nt=nt, nx=nx)
+        p.update(rough_std=15e-9, rough_acl=40e-9, seed=seed)
+        s, up, lo, fg = device_inputs(p)
+        runs.append((s, fg))
+    out = {}
+    for form in ((3, 2), (4, tb)):
+        print('case blocked vs fused', precision, 'form', form, flush=True)
+        b = Batch('tm', np.float32 if precision == 'float32' else np.float64, max_slots=2)
+        b.set_launch(form[1], 0, form[0])
+        slots = [b.add(s, fg) for s, fg in runs]
+        b.advance(-(-max(s.nt for s, _ in runs) // 32))
+        b.sync()
+        out[form] = [({n: b.field(sl, n) for n in ('ez', 'hx', 'hy')}, b.dft(sl)) for sl in slots]
+        b.close()
+    for (fa, da), (fb, dbb) in zip(out[(3, 2)], out[(4, tb)]):
+        errs = {n: rel_l2(fb[n], fa[n]) for n in fa}
+        errs.update({k: rel_l2(dbb[k], da[k]) for k in da})
+        print({k: '%.2e' % e for k, e in errs.items()}, flush=True)
+        assert max(errs.values()) < tol, errs
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize('precision,variant,xc,direction,extra', [
     ('float64', 2, 1, 'f', {}),                                # runs [64,128) and [128,192) outside the slabs
     ('float64', 2, 2, 'b', {'source_condition': 'TM'}),
@@ -87,6 +151,10 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
     ('float32', 3, 8, 'f', {'p1': 63, 'p2': 127}),             # H samples at the z-wave boundary 128
     ('float32', 3, 12, 'b', {'nz': 300, 'p2': 270}),           # three waves per row, one wave idle
     ('float64', 3, 2, 'f', {'nz': 300, 'p2': 270}),            # FP64: two rows of three waves, two idle
+    ('float64', 5, 4, 'f', {'sz': 70}),                        # the halo march: 63-cell z runs
+    ('float64', 5, 8, 'b', {'source_condition': 'TM'}),
+    ('float32', 5, 2, 'f', {'p1': 62, 'p2': 125}),             # H samples at the run boundaries 63 and 126
+    ('float32', 5, 16, 'b', {'nz': 300, 'p2': 270}),
 ])
 def test_3d_long_z_runs(precision, variant, xc, direction, extra):
     """A guide long enough in z (nz = 250) for four 64-cell z runs per row:
