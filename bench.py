@@ -496,9 +496,7 @@ def roofline_of(scheme, es, variant, xchunk, cells, ms, key):
         steps = int(xchunk)
         kname, ms_k, bytes_k = ('k_tmz_tb', ms_e, (be + bh) * cells * steps)
     elif variant == 3:
-        kname, ms_k, bytes_k = (stem + ('_fused' if scheme == '3d' else '_step'), ms_e, (be + bh) * cells)
-    elif variant == 5:
-        kname, ms_k, bytes_k = ('k_3d_halo', ms_e, (be + bh) * cells)
+        kname, ms_k, bytes_k = (stem + ('_halo' if scheme == '3d' else '_step'), ms_e, (be + bh) * cells)
     elif ms_e >= ms_h:
         kname, ms_k, bytes_k = (stem + ('_e2' if variant == 2 else '_e'), ms_e, be * cells)
     else:
@@ -634,7 +632,8 @@ def main():
                          'wi_te, wi_3d')
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=0,
-                    help='kernel form: 1 march, 2 batched loads, 3 fused step; 0: autotuner picks form and xchunk')
+                    help='kernel form: 1 march, 2 batched loads, 3 fused step, 4 blocked TMz step (xchunk steps per launch); '
+                         '0: autotuner picks form and xchunk')
     ap.add_argument('--grid', type=int, default=0, help='persistent grid size cap of the stencil launches (0: 8192)')
     ap.add_argument('--profile-chunks', type=int, default=4)
     ap.add_argument('--cpu-seconds', type=float, default=15.0, help='length of the CPU baseline sample')

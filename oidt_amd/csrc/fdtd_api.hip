@@ -105,7 +105,6 @@ struct SlotHost {
   // set B when ceil(steps / tb) is odd.
   int tb = 0;
   int nplanes = 0, nq = 0, wcells = 0, pcells = 0, wyn = 0;
-  int nzp = 0;                      // padded z extent (3D)
   std::vector<MonHost> mons;
 };
 
@@ -485,7 +484,6 @@ int build_slot(oidt_batch* b, SlotHost& sh, const oidt_run_desc& d, const oidt_r
   s.w = (OIDT_GLOBAL const double*)(base + L.off_w);
   sh.slot.assign(reinterpret_cast<unsigned char*>(&s), reinterpret_cast<unsigned char*>(&s) + sizeof(s));
   sh.nplanes = L.nplanes; sh.nq = L.nq; sh.wcells = L.wcells; sh.pcells = L.pcells; sh.wyn = L.wy1 - L.wy0;
-  sh.nzp = pd.nz;
   return OIDT_OK;
 }
 
@@ -512,29 +510,14 @@ int grow(oidt_batch* b, int4** buf, size_t* cap, size_t need) {
 }
 
 // SlotHost::tb of a run stepped by a launch form.
-int run_tb(int variant, int xchunk) { return variant == 4 ? xchunk : (variant >= 3 ? 1 : 0); }
+int run_tb(int variant, int xchunk) { return variant == 4 ? xchunk : (variant == 3 ? 1 : 0); }
 
 // Kernels whose tiles are per wavefront (four tiles per block): the fused
 // 2D steps.
-bool march_tiles(const oidt_batch* b) {
-  return (b->variant == 3 && b->scheme != SCHEME_3D) || b->variant == 5;
-}
+bool march_tiles(const oidt_batch* b) { return b->variant == 3; }
 
-// z-cells a wave of the 3D halo march owns (its lane 0 is the cell below).
+// z-cells a wave of the fused 3D step owns (its lane 0 is the cell below).
 constexpr int kHaloZ = kBX - 1;
-
-// Rows of a fused 3D step block (the halo row included) for a run of padded z
-// extent nzp, or 0 when its LDS rows do not fit (see k_3d_fused).
-template <typename T>
-int fused3d_rows_t(int nzp) {
-  const int zw = (nzp + Fused3d<T>::ZSEG - 1) / Fused3d<T>::ZSEG;
-  const int R = Fused3d<T>::W / zw;
-  if (zw > Fused3d<T>::ZWMAX || R < 2 || Fused3d<T>::lds_cells(zw) > Fused3d<T>::LDS) return 0;
-  return R;
-}
-int fused3d_rows(int precision, int nzp) {
-  return precision == OIDT_F64 ? fused3d_rows_t<double>(nzp) : fused3d_rows_t<float>(nzp);
-}
 
 // Monitor table and the two work lists (after the E and after the H
 // half-step): one item per kMonCells consecutive box cells.
@@ -592,19 +575,13 @@ int rebuild_lists(oidt_batch* b) {
     if (!sh.used) continue;
     const oidt_run_desc& d = sh.desc;
     if (d.scheme == SCHEME_3D && b->variant == 3) {
-      // fused step: one block per tile of xc planes x (rows per block - 1)
-      // rows x the whole z extent
-      const int rows = fused3d_rows(b->precision, sh.nzp) - 1;
-      for (int x0 = 0; x0 < d.nx; x0 += xc)
-        for (int y0 = 0; y0 < d.ny; y0 += rows) tiles.push_back(make_int4(si, x0, y0, 0));
-    } else if (b->variant == 5) {
-      // 3D halo march: one wavefront per (row, 63 z-cells) x xc planes
+      // fused step: one wavefront per (row, 63 z-cells) x xc planes
       for (int x0 = 0; x0 < d.nx; x0 += xc)
         for (int y0 = 0; y0 < d.ny; ++y0)
           for (int z0 = 0; z0 < d.nz; z0 += kHaloZ) tiles.push_back(make_int4(si, x0, y0, z0));
     } else if (b->variant == 4) {
       // blocked step: one block per tile of its inner cells
-      const int tx = kTbW * kTbK - 2 * xc, ty = 64 - 2 * xc;
+      const int tx = kTbPlanes - 2 * xc, ty = 64 - 2 * xc;
       for (int x0 = 0; x0 < d.nx; x0 += tx)
         for (int y0 = 0; y0 < d.ny; y0 += ty) tiles.push_back(make_int4(si, x0, y0, 0));
     } else if (d.scheme == SCHEME_3D) {
@@ -692,49 +669,42 @@ void launch_fused(oidt_batch* b, int gs, int tstep) {
     hipLaunchKernelGGL((k_tmz_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
   } else if (b->scheme == SCHEME_TEZ) {
     hipLaunchKernelGGL((k_tez_step<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
-  } else {
-    hipLaunchKernelGGL((k_3d_fused<T>), grd, dim3(64, Fused3d<T>::W), 0, b->stream, sl, b->d_tiles, b->d_counts,
-                       tstep, XM);
+  } else if constexpr (XM == 1 || XM == 2 || XM == 4 || XM == 8 || XM == 16) {
+    hipLaunchKernelGGL((k_3d_halo<T, XM>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
   }
 }
 
-// The temporally blocked TMz step: xchunk steps per launch.
+// The temporally blocked TMz step: xchunk steps per launch.  Blocks of 16
+// waves x 2 planes in FP64 (half the registers per wave of 8 x 4, and the
+// FP64 workloads are single small runs that need the extra waves per tile to
+// hide latency), 8 x 4 in FP32 (better on large batches).  The shape depends
+// only on the precision, so a run gives the same bits alone and in a batch.
+template <typename T, int TB, int W>
+void launch_blocked_w(oidt_batch* b, int gs, int tstep) {
+  hipLaunchKernelGGL((k_tmz_tb<T, TB, W>), dim3(gs), dim3(64, W), 0, b->stream,
+                     static_cast<const Slot<T>*>(b->d_slots), b->d_tiles, b->d_counts, tstep);
+}
+template <typename T, int W>
+void launch_blocked_tb(oidt_batch* b, int gs, int tstep) {
+  switch (b->xchunk) {
+    case 2: launch_blocked_w<T, 2, W>(b, gs, tstep); break;
+    case 4: launch_blocked_w<T, 4, W>(b, gs, tstep); break;
+    default: launch_blocked_w<T, 8, W>(b, gs, tstep); break;
+  }
+}
 template <typename T>
 void launch_blocked(oidt_batch* b, int gs, int tstep) {
-  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
-  const dim3 blk(64, kTbW), grd(gs);
-  switch (b->xchunk) {
-    case 2: hipLaunchKernelGGL((k_tmz_tb<T, 2>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-    case 4: hipLaunchKernelGGL((k_tmz_tb<T, 4>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-    default: hipLaunchKernelGGL((k_tmz_tb<T, 8>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-  }
+  if (sizeof(T) == 8) launch_blocked_tb<T, 16>(b, gs, tstep);
+  else launch_blocked_tb<T, 8>(b, gs, tstep);
 }
 
 // Steps one stencil launch advances: the blocked step's depth, else 1.
 int launch_steps(const oidt_batch* b) { return b->variant == 4 ? b->xchunk : 1; }
 
-// The 3D halo march: the whole step in one launch, xchunk planes per wave.
-template <typename T>
-void launch_halo(oidt_batch* b, int gs, int tstep) {
-  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
-  const dim3 blk(kBX, kBY), grd(gs);
-  switch (b->xchunk) {
-    case 1: hipLaunchKernelGGL((k_3d_halo<T, 1>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-    case 2: hipLaunchKernelGGL((k_3d_halo<T, 2>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-    case 4: hipLaunchKernelGGL((k_3d_halo<T, 4>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-    case 16: hipLaunchKernelGGL((k_3d_halo<T, 16>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-    default: hipLaunchKernelGGL((k_3d_halo<T, 8>), grd, blk, 0, b->stream, sl, b->d_tiles, b->d_counts, tstep); break;
-  }
-}
-
 template <typename T>
 void launch_stencil(oidt_batch* b, int gs, int tstep, bool e_half) {
   if (b->variant == 4) {
     if (e_half) launch_blocked<T>(b, gs, tstep);
-    return;
-  }
-  if (b->variant == 5) {
-    if (e_half) launch_halo<T>(b, gs, tstep);
     return;
   }
   if (b->variant == 3) {
@@ -976,6 +946,7 @@ int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt
     delete b;
     return fail(OIDT_EHIP, "hipGetDevice failed");
   }
+
   // measured defaults on MI355X: the prefetching march over 4-plane tiles in
   // 3D, the batched-load form with four cells per thread in 2D
   b->variant = scheme == SCHEME_3D ? 1 : 2;
@@ -1002,11 +973,6 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
   if (rc != OIDT_OK) return rc;
   if (desc->scheme != b->scheme || desc->precision != b->precision)
     return fail(OIDT_EINVAL, "run scheme/precision differs from the batch");
-  if (b->variant == 3 && b->scheme == SCHEME_3D) {
-    const int nzp = b->precision == OIDT_F64 ? padded<double>(*desc).nz : padded<float>(*desc).nz;
-    if (fused3d_rows(b->precision, nzp) == 0 || desc->num_cpml > kCpmlMax)
-      return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64) and num_cpml <= 64");
-  }
   int si = -1;
   for (int i = 0; i < b->max_slots; ++i)
     if (!b->slots[i].used) { si = i; break; }
@@ -1032,18 +998,15 @@ int oidt_batch_add(oidt_batch* b, const oidt_run_desc* desc, const oidt_run_inpu
 
 int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, int32_t variant) {
   if (!b) return fail(OIDT_EINVAL, "null batch");
-  if (variant < 0 || variant > 5) return fail(OIDT_EINVAL, "variant must be 0 (keep), 1, 2, 3, 4 or 5");
+  if (variant < 0 || variant > 4) return fail(OIDT_EINVAL, "variant must be 0 (keep), 1, 2, 3 or 4");
   if (variant == 0) variant = b->variant;
   const int xc = xchunk ? xchunk : b->xchunk;   // the depth this launch form will use
   if (variant == 4) {
     if (b->scheme != SCHEME_TMZ) return fail(OIDT_EINVAL, "the blocked step (variant 4) is for TMz runs");
     if (xc != 2 && xc != 4 && xc != 8) return fail(OIDT_EINVAL, "the blocked step takes 2, 4 or 8 steps per launch");
   }
-  if (variant == 5) {
-    if (b->scheme != SCHEME_3D) return fail(OIDT_EINVAL, "the halo march (variant 5) is for 3D runs");
-    if (xc != 1 && xc != 2 && xc != 4 && xc != 8 && xc != 16)
-      return fail(OIDT_EINVAL, "the halo march takes 1, 2, 4, 8 or 16 planes per wave");
-  }
+  if (variant == 3 && b->scheme == SCHEME_3D && xc != 1 && xc != 2 && xc != 4 && xc != 8 && xc != 16)
+    return fail(OIDT_EINVAL, "the fused 3D step takes 1, 2, 4, 8 or 16 planes per wave");
   for (const auto& sh : b->slots)
     if (sh.used && b->chunks_enqueued > sh.admit_chunk && sh.tb != run_tb(variant, xc))
       return fail(OIDT_ESTATE, "cannot change how a run in progress swaps field sets (two-kernel, fused or blocked form and depth)");
@@ -1052,10 +1015,6 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (!pow2 && !(fused_only && variant == 3))
     return fail(OIDT_EINVAL, "xchunk must be 0 (keep), 1, 2, 4, 8, 16 or 32 (also 12, 20 or 24 for the fused step)");
   if (grid_blocks < 0) return fail(OIDT_EINVAL, "negative grid size");
-  if (variant == 3 && b->scheme == SCHEME_3D)
-    for (const auto& sh : b->slots)
-      if (sh.used && (fused3d_rows(b->precision, sh.nzp) == 0 || sh.desc.num_cpml > kCpmlMax))
-        return fail(OIDT_EINVAL, "the fused 3D step takes runs of nz <= 1024 (FP32) or 512 (FP64) and num_cpml <= 64");
   if (variant == 3 && b->scheme == SCHEME_TMZ_LOSSY)
     return fail(OIDT_EINVAL, "the lossy TMz form has no fused step (variant 1 or 2)");
   if (variant >= 3 && b->has_monitors())

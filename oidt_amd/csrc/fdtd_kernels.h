@@ -2118,11 +2118,11 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // ------------------------------------------------ temporally blocked TMz ---
-// TB Yee steps per launch.  A block of kTbW waves holds an extended tile of
-// kTbW * kTbK planes (x) by 64 rows (y, one per lane) in registers: wave w
-// owns planes ib0 + w*kTbK .. + kTbK - 1, every lane its row of those
+// TB Yee steps per launch.  A block of W waves holds an extended tile of
+// kTbPlanes planes (x) by 64 rows (y, one per lane) in registers: wave w
+// owns K = kTbPlanes / W planes from ib0 + w*K, every lane its row of those
 // planes (Ez, Hx, Hy, the four psi values and the material byte of each).
-// The tile's own cells are the inner (kTbW*kTbK - 2 TB) x (64 - 2 TB); the
+// The tile's own cells are the inner (kTbPlanes - 2 TB) x (64 - 2 TB); the
 // TB-cell ghost frame on every side is loaded and stepped like the rest and
 // discarded: each step (E then H) moves the region that depends on values
 // outside the tile one cell inwards from each side, so after TB steps the
@@ -2136,9 +2136,9 @@ __device__ __forceinline__ void lds_barrier() {
 // ((n0 / TB) & 1) and writes the inner cells' state after its steps to the
 // other set.
 // HBM traffic per cell and launch: every field and psi value read once
-// (ghost frame included) and written once, for TB steps.
-constexpr int kTbW = 8;    // waves per block
-constexpr int kTbK = 4;    // planes per wave
+// (ghost frame included) and written once, for TB steps.  Two block shapes
+// share the tiles: 8 waves of 4 planes, or 16 waves of 2 planes.
+constexpr int kTbPlanes = 32;   // planes of a block's extended tile
 
 __device__ __forceinline__ float lane_down1(float v, float hi) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(hi), __float_as_int(v), 0x130, 0xf, 0xf, false));
@@ -2150,12 +2150,12 @@ __device__ __forceinline__ double lane_down1(double v, double hi) {
   return __longlong_as_double(((long long)bhi << 32) | (unsigned)blo);
 }
 
-template <typename T, int TB>
-__global__ __launch_bounds__(64 * kTbW) void k_tmz_tb(const Slot<T>* __restrict__ slots,
-                                                      const int4* __restrict__ tiles,
-                                                      const int* __restrict__ counts, int tstep) {
-  constexpr int W = kTbW, K = kTbK;
-  constexpr int TX = W * K - 2 * TB, TY = 64 - 2 * TB;   // inner cells of a tile
+template <typename T, int TB, int W>
+__global__ __launch_bounds__(64 * W) void k_tmz_tb(const Slot<T>* __restrict__ slots,
+                                                   const int4* __restrict__ tiles,
+                                                   const int* __restrict__ counts, int tstep) {
+  constexpr int K = kTbPlanes / W;
+  constexpr int TX = kTbPlanes - 2 * TB, TY = 64 - 2 * TB;   // inner cells of a tile
   static_assert(TX > 0 && TY > 0 && kChunk % TB == 0, "blocking depth");
   constexpr unsigned es = sizeof(T);
   __shared__ T xhy[W][64];   // old Hy of each wave's first plane
@@ -2351,8 +2351,8 @@ __global__ __launch_bounds__(64 * kTbW) void k_tmz_tb(const Slot<T>* __restrict_
   }
 }
 
-// ----------------------------------------------- fused 3D step, halo march ---
-// One launch per 3D time step with no LDS and no barriers.  Each wavefront
+// ------------------------------------------------------- fused 3D step ---
+// One launch per 3D time step (form 3 in 3D), with no LDS and no barriers.  Each wavefront
 // owns one row j and 63 z-cells of XM planes: lane l holds z-cell k0 - 1 + l,
 // lane 0 being the cell below the tile, which only computes.  It reads
 // field/psi set (n & 1), writes the other, and marches along x; per plane i:
@@ -2585,64 +2585,65 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
       const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es;
       const unsigned iz = (((unsigned)i * ny + j) * N + dk) * es + zside;
       const bool st = own && owner;
-      // ---- new E, row j (k_3d_e per cell)
+      // ---- new E, row j (k_3d_e per cell).  The plane and row conditions
+      // are wave-uniform branches; the z conditions select per lane (outside
+      // the z slabs the z-psi coefficients and values are zero and add an
+      // exact zero), so a wave does not split on them.
       const T hxk = lane_down1(pc.hx, pc.hxt);
       const T hyk = lane_down1(pc.hy, pc.hyt);
       T vx = pc.ex, vy = pc.ey, vz = pc.ez;
-      {
-        const bool bx = iin1 && jin && kin;
-        const bool by = iin && jin1 && kin;
-        const bool bz = iin && jin && kin1;
-        const bool src = srcE && i >= sx0 && i < sx1;
-        if (bx) {
-          const T cb = cb_of(cx0, cx1, cx2, cx3, pc.m, 0);
-          const T dHz = pc.hzj - pc.hz;
-          const T dHy = hyk - pc.hy;
-          vx = vx + cb * (dHz * iky - dHy * ikz);
-          if (sj >= 0) { const T q = bej * pc.qxy + cej * dHz; if (st) stg<T>(psiB[psi_id(F_EX, 1, sj)], iy, q); vx += cb * q; }
-          if (sk >= 0) { const T q = bek * pc.qxz + cek * dHy; if (st) stg<T>(psiB[psi_id(F_EX, 2, 0)], iz, q); vx -= cb * q; }
-        }
-        if (by) {
-          const T cb = cb_of(cy0, cy1, cy2, cy3, pc.m, 1);
-          const T dHx = hxk - pc.hx;
-          const T dHz = pc.hz1 - pc.hz;
-          vy = vy + cb * (dHx * ikz - dHz * ikx);
-          if (src) vy -= jv;
-          if (si >= 0) { const T q = bex * pc.qyx + cex * dHz; if (st) stg<T>(psiB[psi_id(F_EY, 0, 0)], ix, q); vy -= cb * q; }
-          if (sk >= 0) { const T q = bek * pc.qyz + cek * dHx; if (st) stg<T>(psiB[psi_id(F_EY, 2, 0)], iz, q); vy += cb * q; }
-        } else if (src) {
-          vy -= jv;
-        }
-        if (bz) {
-          const T cb = cb_of(cz0, cz1, cz2, cz3, pc.m, 2);
-          const T dHy = pc.hy1 - pc.hy;
-          const T dHx = pc.hxj - pc.hx;
-          vz = vz + cb * (dHy * ikx - dHx * iky);
-          if (si >= 0) { const T q = bex * pc.qzx + cex * dHy; if (st) stg<T>(psiB[psi_id(F_EZ, 0, 0)], ix, q); vz += cb * q; }
-          if (sj >= 0) { const T q = bej * pc.qzy + cej * dHx; if (st) stg<T>(psiB[psi_id(F_EZ, 1, sj)], iy, q); vz -= cb * q; }
-        }
+      const bool srcp = srcE && i >= sx0 && i < sx1;
+      if (iin1 && jin) {
+        const T cb = cb_of(cx0, cx1, cx2, cx3, pc.m, 0);
+        const T dHz = pc.hzj - pc.hz;
+        const T dHy = hyk - pc.hy;
+        T v = vx + cb * (dHz * iky - dHy * ikz);
+        if (sj >= 0) { const T q = bej * pc.qxy + cej * dHz; if (st && kin) stg<T>(psiB[psi_id(F_EX, 1, sj)], iy, q); v += cb * q; }
+        { const T q = bek * pc.qxz + cek * dHy; if (st && sk >= 0) stg<T>(psiB[psi_id(F_EX, 2, 0)], iz, q); v -= cb * q; }
+        vx = kin ? v : vx;
+      }
+      if (iin && jin1) {
+        const T cb = cb_of(cy0, cy1, cy2, cy3, pc.m, 1);
+        const T dHx = hxk - pc.hx;
+        const T dHz = pc.hz1 - pc.hz;
+        T v = vy + cb * (dHx * ikz - dHz * ikx);
+        v = srcp ? v - jv : v;
+        if (si >= 0) { const T q = bex * pc.qyx + cex * dHz; if (st && kin) stg<T>(psiB[psi_id(F_EY, 0, 0)], ix, q); v -= cb * q; }
+        { const T q = bek * pc.qyz + cek * dHx; if (st && sk >= 0) stg<T>(psiB[psi_id(F_EY, 2, 0)], iz, q); v += cb * q; }
+        vy = kin ? v : (srcp ? vy - jv : vy);
+      } else {
+        vy = srcp ? vy - jv : vy;
+      }
+      if (iin && jin) {
+        const T cb = cb_of(cz0, cz1, cz2, cz3, pc.m, 2);
+        const T dHy = pc.hy1 - pc.hy;
+        const T dHx = pc.hxj - pc.hx;
+        T v = vz + cb * (dHy * ikx - dHx * iky);
+        if (si >= 0) { const T q = bex * pc.qzx + cex * dHy; if (st && kin1) stg<T>(psiB[psi_id(F_EZ, 0, 0)], ix, q); v += cb * q; }
+        if (sj >= 0) { const T q = bej * pc.qzy + cej * dHx; if (st && kin1) stg<T>(psiB[psi_id(F_EZ, 1, sj)], iy, q); v -= cb * q; }
+        vz = kin1 ? v : vz;
       }
       if (own) {
         // ---- new Ex, Ez of row j-1 (its owner's per-cell code, not stored)
         T vxm = pc.ex_, vzm = pc.ez_;
-        {
-          const T hykm = lane_down1(pc.hy_, pc.hyt_);
-          if (iin1 && jinm && kin) {
-            const T cb = cb_of(cx0, cx1, cx2, cx3, pc.m_, 0);
-            const T dHz = pc.hz - pc.hz_;
-            const T dHy = hykm - pc.hy_;
-            vxm = vxm + cb * (dHz * ikym - dHy * ikz);
-            if (sjm >= 0) { const T q = bejm * pc.pxy_ + cejm * dHz; vxm += cb * q; }
-            if (sk >= 0) { const T q = bek * pc.pxz_ + cek * dHy; vxm -= cb * q; }
-          }
-          if (iin && jinm && kin1) {
-            const T cb = cb_of(cz0, cz1, cz2, cz3, pc.m_, 2);
-            const T dHy = pc.hy1_ - pc.hy_;
-            const T dHx = pc.hx - pc.hx_;
-            vzm = vzm + cb * (dHy * ikx - dHx * ikym);
-            if (si >= 0) { const T q = bex * pc.pzx_ + cex * dHy; vzm += cb * q; }
-            if (sjm >= 0) { const T q = bejm * pc.pzy_ + cejm * dHx; vzm -= cb * q; }
-          }
+        const T hykm = lane_down1(pc.hy_, pc.hyt_);
+        if (iin1 && jinm) {
+          const T cb = cb_of(cx0, cx1, cx2, cx3, pc.m_, 0);
+          const T dHz = pc.hz - pc.hz_;
+          const T dHy = hykm - pc.hy_;
+          T v = vxm + cb * (dHz * ikym - dHy * ikz);
+          if (sjm >= 0) { const T q = bejm * pc.pxy_ + cejm * dHz; v += cb * q; }
+          { const T q = bek * pc.pxz_ + cek * dHy; v -= cb * q; }
+          vxm = kin ? v : vxm;
+        }
+        if (iin && jinm) {
+          const T cb = cb_of(cz0, cz1, cz2, cz3, pc.m_, 2);
+          const T dHy = pc.hy1_ - pc.hy_;
+          const T dHx = pc.hx - pc.hx_;
+          T v = vzm + cb * (dHy * ikx - dHx * ikym);
+          if (si >= 0) { const T q = bex * pc.pzx_ + cex * dHy; v += cb * q; }
+          if (sjm >= 0) { const T q = bejm * pc.pzy_ + cejm * dHx; v -= cb * q; }
+          vzm = kin1 ? v : vzm;
         }
         // ---- new H, row j (k_3d_h per cell)
         const T exk = lane_up1(vx, vx), eyk = lane_up1(vy, vy);
@@ -2652,34 +2653,33 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
         const unsigned ixh = (((unsigned)dih * ny + j) * nz + kc) * es + (sih == 1 ? xhi : 0u);
         const unsigned iyh = (((unsigned)i * N + djh) * nz + kc) * es;
         const unsigned izh = (((unsigned)i * ny + j) * N + dkh) * es + zsideh;
-        const bool bx = iin && jin1 && kin1;
-        const bool by = iin1 && jin && kin1;
-        const bool bz = iin1 && jin1 && kin;
-        const bool src = srcH && i >= sx0 && i < sx1;
+        const bool srch = srcH && i >= sx0 && i < sx1;
         T hx = pc.hx;
-        if (bx) {
+        if (iin && jin1) {
           const T dEy = vy - eyk;
           const T dEz = vz - vzm;
-          hx = hx + db * (dEy * ikzh - dEz * ikyh);
-          if (sjh >= 0) { const T q = bhj * pc.rxy + chj * dEz; if (owner) stg<T>(psiB[psi_id(F_HX, 1, sjh)], iyh, q); hx -= db * q; }
-          if (skh >= 0) { const T q = bhk * pc.rxz + chk * dEy; if (owner) stg<T>(psiB[psi_id(F_HX, 2, 0)], izh, q); hx += db * q; }
+          T v = hx + db * (dEy * ikzh - dEz * ikyh);
+          if (sjh >= 0) { const T q = bhj * pc.rxy + chj * dEz; if (owner && kin1) stg<T>(psiB[psi_id(F_HX, 1, sjh)], iyh, q); v -= db * q; }
+          { const T q = bhk * pc.rxz + chk * dEy; if (owner && skh >= 0) stg<T>(psiB[psi_id(F_HX, 2, 0)], izh, q); v += db * q; }
+          hx = kin1 ? v : hx;
         }
-        T hy = pc.hy;
-        if (src) hy -= jv;
-        if (by) {
+        T hy = srch ? pc.hy - jv : pc.hy;
+        if (iin1 && jin) {
           const T dEz = vz - ezp;
           const T dEx = vx - exk;
-          hy = hy + db * (dEz * ikxh - dEx * ikzh);
-          if (sih >= 0) { const T q = bhx * pc.ryx + chx * dEz; if (owner) stg<T>(psiB[psi_id(F_HY, 0, 0)], ixh, q); hy += db * q; }
-          if (skh >= 0) { const T q = bhk * pc.ryz + chk * dEx; if (owner) stg<T>(psiB[psi_id(F_HY, 2, 0)], izh, q); hy -= db * q; }
+          T v = hy + db * (dEz * ikxh - dEx * ikzh);
+          if (sih >= 0) { const T q = bhx * pc.ryx + chx * dEz; if (owner && kin1) stg<T>(psiB[psi_id(F_HY, 0, 0)], ixh, q); v += db * q; }
+          { const T q = bhk * pc.ryz + chk * dEx; if (owner && skh >= 0) stg<T>(psiB[psi_id(F_HY, 2, 0)], izh, q); v -= db * q; }
+          hy = kin1 ? v : hy;
         }
         T hz = pc.hz;
-        if (bz) {
+        if (iin1 && jin1) {
           const T dEx = vx - vxm;
           const T dEy = vy - eyp;
-          hz = hz + db * (dEx * ikyh - dEy * ikxh);
-          if (sih >= 0) { const T q = bhx * pc.rzx + chx * dEy; if (owner) stg<T>(psiB[psi_id(F_HZ, 0, 0)], ixh, q); hz -= db * q; }
-          if (sjh >= 0) { const T q = bhj * pc.rzy + chj * dEx; if (owner) stg<T>(psiB[psi_id(F_HZ, 1, sjh)], iyh, q); hz += db * q; }
+          T v = hz + db * (dEx * ikyh - dEy * ikxh);
+          if (sih >= 0) { const T q = bhx * pc.rzx + chx * dEy; if (owner && kin) stg<T>(psiB[psi_id(F_HZ, 0, 0)], ixh, q); v -= db * q; }
+          if (sjh >= 0) { const T q = bhj * pc.rzy + chj * dEx; if (owner && kin) stg<T>(psiB[psi_id(F_HZ, 1, sjh)], iyh, q); v += db * q; }
+          hz = kin ? v : hz;
         }
         if (owner) {
           stg_nt<T>(ExB, bo, vx);
@@ -2709,616 +2709,6 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
       ezp = vz;
       if (u < XM && i + 1 < iend) pc = pn;
     }
-  }
-}
-
-// ------------------------------------------------------- fused 3D step ---
-// One launch per time step for 3D: both half-steps, reading field/psi set
-// (n & 1) and writing the other, so no block races another.  A block of W
-// waves owns R rows (y) x the whole padded z extent x XM planes (x): each row
-// is covered by ZW = ceil(nz / 128) waves of 64 lanes, a lane holding 2
-// consecutive z-cells (one 8-byte access per field in FP32), so R = W / ZW.
-// Row 0 of the block is the row below the tile (j0 - 1): it computes new E
-// only (not stored), which the first tile row's H update needs; rows 1..R-1
-// are the tile's own.  The block marches along x; per plane i:
-//   1. every wave writes its old H (plane i) to the LDS row buffer (row R:
-//      the row above the tile, Hx and Hz, for the last row's E update);
-//   2. barrier;
-//   3. new E of plane i: j+1 neighbours from the LDS rows, k+1 from the
-//      lane's next cell (the next lane's first cell from LDS), x+1 from the
-//      old H of plane i+1 held in registers;
-//   4. new E to the LDS row buffer, then the loads of plane i+1 are issued;
-//   5. barrier;
-//   6. new H of the tile rows: j-1 and k-1 neighbours from the LDS rows and
-//      the lane's previous cell, x-1 from the previous plane's new E in
-//      registers; E and H of the tile rows stored to the other set.
-// Before the first plane the block computes new E of plane i0 - 1 (not
-// stored) the same way.  Halo cells (row j0 - 1, plane i0 - 1) go through the
-// same per-cell code as their owners, so they are bit-identical to them.
-// The per-cell arithmetic and the update ranges are those of k_3d_e and
-// k_3d_h (reference order: curl, source, psi of the first axis, psi of the
-// second axis; the TM source on Hy at the start of the H half-step).
-// Field traffic per cell and step: each component read once and written
-// once (24 + 24 B in FP32) plus the halo row, the material byte and psi.
-
-// Waves per block of the fused 3D step: 16 in FP32, 8 in FP64; z-cells per
-// lane: 2.  The LDS rows of a block take (6 R + 3) x ZW x 128 cells, at most
-// 61 KB for a padded z extent up to 1024 (FP32) or 512 (FP64).
-template <typename T>
-struct Fused3d {
-  static constexpr int W = sizeof(T) == 4 ? 16 : 8;
-  static constexpr int C = 2;                       // z-cells per lane
-  static constexpr int ZSEG = 64 * C;               // z-cells per wave
-  static constexpr int ZWMAX = sizeof(T) == 4 ? 8 : 4;
-  static constexpr int lds_cells(int zw) { return (6 * (W / zw) + 3) * zw * ZSEG; }
-  static constexpr int max_cells(int zw) {
-    return zw > ZWMAX ? 0 : (lds_cells(zw) > max_cells(zw + 1) ? lds_cells(zw) : max_cells(zw + 1));
-  }
-  static constexpr int LDS = max_cells(1);
-};
-
-// The lane's consecutive z-cells of one field.
-template <typename T>
-struct Q4 {
-  T v[Fused3d<T>::C];
-};
-
-typedef float oidt_f32x2 __attribute__((ext_vector_type(2)));
-typedef float oidt_f32x4 __attribute__((ext_vector_type(4)));
-typedef double oidt_f64x2 __attribute__((ext_vector_type(2)));
-
-// C cells as 8- or 16-byte vector accesses (2 x 16 bytes for 4 doubles).
-template <typename T, typename F>
-__device__ __forceinline__ void q4_chunks(F&& f) {
-  constexpr int C = Fused3d<T>::C;
-  if constexpr (C * sizeof(T) == 8) f(oidt_f32x2{}, 0);
-  else if constexpr (sizeof(T) == 4) f(oidt_f32x4{}, 0);
-  else {
-    f(oidt_f64x2{}, 0);
-    if constexpr (C == 4) f(oidt_f64x2{}, 2);
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ Q4<T> ld4(const T* base, unsigned boff) {
-  Q4<T> q;
-  q4_chunks<T>([&](auto v, int c0) {
-    using V = decltype(v);
-    constexpr int n = sizeof(V) / sizeof(T);
-    const V a = *(const OIDT_GLOBAL V*)(reinterpret_cast<const char*>(base) + boff + c0 * sizeof(T));
-#pragma unroll
-    for (int c = 0; c < n; ++c) q.v[c0 + c] = a[c];
-  });
-  return q;
-}
-
-// Non-temporal store of the lane's cells (nothing in the same launch reads them).
-template <typename T>
-__device__ __forceinline__ void st4_nt(T* base, unsigned boff, const Q4<T>& q) {
-  q4_chunks<T>([&](auto v, int c0) {
-    using V = decltype(v);
-    constexpr int n = sizeof(V) / sizeof(T);
-    V a;
-#pragma unroll
-    for (int c = 0; c < n; ++c) a[c] = q.v[c0 + c];
-    __builtin_nontemporal_store(a, (OIDT_GLOBAL V*)(reinterpret_cast<char*>(base) + boff + c0 * sizeof(T)));
-  });
-}
-
-template <typename T>
-__device__ __forceinline__ void lds_put4(T* p, const Q4<T>& q) {
-  q4_chunks<T>([&](auto v, int c0) {
-    using V = decltype(v);
-    constexpr int n = sizeof(V) / sizeof(T);
-    V a;
-#pragma unroll
-    for (int c = 0; c < n; ++c) a[c] = q.v[c0 + c];
-    *reinterpret_cast<V*>(p + c0) = a;
-  });
-}
-
-template <typename T>
-__device__ __forceinline__ Q4<T> lds_get4(const T* p) {
-  Q4<T> q;
-  q4_chunks<T>([&](auto v, int c0) {
-    using V = decltype(v);
-    constexpr int n = sizeof(V) / sizeof(T);
-    const V a = *reinterpret_cast<const V*>(p + c0);
-#pragma unroll
-    for (int c = 0; c < n; ++c) q.v[c0 + c] = a[c];
-  });
-  return q;
-}
-
-// Old values a cell group loads for one plane (set A): E, the material bytes
-// and the psi of the E update (x and y slabs contiguous in z; z slab per
-// cell), and separately the psi of the H update.
-template <typename T>
-struct E3Old {
-  Q4<T> ex, ey, ez;
-  unsigned m;                 // material bytes of the lane's cells
-  Q4<T> pyx, pzx, pxy, pzy, pxz, pyz;   // psi ey_x, ez_x, ex_y, ez_y, ex_z, ey_z
-};
-template <typename T>
-struct H3Old {
-  Q4<T> pyx, pzx, pxy, pzy, pxz, pyz;   // psi hy_x, hz_x, hx_y, hz_y, hx_z, hy_z
-};
-
-template <typename T>
-__device__ __forceinline__ Q4<T> q4_zero() {
-  Q4<T> q;
-#pragma unroll
-  for (int c = 0; c < Fused3d<T>::C; ++c) q.v[c] = T(0);
-  return q;
-}
-
-template <typename T>
-__global__ __launch_bounds__(64 * Fused3d<T>::W) void k_3d_fused(const Slot<T>* __restrict__ slots,
-                                                                 const int4* __restrict__ tiles,
-                                                                 const int* __restrict__ counts, int tstep,
-                                                                 int xm) {
-  constexpr int W = Fused3d<T>::W;
-  constexpr int C = Fused3d<T>::C;
-  constexpr int ZSEG = Fused3d<T>::ZSEG;
-  constexpr unsigned es = sizeof(T);
-  __shared__ T lds[Fused3d<T>::LDS];
-  // the run's CPML coefficients (be, ce/dx, bh, ch/dx), staged per tile: the
-  // z-slab cells read them per cell, and an LDS read waits only on the LDS
-  // counter, not on the plane loads in flight
-  __shared__ T cpml[4][kCpmlMax];
-  const int ntiles = counts[0];
-  const int lane = threadIdx.x;
-  const int w = wave_id();
-  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
-    const int4 tl = tiles[tt];
-    const Slot<T>& s = slots[tl.x];
-    const int n = s.base + tstep;
-    if (!s.active || n >= s.nt) continue;     // uniform over the block: every wave skips together
-    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
-    const int zw = (nz + ZSEG - 1) / ZSEG;     // waves per row (nz: padded extent)
-    const int R = W / zw;                      // rows of the block, the halo row first
-    const int NZS = zw * ZSEG;                 // LDS row stride (cells)
-    const int r = w / zw, zi = w - r * zw;
-    const int i0 = tl.y, j0 = tl.z;
-    const int iend = min(i0 + xm, nx);
-    const int j = j0 - 1 + r;
-    const int k = zi * ZSEG + C * lane;        // first of the lane's cells
-    const bool inrow = r < R;                  // waves past R * zw (when zw does not divide W) idle
-    const bool live = inrow && j >= 0 && j < ny && k < nz;
-    const bool owner = live && r >= 1;         // stores E and H (the halo row only computes E)
-    const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
-    const bool odd = (n & 1) != 0;
-    OIDT_GLOBAL T* const* FA = odd ? s.f2 : s.f;
-    OIDT_GLOBAL T* const* FB = odd ? s.f : s.f2;
-    const T* __restrict__ ExA = FA[F_EX];
-    const T* __restrict__ EyA = FA[F_EY];
-    const T* __restrict__ EzA = FA[F_EZ];
-    const T* __restrict__ HxA = FA[F_HX];
-    const T* __restrict__ HyA = FA[F_HY];
-    const T* __restrict__ HzA = FA[F_HZ];
-    T* __restrict__ ExB = FB[F_EX];
-    T* __restrict__ EyB = FB[F_EY];
-    T* __restrict__ EzB = FB[F_EZ];
-    T* __restrict__ HxB = FB[F_HX];
-    T* __restrict__ HyB = FB[F_HY];
-    T* __restrict__ HzB = FB[F_HZ];
-    OIDT_GLOBAL T* const* psiA = odd ? s.psi2 : s.psi;
-    OIDT_GLOBAL T* const* psiB = odd ? s.psi : s.psi2;
-    const uint8_t* __restrict__ mat = s.mat;
-    const T db = s.db;
-    const unsigned pl = (unsigned)ny * nz;
-    const int jc = min(max(j, 0), ny - 1);
-    const unsigned kq = (unsigned)min(k, nz - C);
-    const unsigned jk = (unsigned)jc * nz + kq;
-    T* const sH = lds;                         // old H: 3 components x (R + 1) rows
-    T* const sE = lds + 3 * (R + 1) * NZS;     // new E: 3 components x R rows
-    // ---- row (wave-uniform) coefficients
-    const bool jinE = j >= jb && j < je, jin1 = j > jb && j < je;
-    int dje = 0, cje = 0, djh = 0, cjh = 0;
-    const int sjE = jinE ? slab_e(j, jb, je, N, dje, cje) : -1;
-    const int sjH = jin1 ? slab_h(j, jb, je, N, djh, cjh) : -1;
-    const T ikyE = jinE ? s.ike[1][j] : T(0);
-    const T ikyH = jin1 ? s.ikh[1][j] : T(0);
-    const T bej = sjE >= 0 ? s.be[cje] : T(0), cej = sjE >= 0 ? s.ceo[cje] : T(0);
-    const T bhj = sjH >= 0 ? s.bh[cjh] : T(0), chj = sjH >= 0 ? s.cho[cjh] : T(0);
-    // ---- z coefficients of the lane's cells (E: k in [kb, ke); H: k in (kb, ke))
-    T ikzE[C], ikzH[C];
-    bool zslab = false;
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const int kk = k + c;
-      const bool kin = kk >= kb && kk < ke, kin1 = kk > kb && kk < ke;
-      ikzE[c] = live && kin ? s.ike[2][kk] : T(0);
-      ikzH[c] = live && kin1 ? s.ikh[2][kk] : T(0);
-      zslab = zslab || (kk >= kb && kk < kb + N) || (kk >= ke - N && kk < ke);
-    }
-    // ---- x coefficients: lane u holds plane i0 - 1 + u
-    T lx_ikE = T(0), lx_bE = T(0), lx_cE = T(0), lx_ikH = T(0), lx_bH = T(0), lx_cH = T(0);
-    {
-      const int ip = i0 - 1 + lane;
-      if (lane <= xm && ip >= 0 && ip < nx) {
-        int d = 0, ci = 0;
-        lx_ikE = s.ike[0][ip];
-        if (slab_e(ip, ib, ie, N, d, ci) >= 0) { lx_bE = s.be[ci]; lx_cE = s.ceo[ci]; }
-        lx_ikH = s.ikh[0][ip];
-        if (slab_h(ip, ib, ie, N, d, ci) >= 0) { lx_bH = s.bh[ci]; lx_cH = s.cho[ci]; }
-      }
-    }
-    const T cx0 = s.cbv[0][0], cx1 = s.cbv[0][1], cx2 = s.cbv[0][2], cx3 = s.cbv[0][3];
-    const T cy0 = s.cbv[1][0], cy1 = s.cbv[1][1], cy2 = s.cbv[1][2], cy3 = s.cbv[1][3];
-    const T cz0 = s.cbv[2][0], cz1 = s.cbv[2][1], cz2 = s.cbv[2][2], cz3 = s.cbv[2][3];
-    // sources: TE on Ey (E half-step), TM on Hy (start of the H half-step)
-    const bool jsrcE = s.src_field == F_EY && j >= s.sy0 && j < s.sy1;
-    const bool jsrcH = s.src_field == F_HY && j >= s.sy0 && j < s.sy1;
-    const T jv = (jsrcE || jsrcH) ? s.jsrc[n] : T(0);
-    const int sz0 = s.sz0, sz1 = s.sz1, sx0 = s.sx0, sx1 = s.sx1;
-    // capture window (rows and planes inclusive of wy1 / wx1) and port planes
-    const bool jcap = owner && j >= s.wy0 && j <= s.wy1;
-    const int wx0 = s.wx0, wx1 = s.wx1, prow = s.prow, pcells = s.pcells, nplanes = s.nplanes, wy0 = s.wy0;
-    const int port0 = s.port[0], port1 = s.port[1];
-    T* const ring = s.ring;
-    const unsigned xhi = s.psi_hi[0], zhi = s.psi_hi[2];
-    if (w == 0 && lane < N) {
-      cpml[0][lane] = s.be[lane];
-      cpml[1][lane] = s.ceo[lane];
-      cpml[2][lane] = s.bh[lane];
-      cpml[3][lane] = s.cho[lane];
-    }
-    const T* const beT = cpml[0];
-    const T* const ceT = cpml[1];
-    const T* const bhT = cpml[2];
-    const T* const chT = cpml[3];
-
-    // x slab of plane i: side (-1: none), offset of the lane's cells in the slab arrays
-    auto xslab = [&](int i, unsigned& ix) {
-      int di = 0, ci = 0;
-      const int si = slab_e(i, ib, ie, N, di, ci);    // E and H psi share the cell index d
-      ix = (((unsigned)di * ny + jc) * nz + kq) * es + (si == 1 ? xhi : 0u);
-      return si;
-    };
-    auto zoff = [&](int i, int kk, bool hside, int& sk, int& ck) {
-      int dk = 0;
-      ck = 0;
-      sk = hside ? ((kk > kb && kk < ke) ? slab_h(kk, kb, ke, N, dk, ck) : -1)
-                 : ((kk >= kb && kk < ke) ? slab_e(kk, kb, ke, N, dk, ck) : -1);
-      return (((unsigned)i * ny + jc) * N + dk) * es + (sk == 1 ? zhi : 0u);
-    };
-    // E-update values of plane i (set A)
-    auto loadE = [&](int i, E3Old<T>& p) {
-      const unsigned idx = (unsigned)i * pl + jk;
-      const unsigned bo = idx * es;
-      p.ex = ld4<T>(ExA, bo);
-      p.ey = ld4<T>(EyA, bo);
-      p.ez = ld4<T>(EzA, bo);
-      if constexpr (C == 2) p.m = *(const OIDT_GLOBAL unsigned short*)(reinterpret_cast<const char*>(mat) + idx);
-      else p.m = *(const OIDT_GLOBAL unsigned*)(reinterpret_cast<const char*>(mat) + idx);
-      unsigned ix;
-      if (xslab(i, ix) >= 0) {
-        p.pyx = ld4<T>(psiA[psi_id(F_EY, 0, 0)], ix);
-        p.pzx = ld4<T>(psiA[psi_id(F_EZ, 0, 0)], ix);
-      }
-      if (sjE >= 0) {
-        const unsigned iy = (((unsigned)i * N + dje) * nz + kq) * es;
-        p.pxy = ld4<T>(psiA[psi_id(F_EX, 1, sjE)], iy);
-        p.pzy = ld4<T>(psiA[psi_id(F_EZ, 1, sjE)], iy);
-      }
-      if (zslab) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          int sk, ck;
-          const unsigned iz = zoff(i, k + c, false, sk, ck);
-          if (sk >= 0) {
-            p.pxz.v[c] = ldg<T>(psiA[psi_id(F_EX, 2, 0)], iz);
-            p.pyz.v[c] = ldg<T>(psiA[psi_id(F_EY, 2, 0)], iz);
-          }
-        }
-      }
-    };
-    // H-update psi of plane i (set A)
-    auto loadHpsi = [&](int i, H3Old<T>& p) {
-      unsigned ix;
-      if (xslab(i, ix) >= 0) {
-        p.pyx = ld4<T>(psiA[psi_id(F_HY, 0, 0)], ix);
-        p.pzx = ld4<T>(psiA[psi_id(F_HZ, 0, 0)], ix);
-      }
-      if (sjH >= 0) {
-        const unsigned iy = (((unsigned)i * N + djh) * nz + kq) * es;
-        p.pxy = ld4<T>(psiA[psi_id(F_HX, 1, sjH)], iy);
-        p.pzy = ld4<T>(psiA[psi_id(F_HZ, 1, sjH)], iy);
-      }
-      if (zslab) {
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          int sk, ck;
-          const unsigned iz = zoff(i, k + c, true, sk, ck);
-          if (sk >= 0) {
-            p.pxz.v[c] = ldg<T>(psiA[psi_id(F_HX, 2, 0)], iz);
-            p.pyz.v[c] = ldg<T>(psiA[psi_id(F_HY, 2, 0)], iz);
-          }
-        }
-      }
-    };
-    auto loadH = [&](int i, Q4<T>& hx, Q4<T>& hy, Q4<T>& hz) {
-      const unsigned bo = ((unsigned)i * pl + jk) * es;
-      hx = ld4<T>(HxA, bo);
-      hy = ld4<T>(HyA, bo);
-      hz = ld4<T>(HzA, bo);
-    };
-    // the row above the tile (row R of the block): old Hx, Hz of plane i,
-    // loaded by the halo row's waves
-    const int jtop = j0 - 1 + R;
-    const bool toprow = r == 0 && jtop < ny && k < nz;   // (jtop >= 0: R >= 2)
-    auto loadTop = [&](int i, Q4<T>& hx, Q4<T>& hz) {
-      const unsigned bo = ((unsigned)i * pl + (unsigned)jtop * nz + kq) * es;
-      hx = ld4<T>(HxA, bo);
-      hz = ld4<T>(HzA, bo);
-    };
-
-    // new E of plane i for the lane's cells; `store`: write its psi to set B
-    auto updE = [&](int i, const E3Old<T>& p, const Q4<T>& hy1, const Q4<T>& hz1, bool store, Q4<T>& ex, Q4<T>& ey,
-                    Q4<T>& ez) {
-      // the lane's own old H, from the LDS row it wrote
-      const Q4<T> hx = lds_get4(sH + (0 * (R + 1) + r) * NZS + k);
-      const Q4<T> hy = lds_get4(sH + (1 * (R + 1) + r) * NZS + k);
-      const Q4<T> hz = lds_get4(sH + (2 * (R + 1) + r) * NZS + k);
-      const int u = i - (i0 - 1);
-      const T ikx = lane_get(lx_ikE, u), bex = lane_get(lx_bE, u), cex = lane_get(lx_cE, u);
-      unsigned ix;
-      const int si = xslab(i, ix);
-      const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
-      const bool isrc = i >= sx0 && i < sx1;
-      // neighbours j+1 (LDS row r + 1) and k+1 (next cell; the next lane's first from LDS)
-      const Q4<T> hxj = lds_get4(sH + (0 * (R + 1) + r + 1) * NZS + k);
-      const Q4<T> hzj = lds_get4(sH + (2 * (R + 1) + r + 1) * NZS + k);
-      const int kn = min(k + C, NZS - 1);
-      const T hxk4 = sH[(0 * (R + 1) + r) * NZS + kn], hyk4 = sH[(1 * (R + 1) + r) * NZS + kn];
-      Q4<T> qyx = p.pyx, qzx = p.pzx, qxy = p.pxy, qzy = p.pzy;
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int kk = k + c;
-        const bool kin = kk >= kb && kk < ke, kin1 = kk > kb && kk < ke;
-        const bool bx = iin1 && jinE && kin;
-        const bool by = iin && jin1 && kin;
-        const bool bz = iin && jinE && kin1;
-        const bool src = jsrcE && isrc && kk >= sz0 && kk < sz1;
-        const unsigned mc = (p.m >> (8 * c)) & 0xffu;
-        const T hxk = c < C - 1 ? hx.v[c + 1] : hxk4;
-        const T hyk = c < C - 1 ? hy.v[c + 1] : hyk4;
-        const T hz0 = hz.v[c], hy0 = hy.v[c];
-        const T ikz = ikzE[c];
-        int sk = -1, ck = 0;
-        unsigned iz = 0;
-        if (zslab) iz = zoff(i, kk, false, sk, ck);
-        T vx = p.ex.v[c], vy = p.ey.v[c], vz = p.ez.v[c];
-        // update_ex: i (ib,ie), j [jb,je), k [kb,ke); psi ex_y (+), ex_z (-)
-        if (bx) {
-          const T cb = cb_of(cx0, cx1, cx2, cx3, mc, 0);
-          const T dHz = hzj.v[c] - hz0;
-          const T dHy = hyk - hy0;
-          vx = vx + cb * (dHz * ikyE - dHy * ikz);
-          if (sjE >= 0) { const T q = bej * p.pxy.v[c] + cej * dHz; qxy.v[c] = q; vx += cb * q; }
-          if (sk >= 0) {
-            const T q = beT[ck] * p.pxz.v[c] + ceT[ck] * dHy;
-            if (store) stg<T>(psiB[psi_id(F_EX, 2, 0)], iz, q);
-            vx -= cb * q;
-          }
-        }
-        // update_ey: i [ib,ie), j (jb,je), k [kb,ke); source (TE); psi ey_x (-), ey_z (+)
-        if (by) {
-          const T cb = cb_of(cy0, cy1, cy2, cy3, mc, 1);
-          const T dHx = hxk - hx.v[c];
-          const T dHz = hz1.v[c] - hz0;
-          vy = vy + cb * (dHx * ikz - dHz * ikx);
-          if (src) vy -= jv;
-          if (si >= 0) { const T q = bex * p.pyx.v[c] + cex * dHz; qyx.v[c] = q; vy -= cb * q; }
-          if (sk >= 0) {
-            const T q = beT[ck] * p.pyz.v[c] + ceT[ck] * dHx;
-            if (store) stg<T>(psiB[psi_id(F_EY, 2, 0)], iz, q);
-            vy += cb * q;
-          }
-        } else if (src) {
-          vy -= jv;
-        }
-        // update_ez: i [ib,ie), j [jb,je), k (kb,ke); psi ez_x (+), ez_y (-)
-        if (bz) {
-          const T cb = cb_of(cz0, cz1, cz2, cz3, mc, 2);
-          const T dHy = hy1.v[c] - hy0;
-          const T dHx = hxj.v[c] - hx.v[c];
-          vz = vz + cb * (dHy * ikx - dHx * ikyE);
-          if (si >= 0) { const T q = bex * p.pzx.v[c] + cex * dHy; qzx.v[c] = q; vz += cb * q; }
-          if (sjE >= 0) { const T q = bej * p.pzy.v[c] + cej * dHx; qzy.v[c] = q; vz -= cb * q; }
-        }
-        ex.v[c] = vx;
-        ey.v[c] = vy;
-        ez.v[c] = vz;
-      }
-      if (store) {
-        // x- and y-slab psi of the lane's cells: cells outside a component's
-        // range keep their value (0: never updated)
-        if (si >= 0) {
-          st4_nt<T>(psiB[psi_id(F_EY, 0, 0)], ix, qyx);
-          st4_nt<T>(psiB[psi_id(F_EZ, 0, 0)], ix, qzx);
-        }
-        if (sjE >= 0) {
-          const unsigned iy = (((unsigned)i * N + dje) * nz + kq) * es;
-          st4_nt<T>(psiB[psi_id(F_EX, 1, sjE)], iy, qxy);
-          st4_nt<T>(psiB[psi_id(F_EZ, 1, sjE)], iy, qzy);
-        }
-      }
-    };
-
-    Q4<T> hxc, hyc, hzc;        // old H of the plane being updated (to the LDS rows)
-    Q4<T> hxn, hyn, hzn;        // old H of the next plane
-    Q4<T> tpx, tpz;             // old Hx, Hz of the row above the tile, plane being updated
-    Q4<T> eyp, ezp;             // new Ey, Ez of the previous plane
-    E3Old<T> pc{};
-    hxc = hyc = hzc = hxn = hyn = hzn = tpx = tpz = eyp = ezp = q4_zero<T>();
-    const int ifirst = i0 > 0 ? i0 - 1 : i0;   // the plane below the tile: new E only
-    if (live) {
-      loadE(ifirst, pc);
-      loadH(ifirst, hxc, hyc, hzc);
-      loadH(min(ifirst + 1, nx - 1), hxn, hyn, hzn);
-    }
-    if (toprow) loadTop(ifirst, tpx, tpz);
-    for (int i = ifirst; i < iend; ++i) {
-      const bool pro = i < i0;
-      // 1. old H of plane i to the LDS rows
-      if (inrow) {
-        lds_put4(sH + (0 * (R + 1) + r) * NZS + k, hxc);
-        lds_put4(sH + (1 * (R + 1) + r) * NZS + k, hyc);
-        lds_put4(sH + (2 * (R + 1) + r) * NZS + k, hzc);
-        if (r == 0) {
-          lds_put4(sH + (0 * (R + 1) + R) * NZS + k, tpx);
-          lds_put4(sH + (2 * (R + 1) + R) * NZS + k, tpz);
-        }
-      }
-      lds_barrier();
-      // 2. new E of plane i
-      Q4<T> exn, eyn, ezn;
-      if (inrow) updE(i, pc, hyn, hzn, owner && !pro, exn, eyn, ezn);
-      // 3. new E to the LDS rows; H psi of plane i and the loads of plane i + 1
-      if (inrow) {
-        lds_put4(sE + (0 * R + r) * NZS + k, exn);
-        lds_put4(sE + (1 * R + r) * NZS + k, eyn);
-        lds_put4(sE + (2 * R + r) * NZS + k, ezn);
-      }
-      H3Old<T> ph{};
-      const bool hrow_ = owner && !pro;
-      if (hrow_) loadHpsi(i, ph);
-      const bool more = i + 1 < iend;
-      E3Old<T> pn{};
-      Q4<T> hxn2 = hxn, hyn2 = hyn, hzn2 = hzn, tpxn = tpx, tpzn = tpz;
-      if (more) {
-        if (live) {
-          loadE(i + 1, pn);
-          loadH(min(i + 2, nx - 1), hxn2, hyn2, hzn2);
-        }
-        if (toprow) loadTop(i + 1, tpxn, tpzn);
-      }
-      lds_barrier();
-      if (hrow_) {
-        // 4. new H of plane i, tile rows
-        const int u = i - (i0 - 1);
-        const T ikx = lane_get(lx_ikH, u), bhx = lane_get(lx_bH, u), chx = lane_get(lx_cH, u);
-        int di = 0, ci = 0;
-        const int si = slab_h(i, ib, ie, N, di, ci);
-        unsigned ix;
-        (void)xslab(i, ix);
-        const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
-        const bool isrc = i >= sx0 && i < sx1;
-        const Q4<T> exj = lds_get4(sE + (0 * R + r - 1) * NZS + k);
-        const Q4<T> ezj = lds_get4(sE + (2 * R + r - 1) * NZS + k);
-        const int kp = max(k - 1, 0);
-        const T exk0 = sE[(0 * R + r) * NZS + kp], eyk0 = sE[(1 * R + r) * NZS + kp];
-        Q4<T> hxo, hyo, hzo;
-        Q4<T> qyx = ph.pyx, qzx = ph.pzx, qxy = ph.pxy, qzy = ph.pzy;
-        // the lane's own old H, from the LDS row it wrote
-        const Q4<T> hxc = lds_get4(sH + (0 * (R + 1) + r) * NZS + k);
-        const Q4<T> hyc = lds_get4(sH + (1 * (R + 1) + r) * NZS + k);
-        const Q4<T> hzc = lds_get4(sH + (2 * (R + 1) + r) * NZS + k);
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const int kk = k + c;
-          const bool kin = kk >= kb && kk < ke, kin1 = kk > kb && kk < ke;
-          const bool bx = iin && jin1 && kin1;
-          const bool by = iin1 && jinE && kin1;
-          const bool bz = iin1 && jin1 && kin;
-          const bool src = jsrcH && isrc && kk >= sz0 && kk < sz1;
-          const T ikz = ikzH[c];
-          const T ex = exn.v[c], ey = eyn.v[c], ez = ezn.v[c];
-          const T exk = c > 0 ? exn.v[c - 1] : exk0;
-          const T eyk = c > 0 ? eyn.v[c - 1] : eyk0;
-          int sk = -1, ck = 0;
-          unsigned iz = 0;
-          if (zslab) iz = zoff(i, kk, true, sk, ck);
-          // update_hx: i [ib,ie), j (jb,je), k (kb,ke); psi hx_y (-), hx_z (+)
-          T vx = hxc.v[c];
-          if (bx) {
-            const T dEy = ey - eyk;
-            const T dEz = ez - ezj.v[c];
-            vx = vx + db * (dEy * ikz - dEz * ikyH);
-            if (sjH >= 0) { const T q = bhj * ph.pxy.v[c] + chj * dEz; qxy.v[c] = q; vx -= db * q; }
-            if (sk >= 0) {
-              const T q = bhT[ck] * ph.pxz.v[c] + chT[ck] * dEy;
-              stg<T>(psiB[psi_id(F_HX, 2, 0)], iz, q);
-              vx += db * q;
-            }
-          }
-          // update_hy: i (ib,ie), j [jb,je), k (kb,ke); TM source first; psi hy_x (+), hy_z (-)
-          T vy = hyc.v[c];
-          if (src) vy -= jv;
-          if (by) {
-            const T dEz = ez - ezp.v[c];
-            const T dEx = ex - exk;
-            vy = vy + db * (dEz * ikx - dEx * ikz);
-            if (si >= 0) { const T q = bhx * ph.pyx.v[c] + chx * dEz; qyx.v[c] = q; vy += db * q; }
-            if (sk >= 0) {
-              const T q = bhT[ck] * ph.pyz.v[c] + chT[ck] * dEx;
-              stg<T>(psiB[psi_id(F_HY, 2, 0)], iz, q);
-              vy -= db * q;
-            }
-          }
-          // update_hz: i (ib,ie), j (jb,je), k [kb,ke); psi hz_x (-), hz_y (+)
-          T vz = hzc.v[c];
-          if (bz) {
-            const T dEx = ex - exj.v[c];
-            const T dEy = ey - eyp.v[c];
-            vz = vz + db * (dEx * ikyH - dEy * ikx);
-            if (si >= 0) { const T q = bhx * ph.pzx.v[c] + chx * dEy; qzx.v[c] = q; vz -= db * q; }
-            if (sjH >= 0) { const T q = bhj * ph.pzy.v[c] + chj * dEx; qzy.v[c] = q; vz += db * q; }
-          }
-          hxo.v[c] = vx;
-          hyo.v[c] = vy;
-          hzo.v[c] = vz;
-        }
-        const unsigned bo = ((unsigned)i * pl + jk) * es;
-        st4_nt<T>(ExB, bo, exn);
-        st4_nt<T>(EyB, bo, eyn);
-        st4_nt<T>(EzB, bo, ezn);
-        st4_nt<T>(HxB, bo, hxo);
-        st4_nt<T>(HyB, bo, hyo);
-        st4_nt<T>(HzB, bo, hzo);
-        if (si >= 0) {
-          st4_nt<T>(psiB[psi_id(F_HY, 0, 0)], ix, qyx);
-          st4_nt<T>(psiB[psi_id(F_HZ, 0, 0)], ix, qzx);
-        }
-        if (sjH >= 0) {
-          const unsigned iy = (((unsigned)i * N + djh) * nz + kq) * es;
-          st4_nt<T>(psiB[psi_id(F_HX, 1, sjH)], iy, qxy);
-          st4_nt<T>(psiB[psi_id(F_HZ, 1, sjH)], iy, qzy);
-        }
-        // port-plane capture: E at k = port, H at k = port and port + 1
-        if (jcap && i >= wx0 && i <= wx1) {
-          const size_t cc = (size_t)((i - wx0) * prow + (j - wy0));
-          const size_t base = (size_t)tstep * nplanes;
-#pragma unroll
-          for (int c = 0; c < C; ++c) {
-            const int kk = k + c;
-            if (kk == port0 || kk == port1) {
-              const int cap = kk == port0 ? 0 : 1;
-              ring[(base + cap * 6 + 0) * pcells + cc] = exn.v[c];
-              ring[(base + cap * 6 + 1) * pcells + cc] = eyn.v[c];
-              if (port0 == port1) {
-                ring[(base + 6) * pcells + cc] = exn.v[c];
-                ring[(base + 7) * pcells + cc] = eyn.v[c];
-              }
-            }
-            if (kk == port0) { ring[(base + 2) * pcells + cc] = hxo.v[c]; ring[(base + 4) * pcells + cc] = hyo.v[c]; }
-            if (kk == port0 + 1) { ring[(base + 3) * pcells + cc] = hxo.v[c]; ring[(base + 5) * pcells + cc] = hyo.v[c]; }
-            if (kk == port1) { ring[(base + 8) * pcells + cc] = hxo.v[c]; ring[(base + 10) * pcells + cc] = hyo.v[c]; }
-            if (kk == port1 + 1) { ring[(base + 9) * pcells + cc] = hxo.v[c]; ring[(base + 11) * pcells + cc] = hyo.v[c]; }
-          }
-        }
-      }
-      // 5. advance the march
-      eyp = eyn;
-      ezp = ezn;
-      hxc = hxn; hyc = hyn; hzc = hzn;
-      hxn = hxn2; hyn = hyn2; hzn = hzn2;
-      tpx = tpxn; tpz = tpzn;
-      pc = pn;
-    }
-    lds_barrier();   // the next tile reuses the LDS rows
   }
 }
 

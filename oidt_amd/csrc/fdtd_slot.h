@@ -29,9 +29,6 @@ constexpr int kBY = 4;
 constexpr int kThreads = kBX * kBY;
 // Frequencies per DFT work item.
 constexpr int kFB = 16;
-// Largest CPML thickness the fused 3D step stages in LDS (a run with more
-// cells is refused for that form).
-constexpr int kCpmlMax = 64;
 // Distinct cb values per E component (2-bit index per component in the
 // material byte).
 constexpr int kMatMax = 4;

@@ -10,9 +10,8 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
-FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32), (4, 2), (4, 4), (4, 8)],
-              '3d': [(3, 12), (3, 16), (5, 1), (5, 2), (5, 4), (5, 8), (5, 16)],
-              'te': [(3, 16), (3, 32)]}   # TMz also the blocked step (variant 4), 3D the halo march (5)
+FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32), (4, 2), (4, 4), (4, 8)], '3d': [(3, 16)],
+              'te': [(3, 16), (3, 32)]}   # TMz also the temporally blocked step (variant 4)
 
 
 @pytest.mark.gpu
@@ -146,23 +145,19 @@ def test_blocked_step_matches_fused(precision, tb):
     ('float64', 1, 4, 'f', {'sz': 70}),                        # the march form
     ('float64', 1, 8, 'b', {'source_condition': 'TM'}),
     ('float32', 1, 16, 'f', {'p1': 63, 'p2': 127}),
-    ('float64', 3, 4, 'f', {'sz': 70}),                        # the fused step: two waves per row
-    ('float64', 3, 16, 'b', {'source_condition': 'TM'}),
-    ('float32', 3, 8, 'f', {'p1': 63, 'p2': 127}),             # H samples at the z-wave boundary 128
-    ('float32', 3, 12, 'b', {'nz': 300, 'p2': 270}),           # three waves per row, one wave idle
-    ('float64', 3, 2, 'f', {'nz': 300, 'p2': 270}),            # FP64: two rows of three waves, two idle
-    ('float64', 5, 4, 'f', {'sz': 70}),                        # the halo march: 63-cell z runs
-    ('float64', 5, 8, 'b', {'source_condition': 'TM'}),
-    ('float32', 5, 2, 'f', {'p1': 62, 'p2': 125}),             # H samples at the run boundaries 63 and 126
-    ('float32', 5, 16, 'b', {'nz': 300, 'p2': 270}),
+    ('float64', 3, 4, 'f', {'sz': 70}),                        # the fused step: 63-cell z runs
+    ('float64', 3, 8, 'b', {'source_condition': 'TM'}),
+    ('float32', 3, 2, 'f', {'p1': 62, 'p2': 125}),             # H samples at the run boundaries 63 and 126
+    ('float32', 3, 16, 'b', {'nz': 300, 'p2': 270}),
+    ('float64', 3, 1, 'f', {'nz': 300, 'p2': 270}),
 ])
 def test_3d_long_z_runs(precision, variant, xc, direction, extra):
     """A guide long enough in z (nz = 250) for four 64-cell z runs per row:
     runs wholly outside the z-CPML slabs, runs that hold a source slab or a
     port plane (E samples plane p, H samples p and p+1, here on both sides of
     a run boundary), and the partial last run all give the oracle's fields and
-    port DFT in the two-kernel 3D forms and the fused step (whose rows span
-    two or three waves here, 128 z-cells each)."""
+    port DFT in the two-kernel 3D forms and the fused step (whose waves own
+    63 z-cells each and compute the cell below their run)."""
     p = configs.strip3d(250, nx=36, ny=34, nt=200, nf=6, precision=precision, width=12, height=8, port_len=10,
                         buffer_z=6, source_type=1, gauss_pulse_deg=-1.0)
     p.update(extra)
@@ -214,22 +209,23 @@ def test_launch_form_validation():
         t.close()
     d = Batch('3d', np.float32, max_slots=1)
     try:
-        d.set_launch(20, 0, 3)
+        d.set_launch(16, 0, 3)
+        for xc in (12, 20, 32):         # the fused 3D step takes 1, 2, 4, 8 or 16 planes
+            with pytest.raises(OidtError):
+                d.set_launch(xc, 0, 3)
+        assert d.launch()[:2] == (3, 16)
         with pytest.raises(OidtError):
-            d.set_launch(0, 0, 1)       # keeps xchunk 20, which the two-kernel forms do not take
-        d.set_launch(12, 0, 3)
+            d.set_launch(0, 0, 4)       # the blocked step is TMz only
     finally:
         d.close()
-    # the fused 3D step holds whole z rows in LDS: runs longer than its limit
-    # (FP64: 512 cells) are refused before anything is allocated
-    q = Batch('3d', np.float64, max_slots=1)
+    # the blocked TMz step takes 2, 4 or 8 steps per launch
+    q = Batch('tm', np.float64, max_slots=1)
     try:
-        q.set_launch(4, 0, 3)
-        p = configs.strip3d(520, nx=36, ny=34, nt=64, nf=2, precision='float64', width=12, height=8, port_len=10,
-                            buffer_z=6)
-        s, up, lo, fg = device_inputs(p)
-        with pytest.raises(OidtError):
-            q.add(s, fg)
+        q.set_launch(8, 0, 4)
+        for xc in (1, 16):
+            with pytest.raises(OidtError):
+                q.set_launch(xc, 0, 4)
+        q.set_launch(4, 0, 2)
     finally:
         q.close()
 
