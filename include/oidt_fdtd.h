@@ -70,7 +70,7 @@ typedef struct oidt_run_desc {
   int32_t num_cpml;     /* CPML thickness N (user_config num_cpml) */
   int32_t offset;       /* PEC border, 2 in the reference */
   int32_t nt;           /* number of time steps */
-  int32_t nf;           /* number of DFT frequencies (sparam_num_freqs) */
+  int32_t nf;           /* number of DFT frequencies (sparam_num_freqs), at most 65535 */
   int32_t src_field;    /* oidt_field driven by J_SRC: TMz Ez, TEz Hz, 3D Ey (TE) or Hy (TM) */
   int32_t src_box[6];   /* x0,x1,y0,y1,z0,z1: half-open box of update_source (z0=0,z1=1 in 2D) */
   int32_t port[2];      /* port planes: p1x,p2x (2D) or p1z,p2z (3D) */
@@ -285,7 +285,7 @@ int oidt_batch_remove(oidt_batch* b, int32_t slot);
  * issues the next plane's loads before updating the current one; 2 = batched
  * loads per x-chunk, the 2D default; 3 = fused Yee step, one launch per step,
  * all schemes, xchunk also 12, 20 or 24 in 2D, 1, 2, 4, 8 or 16 in 3D; 4 = temporally
- * blocked step, TMz only, xchunk = steps per launch, 2, 4 or 8).  Once a run
+ * blocked step, TMz and TEz, xchunk = steps per launch, 2, 4 or 8).  Once a run
  * has started, its form cannot change between two-kernel, fused and blocked,
  * nor the blocked step's depth (OIDT_ESTATE).
  * No counterpart in the reference (fixed launch shapes, fdtd_main.py:263-273). */
@@ -300,9 +300,12 @@ int oidt_batch_destroy(oidt_batch* b);
 /* No counterpart in the reference, which times the whole loop with time()
  * (fdtd_main.py:271-341).  These run real work and advance the runs. */
 
-/* Enqueue nchunks chunks as individual launches with a HIP event pair around
- * every kernel on the batch stream, wait, and return the mean duration (ms) of
- * out_ms4[0] E half-step, [1] H half-step, [2] DFT fold, [3] chunk advance. */
+/* Enqueue nchunks chunks as individual launches with HIP event pairs on the
+ * batch stream, wait, and return the mean duration (ms) per launch of
+ * out_ms4[0] E half-step (the whole step in the fused and blocked forms),
+ * [1] H half-step, [2] DFT fold, [3] chunk advance.  Two-kernel forms get an
+ * event pair around every kernel; the fused and blocked forms one pair around
+ * each chunk's back-to-back stencil launches. */
 int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4);
 int oidt_solver_profile(oidt_solver* s, int32_t nchunks, double* out_ms4);
 

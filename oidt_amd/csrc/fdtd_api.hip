@@ -136,6 +136,7 @@ int validate(const oidt_run_desc* d) {
   if (d->precision != OIDT_F32 && d->precision != OIDT_F64) return fail(OIDT_EINVAL, "unknown precision");
   const bool is3d = d->scheme == SCHEME_3D;
   if (!is3d && d->nz != 1) return fail(OIDT_EINVAL, "2D runs need nz == 1");
+  if (d->nf < 0 || d->nf > 65535) return fail(OIDT_EINVAL, "nf must be in [0, 65535]");
   if (d->scheme == SCHEME_TMZ_LOSSY && (d->offset != 1 || d->nf != 0))
     return fail(OIDT_EINVAL, "the lossy TMz form updates from offset 1 and has no port DFT (nf = 0)");
   if (d->scheme == SCHEME_TMZ_LOSSY && d->src_field >= 0 &&
@@ -595,11 +596,26 @@ int rebuild_lists(oidt_batch* b) {
       for (int x0 = 0; x0 < d.nx; x0 += xstep)
         for (int y0 = 0; y0 < d.ny; y0 += kBX) tiles.push_back(make_int4(si, x0, y0, 0));
     }
+  }
+  // DFT work items: kFB frequencies each, fewer when the whole batch would
+  // otherwise have too few items to fill the GPU (a small single run: its
+  // fold is a serial chain per item).  Each frequency's sum is formed the
+  // same way whatever the grouping.
+  int fb = kFB;
+  auto items = [&](int f) {
+    size_t n = 0;
+    for (const SlotHost& sh : b->slots)
+      if (sh.used) n += (size_t)sh.nq * ((sh.wcells + kThreads - 1) / kThreads) * ((sh.desc.nf + f - 1) / f);
+    return n;
+  };
+  while (fb > 1 && items(fb) < 1024) fb >>= 1;
+  for (int si = 0; si < (int)b->slots.size(); ++si) {
+    const SlotHost& sh = b->slots[si];
+    if (!sh.used) continue;
     const int cblocks = (sh.wcells + kThreads - 1) / kThreads;
-    const int fblocks = (d.nf + kFB - 1) / kFB;
     for (int q = 0; q < sh.nq; ++q)
       for (int cb = 0; cb < cblocks; ++cb)
-        for (int fb = 0; fb < fblocks; ++fb) work.push_back(make_int4(si, q, cb, fb));
+        for (int f0 = 0; f0 < sh.desc.nf; f0 += fb) work.push_back(make_int4(si, q, cb, f0 | (fb << 16)));
   }
   int rc;
   if ((rc = grow(b, &b->d_tiles, &b->tiles_cap, std::max<size_t>(tiles.size(), 1))) != OIDT_OK) return rc;
@@ -674,15 +690,18 @@ void launch_fused(oidt_batch* b, int gs, int tstep) {
   }
 }
 
-// The temporally blocked TMz step: xchunk steps per launch.  Blocks of 16
+// The temporally blocked 2D step (TMz, TEz): xchunk steps per launch.  Blocks of 16
 // waves x 2 planes in FP64 (half the registers per wave of 8 x 4, and the
 // FP64 workloads are single small runs that need the extra waves per tile to
 // hide latency), 8 x 4 in FP32 (better on large batches).  The shape depends
 // only on the precision, so a run gives the same bits alone and in a batch.
 template <typename T, int TB, int W>
 void launch_blocked_w(oidt_batch* b, int gs, int tstep) {
-  hipLaunchKernelGGL((k_tmz_tb<T, TB, W>), dim3(gs), dim3(64, W), 0, b->stream,
-                     static_cast<const Slot<T>*>(b->d_slots), b->d_tiles, b->d_counts, tstep);
+  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
+  if (b->scheme == SCHEME_TEZ)
+    hipLaunchKernelGGL((k_tez_tb<T, TB, W>), dim3(gs), dim3(64, W), 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+  else
+    hipLaunchKernelGGL((k_tmz_tb<T, TB, W>), dim3(gs), dim3(64, W), 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
 }
 template <typename T, int W>
 void launch_blocked_tb(oidt_batch* b, int gs, int tstep) {
@@ -781,41 +800,55 @@ int enqueue_chunk(oidt_batch* b) {
 template <typename T>
 int profile_chunks(oidt_batch* b, int nchunks, double* out) {
   const int gs = stencil_grid(b), gd = dft_grid(b);
-  // kinds: 0 E half-step, 1 H half-step, 2 DFT fold, 3 advance, 4 / 5 monitors after E / H
-  std::vector<std::pair<int, hipEvent_t>> ev;
-  auto timed = [&](int kind, auto&& launch) -> int {
-    hipEvent_t a, z;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&z));
-    ev.emplace_back(kind, a);
-    ev.emplace_back(kind, z);
-    HIP_TRY(hipEventRecord(a, b->stream));
+  // kinds: 0 E half-step, 1 H half-step, 2 DFT fold, 3 advance, 4 / 5 monitors after E / H.
+  // The one-launch forms (fused, blocked) have a single stencil kernel per
+  // step: the chunk's stencil launches are timed back to back between one
+  // pair of events (per-launch events would add their own gaps to launches
+  // of a few microseconds); the two-kernel forms time each launch.
+  struct Timed { int kind, launches; hipEvent_t a, z; };
+  std::vector<Timed> ev;
+  auto timed = [&](int kind, int launches, auto&& launch) -> int {
+    Timed t{kind, launches, nullptr, nullptr};
+    HIP_TRY(hipEventCreate(&t.a));
+    HIP_TRY(hipEventCreate(&t.z));
+    ev.push_back(t);
+    HIP_TRY(hipEventRecord(t.a, b->stream));
     launch();
-    HIP_TRY(hipEventRecord(z, b->stream));
+    HIP_TRY(hipEventRecord(t.z, b->stream));
     return OIDT_OK;
   };
   int rc;
+  const int per = launch_steps(b);
   for (int c = 0; c < nchunks; ++c) {
-    for (int t = 0; t < kChunk; t += launch_steps(b))
-      for (int h = 0; h < 2; ++h) {
-        if (!(b->variant >= 3 && h == 1) &&
-            (rc = timed(h, [&] { launch_stencil<T>(b, gs, t, h == 0); })) != OIDT_OK) return rc;
-        if (b->nmwork[h] > 0 && (rc = timed(4 + h, [&] { launch_monitors<T>(b, t, h); })) != OIDT_OK) return rc;
-      }
-    if ((rc = timed(2, [&] { launch_dft<T>(b, gd); })) != OIDT_OK) return rc;
-    if ((rc = timed(3, [&] { launch_advance<T>(b); })) != OIDT_OK) return rc;
+    if (b->variant >= 3) {
+      if ((rc = timed(0, kChunk / per, [&] {
+             for (int t = 0; t < kChunk; t += per) launch_stencil<T>(b, gs, t, true);
+           })) != OIDT_OK)
+        return rc;
+    } else {
+      for (int t = 0; t < kChunk; ++t)
+        for (int h = 0; h < 2; ++h) {
+          if ((rc = timed(h, 1, [&] { launch_stencil<T>(b, gs, t, h == 0); })) != OIDT_OK) return rc;
+          if (b->nmwork[h] > 0 && (rc = timed(4 + h, 1, [&] { launch_monitors<T>(b, t, h); })) != OIDT_OK) return rc;
+        }
+    }
+    if ((rc = timed(2, 1, [&] { launch_dft<T>(b, gd); })) != OIDT_OK) return rc;
+    if ((rc = timed(3, 1, [&] { launch_advance<T>(b); })) != OIDT_OK) return rc;
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(b->stream));
   double sum[6] = {0, 0, 0, 0, 0, 0};
   int cnt[6] = {0, 0, 0, 0, 0, 0};
-  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+  for (const Timed& t : ev) {
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev[i].second, ev[i + 1].second));
-    sum[ev[i].first] += ms;
-    cnt[ev[i].first] += 1;
+    HIP_TRY(hipEventElapsedTime(&ms, t.a, t.z));
+    sum[t.kind] += ms;
+    cnt[t.kind] += t.launches;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e.second);
+  for (auto& t : ev) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.z);
+  }
   for (int i = 0; i < 6; ++i) out[i] = cnt[i] ? sum[i] / cnt[i] : 0.0;
   b->chunks_enqueued += nchunks;
   b->chunks_done = b->chunks_enqueued;
@@ -1002,7 +1035,8 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (variant == 0) variant = b->variant;
   const int xc = xchunk ? xchunk : b->xchunk;   // the depth this launch form will use
   if (variant == 4) {
-    if (b->scheme != SCHEME_TMZ) return fail(OIDT_EINVAL, "the blocked step (variant 4) is for TMz runs");
+    if (b->scheme != SCHEME_TMZ && b->scheme != SCHEME_TEZ)
+      return fail(OIDT_EINVAL, "the blocked step (variant 4) is for TMz and TEz runs");
     if (xc != 2 && xc != 4 && xc != 8) return fail(OIDT_EINVAL, "the blocked step takes 2, 4 or 8 steps per launch");
   }
   if (variant == 3 && b->scheme == SCHEME_3D && xc != 1 && xc != 2 && xc != 4 && xc != 8 && xc != 16)

@@ -2351,6 +2351,223 @@ __global__ __launch_bounds__(64 * W) void k_tmz_tb(const Slot<T>* __restrict__ s
   }
 }
 
+// ------------------------------------------------ temporally blocked TEz ---
+// The TEz counterpart of k_tmz_tb (same tiles, ghost frame, block shapes and
+// field-set rule): per step the E update of a wave's last plane takes the old
+// Hz of the next wave's first plane, and the Hz update of its first plane the
+// new Ey of the previous wave's last plane, through LDS; the j+1 neighbour of
+// Ex's update and the j-1 neighbour of Hz's come from DPP lane shifts.  The
+// per-cell arithmetic is k_tez_step's (tez_e, then the Hz update in its order).
+
+template <typename T, int TB, int W>
+__global__ __launch_bounds__(64 * W) void k_tez_tb(const Slot<T>* __restrict__ slots,
+                                                   const int4* __restrict__ tiles,
+                                                   const int* __restrict__ counts, int tstep) {
+  constexpr int K = kTbPlanes / W;
+  constexpr int TX = kTbPlanes - 2 * TB, TY = 64 - 2 * TB;
+  static_assert(TX > 0 && TY > 0 && kChunk % TB == 0, "blocking depth");
+  constexpr unsigned es = sizeof(T);
+  __shared__ T xhz[W][64];   // old Hz of each wave's first plane
+  __shared__ T xey[W][64];   // new Ey of each wave's last plane
+  const int ntiles = counts[0];
+  const int lane = threadIdx.x;
+  const int w = wave_id();
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>& s = slots[tl.x];
+    const int n0 = s.base + tstep;
+    if (!s.active || n0 >= s.nt) continue;      // uniform over the block
+    const int nsteps = min(TB, s.nt - n0);
+    const TezView<T> v = tez_view(s);
+    const bool odd = ((n0 / TB) & 1) != 0;
+    const T* __restrict__ ExA = odd ? s.f2[F_EX] : s.f[F_EX];
+    const T* __restrict__ EyA = odd ? s.f2[F_EY] : s.f[F_EY];
+    const T* __restrict__ HzA = odd ? s.f2[F_HZ] : s.f[F_HZ];
+    T* __restrict__ ExB = odd ? s.f[F_EX] : s.f2[F_EX];
+    T* __restrict__ EyB = odd ? s.f[F_EY] : s.f2[F_EY];
+    T* __restrict__ HzB = odd ? s.f[F_HZ] : s.f2[F_HZ];
+    OIDT_GLOBAL T* const* psiAll = odd ? s.psi2 : s.psi;
+    OIDT_GLOBAL T* const* psiBll = odd ? s.psi : s.psi2;
+    const int nx = v.nx, ny = v.ny, N = v.N;
+    const int i0 = tl.y, j0 = tl.z;               // first inner cell
+    const int ib0 = i0 - TB + w * K;              // this wave's first plane
+    const int j = j0 - TB + lane;
+    const bool jok = j >= 0 && j < ny;
+    const int jc = min(max(j, 0), ny - 1);
+    const bool jown = jok && j >= j0 && j < j0 + TY;
+    // row (y-only) constants of this lane
+    const bool jin = jok && j >= v.jb && j < v.je;     // Ex rows
+    const bool jin1 = jok && j > v.jb && j < v.je;     // Ey and Hz rows
+    const bool jsrc = jok && v.src_hz && j >= v.sy0 && j < v.sy1;
+    int dye = 0, cye = 0, dyh = 0, cyh = 0;
+    const int sye = jin ? slab_e(j, v.jb, v.je, N, dye, cye) : -1;
+    const int syh = jin1 ? slab_h(j, v.jb, v.je, N, dyh, cyh) : -1;
+    const T ikye = jin ? v.ike1[j] : T(0);
+    const T bey = sye >= 0 ? v.be[cye] : T(0), cey = sye >= 0 ? v.ceo[cye] : T(0);
+    const T ikyh = jin1 ? v.ikh1[j] : T(0);
+    const T bhy = syh >= 0 ? v.bh[cyh] : T(0), chy = syh >= 0 ? v.cho[cyh] : T(0);
+    const bool jcap = jown && j >= v.wy0 && j <= v.wy1;
+    const T* pxyA = sye >= 0 ? psiAll[psi_id(F_EX, 1, sye)] : nullptr;
+    T* pxyB = sye >= 0 ? psiBll[psi_id(F_EX, 1, sye)] : nullptr;
+    const T* pzyA = syh >= 0 ? psiAll[psi_id(F_HZ, 1, syh)] : nullptr;
+    T* pzyB = syh >= 0 ? psiBll[psi_id(F_HZ, 1, syh)] : nullptr;
+    // ---- per-plane flags (bit 0 iin, 1 iin1, 2 source plane, 3 E x-slab,
+    // 4 H x-slab, 5..8 capture at port0, port1, port0 + 1, port1 + 1); lane q
+    // holds plane ib0 + q's coefficients.  The x-slab cell indexes are
+    // recomputed where the psi values are loaded and stored.
+    unsigned pf[K];
+    T lk_ikx = T(0), lk_bex = T(0), lk_cex = T(0), lk_ikh = T(0), lk_bhx = T(0), lk_chx = T(0);
+    auto xslabs = [&](int i, int& sxe, int& dxe, int& cxe, int& sxh, int& dxh, int& cxh) {
+      const bool iok = i >= 0 && i < nx;
+      const bool iin = iok && i >= v.ib && i < v.ie, iin1 = iok && i > v.ib && i < v.ie;
+      dxe = cxe = dxh = cxh = 0;
+      sxe = iin ? slab_e(i, v.ib, v.ie, N, dxe, cxe) : -1;
+      sxh = iin1 ? slab_h(i, v.ib, v.ie, N, dxh, cxh) : -1;
+    };
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = ib0 + q;
+      const bool iok = i >= 0 && i < nx;
+      const bool iin = iok && i >= v.ib && i < v.ie, iin1 = iok && i > v.ib && i < v.ie;
+      const bool inner = iok && i >= i0 && i < i0 + TX;
+      int sxe, dxe, cxe, sxh, dxh, cxh;
+      xslabs(i, sxe, dxe, cxe, sxh, dxh, cxh);
+      pf[q] = (iin ? 1u : 0u) | (iin1 ? 2u : 0u) | (iok && i >= v.sx0 && i < v.sx1 ? 4u : 0u) | (sxe >= 0 ? 8u : 0u) |
+              (sxh >= 0 ? 16u : 0u) | (inner && i == v.port0 ? 32u : 0u) | (inner && i == v.port1 ? 64u : 0u) |
+              (inner && i == v.port0 + 1 ? 128u : 0u) | (inner && i == v.port1 + 1 ? 256u : 0u);
+      if (lane == q) {
+        lk_ikx = iin ? v.ike0[i] : T(0);
+        lk_bex = sxe >= 0 ? v.be[cxe] : T(0);
+        lk_cex = sxe >= 0 ? v.ceo[cxe] : T(0);
+        lk_ikh = iin1 ? v.ikh0[i] : T(0);
+        lk_bhx = sxh >= 0 ? v.bh[cxh] : T(0);
+        lk_chx = sxh >= 0 ? v.cho[cxh] : T(0);
+      }
+    }
+    // ---- the wave's cells, from set A (outside the grid: zero, never updated)
+    T ex[K], ey[K], hz[K], pxy[K], pyx[K], qzx[K], qzy[K], cb0[K], cb1[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = ib0 + q;
+      const bool ok = jok && i >= 0 && i < nx;
+      const unsigned idx = (unsigned)min(max(i, 0), nx - 1) * ny + jc;
+      ex[q] = ok ? ldg<T>(ExA, idx * es) : T(0);
+      ey[q] = ok ? ldg<T>(EyA, idx * es) : T(0);
+      hz[q] = ok ? ldg<T>(HzA, idx * es) : T(0);
+      const unsigned m = ok ? (unsigned)ldg<uint8_t>(v.mat, idx) : 0u;
+      const int q0 = m & 3, q1 = (m >> 2) & 3;
+      cb0[q] = q0 == 0 ? v.cb0[0] : (q0 == 1 ? v.cb0[1] : (q0 == 2 ? v.cb0[2] : v.cb0[3]));
+      cb1[q] = q1 == 0 ? v.cb1[0] : (q1 == 1 ? v.cb1[1] : (q1 == 2 ? v.cb1[2] : v.cb1[3]));
+      const unsigned f = pf[q];
+      int sxe, dxe, cxe, sxh, dxh, cxh;
+      xslabs(i, sxe, dxe, cxe, sxh, dxh, cxh);
+      pxy[q] = ok && (f & 2u) && sye >= 0 ? pxyA[(size_t)i * N + dye] : T(0);
+      pyx[q] = ok && jin1 && sxe >= 0 ? psiAll[psi_id(F_EY, 0, sxe)][(size_t)dxe * ny + jc] : T(0);
+      qzx[q] = ok && jin1 && sxh >= 0 ? psiAll[psi_id(F_HZ, 0, sxh)][(size_t)dxh * ny + jc] : T(0);
+      qzy[q] = ok && (f & 2u) && syh >= 0 ? pzyA[(size_t)i * N + dyh] : T(0);
+    }
+    const T lk_jv = v.src_hz && lane < nsteps ? s.jsrc[n0 + lane] : T(0);
+    const T db = v.db;
+    T* const ring = v.ring;
+    const int pcells = v.pcells;
+    const size_t cw = (size_t)(j - v.wy0);
+    for (int t = 0; t < nsteps; ++t) {
+      const T jv = lane_get(lk_jv, t);
+      const size_t rbase = (size_t)(tstep + t) * v.nplanes;
+      // ---- E half-step (tez_e per cell): Hz(i+1) of the last plane from the next wave
+      xhz[w][lane] = hz[0];
+      lds_barrier();
+      const T hzup = w + 1 < W ? xhz[w + 1][lane] : hz[K - 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const unsigned f = pf[q];
+        const T hzj = lane_down1(hz[q], hz[q]);
+        const T hzi = q + 1 < K ? hz[q + 1] : hzup;
+        if ((f & 2u) && jin) {       // Ex: i (ib, ie), j [jb, je)
+          const T dHz = hzj - hz[q];
+          T e = ex[q] + cb0[q] * dHz * ikye;
+          if (sye >= 0) {
+            const T qv = bey * pxy[q] + cey * dHz;
+            pxy[q] = qv;
+            e += cb0[q] * qv;
+          }
+          ex[q] = e;
+        }
+        if ((f & 1u) && jin1) {      // Ey: i [ib, ie), j (jb, je)
+          const T dHz = hzi - hz[q];
+          T e = ey[q] + cb1[q] * (-dHz) * lane_get(lk_ikx, q);
+          if (f & 8u) {
+            const T qv = lane_get(lk_bex, q) * pyx[q] + lane_get(lk_cex, q) * dHz;
+            pyx[q] = qv;
+            e -= cb1[q] * qv;
+          }
+          ey[q] = e;
+        }
+        if (jcap && (f & 96u)) {
+          if (f & 32u) ring[(rbase + 0) * pcells + cw] = ey[q];
+          if (f & 64u) ring[(rbase + 3) * pcells + cw] = ey[q];
+        }
+      }
+      // ---- H half-step: Ey(i-1) of the first plane from the previous wave
+      xey[w][lane] = ey[K - 1];
+      lds_barrier();
+      const T eydn = w > 0 ? xey[w - 1][lane] : ey[0];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const unsigned f = pf[q];
+        const T exj = lane_up1(ex[q], ex[q]);
+        const T prevEy = q > 0 ? ey[q - 1] : eydn;
+        const bool src = jsrc && (f & 4u);
+        T hv = hz[q];
+        if ((f & 2u) && jin1) {
+          const T dEx = ex[q] - exj;
+          const T dEy = ey[q] - prevEy;
+          hv = hv + db * (dEx * ikyh - dEy * lane_get(lk_ikh, q));
+          if (src) hv -= jv;
+          if (f & 16u) {
+            const T qv = lane_get(lk_bhx, q) * qzx[q] + lane_get(lk_chx, q) * dEy;
+            qzx[q] = qv;
+            hv -= db * qv;
+          }
+          if (syh >= 0) {
+            const T qv = bhy * qzy[q] + chy * dEx;
+            qzy[q] = qv;
+            hv += db * qv;
+          }
+        } else if (src) {
+          hv -= jv;
+        }
+        hz[q] = hv;
+        if (jcap && (f & 480u)) {
+          if (f & 32u) ring[(rbase + 1) * pcells + cw] = hv;
+          if (f & 128u) ring[(rbase + 2) * pcells + cw] = hv;
+          if (f & 64u) ring[(rbase + 4) * pcells + cw] = hv;
+          if (f & 256u) ring[(rbase + 5) * pcells + cw] = hv;
+        }
+      }
+    }
+    // ---- the inner cells to set B (fields unconditionally: cells no step
+    // updates hold the same value in both sets)
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const int i = ib0 + q;
+      if (!jown || i < i0 || i >= i0 + TX || i >= nx) continue;
+      const unsigned f = pf[q];
+      int sxe, dxe, cxe, sxh, dxh, cxh;
+      xslabs(i, sxe, dxe, cxe, sxh, dxh, cxh);
+      const unsigned idx = (unsigned)i * ny + j;
+      stg_nt<T>(ExB, idx * es, ex[q]);
+      stg_nt<T>(EyB, idx * es, ey[q]);
+      stg_nt<T>(HzB, idx * es, hz[q]);
+      if ((f & 2u) && jin && sye >= 0) pxyB[(size_t)i * N + dye] = pxy[q];
+      if ((f & 1u) && jin1 && sxe >= 0) psiBll[psi_id(F_EY, 0, sxe)][(size_t)dxe * ny + j] = pyx[q];
+      if ((f & 2u) && jin1 && sxh >= 0) psiBll[psi_id(F_HZ, 0, sxh)][(size_t)dxh * ny + j] = qzx[q];
+      if ((f & 2u) && jin1 && syh >= 0) pzyB[(size_t)i * N + dyh] = qzy[q];
+    }
+    lds_barrier();   // the next tile reuses the exchange planes
+  }
+}
+
 // ------------------------------------------------------- fused 3D step ---
 // One launch per 3D time step (form 3 in 3D), with no LDS and no barriers.  Each wavefront
 // owns one row j and 63 z-cells of XM planes: lane l holds z-cell k0 - 1 + l,
@@ -2713,7 +2930,8 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
 }
 
 // ------------------------------------------------------------------- DFT ---
-// Work item (slot, q, cell block, frequency block).  Folds the kChunk samples
+// Work item (slot, q, cell block, first frequency | frequencies << 16, at most
+// kFB).  Folds the kChunk samples
 // of the chunk in flight into acc[q][f][cell] (re += s cos, im -= s sin, as
 // simul_fft_* of fdtd_funcs.py:128-144 and FDTD_3D :270-292) in FP64.  The
 // sample is formed from the raw planes in the run precision with the
@@ -2763,9 +2981,9 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
     const int q = wk.y;
     const bool hq = (s.scheme == SCHEME_3D) ? ((q & 3) >= 2) : ((q & 1) == 1);
     const double phi = hq ? 0.5 : 0.0;
-    const int f0 = wk.w * kFB;
+    const int f0 = wk.w & 0xffff, fbn = wk.w >> 16;
     __syncthreads();
-    for (int e = threadIdx.x; e < kFB * kChunk; e += kThreads) {
+    for (int e = threadIdx.x; e < fbn * kChunk; e += kThreads) {
       const int fl = e / kChunk, t = e - fl * kChunk;
       const int f = f0 + fl;
       double c = 0.0, sv = 0.0;
@@ -2783,7 +3001,7 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
 #pragma unroll
     for (int t = 0; t < kChunk; ++t)
       smp[t] = (t < tv) ? sample_at<T>(s, s.ring + (size_t)t * s.nplanes * s.pcells, q, c) : T(0);
-    const int fend = min(kFB, s.nf - f0);
+    const int fend = min(fbn, s.nf - f0);
     for (int fl = 0; fl < fend; ++fl) {
       double re = 0.0, im = 0.0;
 #pragma unroll

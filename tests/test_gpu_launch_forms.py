@@ -11,7 +11,7 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 FORMS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
 FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32), (4, 2), (4, 4), (4, 8)], '3d': [(3, 16)],
-              'te': [(3, 16), (3, 32)]}   # TMz also the temporally blocked step (variant 4)
+              'te': [(3, 16), (3, 32), (4, 2), (4, 4), (4, 8)]}   # 2D also the temporally blocked step (variant 4)
 
 
 @pytest.mark.gpu
@@ -74,18 +74,20 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('tb,direction,extra', [(2, 'b', {'nt': 101}), (4, 'f', {'nt': 101}), (8, 'f', {}),
-                                                (8, 'b', {'nt': 99})])
-def test_blocked_step_fp64(tb, direction, extra):
-    """The temporally blocked TMz step (tb steps per launch) in FP64 equals
+@pytest.mark.parametrize('scheme,tb,direction,extra', [('tm', 2, 'b', {'nt': 101}), ('tm', 4, 'f', {'nt': 101}),
+                                                       ('tm', 8, 'f', {}), ('tm', 8, 'b', {'nt': 99}),
+                                                       ('te', 2, 'f', {'nt': 101}), ('te', 4, 'b', {}),
+                                                       ('te', 8, 'b', {'nt': 99})])
+def test_blocked_step_fp64(scheme, tb, direction, extra):
+    """The temporally blocked 2D step (tb steps per launch) in FP64 equals
     the oracle to 1e-10, also when the step count is not a multiple of tb (the
     last launch takes fewer steps and the state ends in the other field set)."""
-    p = configs.small('tm', 'float64', **extra)
+    p = configs.small(scheme, 'float64', **extra)
     p.update(rough_std=15e-9, rough_acl=40e-9, seed=5)
     s, up, lo, fg = device_inputs(p)
     o, _ = oracle_run(p, up, lo, direction)
-    print('case blocked tm fp64 tb', tb, direction, extra, flush=True)
-    b = Batch('tm', np.float64, max_slots=1)
+    print('case blocked', scheme, 'fp64 tb', tb, direction, extra, flush=True)
+    b = Batch(scheme, np.float64, max_slots=1)
     b.set_launch(tb, 0, 4)
     slot = b.add(s, fg, direction)
     b.advance(-(-s.nt // 32))
@@ -94,7 +96,7 @@ def test_blocked_step_fp64(tb, direction, extra):
     for name, ref in o.fields().items():
         err = rel_l2(b.field(slot, name), ref)
         assert err < 1e-10, (name, err)
-    w = window('tm', o.c)
+    w = window(scheme, o.c)
     od = o.dft()
     for key, z in b.dft(slot).items():
         err = rel_l2(z[w], od[key][w])
@@ -103,9 +105,10 @@ def test_blocked_step_fp64(tb, direction, extra):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('scheme', ['tm', 'te'])
 @pytest.mark.parametrize('precision', ['float32', 'float64'])
 @pytest.mark.parametrize('tb', [2, 4, 8])
-def test_blocked_step_matches_fused(precision, tb):
+def test_blocked_step_matches_fused(scheme, precision, tb):
     """The blocked step applies the fused step's per-cell arithmetic: on an
     uneven pair of runs in one batch (tile edges fall at different cells) its
     fields and port DFT equal the one-step fused form's to rounding (the two
@@ -114,19 +117,20 @@ def test_blocked_step_matches_fused(precision, tb):
     tol = 1e-12 if precision == 'float64' else 2e-6
     runs = []
     for nt, nx, seed in ((101, 140, 5), (96, 181, 6)):
-        p = configs.small('tm', precision, nt=nt, nx=nx)
+        p = configs.small(scheme, precision, nt=nt, nx=nx)
         p.update(rough_std=15e-9, rough_acl=40e-9, seed=seed)
         s, up, lo, fg = device_inputs(p)
         runs.append((s, fg))
     out = {}
     for form in ((3, 2), (4, tb)):
-        print('case blocked vs fused', precision, 'form', form, flush=True)
-        b = Batch('tm', np.float32 if precision == 'float32' else np.float64, max_slots=2)
+        print('case blocked vs fused', scheme, precision, 'form', form, flush=True)
+        b = Batch(scheme, np.float32 if precision == 'float32' else np.float64, max_slots=2)
         b.set_launch(form[1], 0, form[0])
         slots = [b.add(s, fg) for s, fg in runs]
         b.advance(-(-max(s.nt for s, _ in runs) // 32))
         b.sync()
-        out[form] = [({n: b.field(sl, n) for n in ('ez', 'hx', 'hy')}, b.dft(sl)) for sl in slots]
+        names = ('ez', 'hx', 'hy') if scheme == 'tm' else ('ex', 'ey', 'hz')
+        out[form] = [({n: b.field(sl, n) for n in names}, b.dft(sl)) for sl in slots]
         b.close()
     for (fa, da), (fb, dbb) in zip(out[(3, 2)], out[(4, tb)]):
         errs = {n: rel_l2(fb[n], fa[n]) for n in fa}
@@ -215,7 +219,7 @@ def test_launch_form_validation():
                 d.set_launch(xc, 0, 3)
         assert d.launch()[:2] == (3, 16)
         with pytest.raises(OidtError):
-            d.set_launch(0, 0, 4)       # the blocked step is TMz only
+            d.set_launch(0, 0, 4)       # the blocked step is 2D only
     finally:
         d.close()
     # the blocked TMz step takes 2, 4 or 8 steps per launch
