@@ -354,6 +354,20 @@ def bench_serving(args, count=None):
     pdone = srv2.run(preqs)
     pw = time.perf_counter() - t0
     pois = serve.summarize(pdone, pw)
+    # each scheme's requests alone, saturated under the same budget: how much
+    # continuous batching gains per scheme, against that scheme's share of
+    # the plain leg's time
+    by_scheme_served = {}
+    for scheme in sorted(by_scheme):
+        idx = [k for k, r in enumerate(reqs) if r.setup.scheme == scheme]
+        sreqs = serve.make_requests([descs[k] for k in idx], [0.0] * len(idx))
+        for a, k in zip(sreqs, idx):
+            a.setup, a.fg, a.footprint = reqs[k].setup, reqs[k].fg, reqs[k].footprint
+        log('c4: saturated, %s requests only' % scheme)
+        srv3 = serve.Server(be, hbm_budget=budget, chunk_group=4, prepare=lambda r, b: None)
+        t0 = time.perf_counter()
+        srv3.run(sreqs)
+        by_scheme_served[scheme] = time.perf_counter() - t0
     be.close()
     # dominant kernel: per scheme, a batch of (up to) 16 of its requests
     roofs = {}
@@ -384,11 +398,13 @@ def bench_serving(args, count=None):
         'plain_mcell_updates_per_s': round(plain_rate, 1), 'plain_requests_per_s': round(plain_rps, 3),
         'plain_wall_s': round(plain_wall, 3),
         'served_over_plain': round(sat['mcell_updates_per_s'] / plain_rate, 3),
-        # a 3D request of this mix already fills the GPU alone, so batching
-        # gains little on it: the ratio if the 2D requests cost nothing and
-        # the 3D ones ran as in the plain leg bounds what serving can reach
+        # per scheme: the plain leg's time, the same requests served alone
+        # (saturated), and the ratio; the mix's ratio is bounded by the
+        # schemes' ratios weighted by their plain time
         'plain_seconds_by_scheme': {k: round(v, 3) for k, v in by_scheme.items()},
-        'served_over_plain_bound_3d_as_plain': round(plain_wall / by_scheme.get('3d', plain_wall), 3),
+        'served_seconds_by_scheme': {k: round(v, 3) for k, v in by_scheme_served.items()},
+        'served_over_plain_by_scheme': {k: round(by_scheme[k] / v, 3) for k, v in by_scheme_served.items()},
+        'served_over_plain_if_schemes_served_apart': round(plain_wall / sum(by_scheme_served.values()), 3),
         'saturated': dict(rnd(sat), admissions_deferred_for_memory=sat_deferred,
                           peak_resident_runs=srv.peak_resident, peak_resident_bytes=srv.peak_bytes),
         'poisson': dict(rnd(pois), rate_per_s=round(rate, 3),

@@ -65,6 +65,17 @@ template <typename T>
 __device__ __forceinline__ T ldg(const T* base, unsigned boff) {
   return *(const OIDT_GLOBAL T*)(reinterpret_cast<const char*>(base) + boff);
 }
+// Load that the caches may drop at once (non-temporal): for values a kernel
+// reads exactly once when the batch's working set is far larger than the
+// last-level cache, so they do not evict the neighbour values other waves
+// re-read.
+template <bool NT, typename T>
+__device__ __forceinline__ T ldx(const T* base, unsigned boff) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load((const OIDT_GLOBAL T*)(reinterpret_cast<const char*>(base) + boff));
+  else
+    return ldg<T>(base, boff);
+}
 template <typename T>
 __device__ __forceinline__ void stg(T* base, unsigned boff, T v) {
   *(OIDT_GLOBAL T*)(reinterpret_cast<char*>(base) + boff) = v;
@@ -380,7 +391,9 @@ __global__ __launch_bounds__(kThreads) void k_tez_h(const Slot<T>* __restrict__ 
 // coefficients are loaded once per tile, one plane per lane, and broadcast
 // with readlane; the cb values are copied to registers first and the source
 // and capture planes become bit masks over the tile (stores through the field
-// pointers could otherwise alias the descriptor and force reloads).
+// pointers could otherwise alias the descriptor and force reloads).  With NT
+// the values read only once (the updated component's own old value and its
+// psi) are loaded non-temporally.
 
 // Bits u of the tile's planes i0 + u (u < 64) that lie in [lo, hi).
 __device__ __forceinline__ unsigned long long plane_bits(int i0, int lo, int hi) {
@@ -403,7 +416,7 @@ struct E3Plane {
   unsigned m;   // material byte
 };
 
-template <typename T, int XC>
+template <typename T, int XC, bool NT>
 __global__ __launch_bounds__(kThreads) void k_3d_e(const Slot<T>* __restrict__ slots,
                                                    const int4* __restrict__ tiles,
                                                    const int* __restrict__ counts, int tstep) {
@@ -476,9 +489,9 @@ __global__ __launch_bounds__(kThreads) void k_3d_e(const Slot<T>* __restrict__ s
       const unsigned idx = (unsigned)i * pl + jk;
       const unsigned bo = idx * es;
       const unsigned bo1 = (i + 1 < nx ? idx + pl : idx) * es;
-      p.ex = ldg<T>(Ex, bo);
-      p.ey = ldg<T>(Ey, bo);
-      p.ez = ldg<T>(Ez, bo);
+      p.ex = ldx<NT, T>(Ex, bo);
+      p.ey = ldx<NT, T>(Ey, bo);
+      p.ez = ldx<NT, T>(Ez, bo);
       p.hx = ldg<T>(Hx, bo);
       p.hxk = ldg<T>(Hx, bo + offk);
       p.hxj = ldg<T>(Hx, bo + offj);
@@ -488,15 +501,15 @@ __global__ __launch_bounds__(kThreads) void k_3d_e(const Slot<T>* __restrict__ s
       p.hz1 = ldg<T>(Hz, bo1);
       p.m = ldg<uint8_t>(mat, idx);
       const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
-      p.qxy = pExy ? ldg<T>(pExy, iy) : T(0);
-      p.qzy = pEzy ? ldg<T>(pEzy, iy) : T(0);
-      p.qxz = sk >= 0 ? ldg<T>(pExz, iz + zside) : T(0);
-      p.qyz = sk >= 0 ? ldg<T>(pEyz, iz + zside) : T(0);
+      p.qxy = pExy ? ldx<NT, T>(pExy, iy) : T(0);
+      p.qzy = pEzy ? ldx<NT, T>(pEzy, iy) : T(0);
+      p.qxz = sk >= 0 ? ldx<NT, T>(pExz, iz + zside) : T(0);
+      p.qyz = sk >= 0 ? ldx<NT, T>(pEyz, iz + zside) : T(0);
       int di = 0, ci = 0;
       const int si = slab_e(i, ib, ie, N, di, ci);
       const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
-      p.qyx = si >= 0 ? ldg<T>(pEyx, ix) : T(0);
-      p.qzx = si >= 0 ? ldg<T>(pEzx, ix) : T(0);
+      p.qyx = si >= 0 ? ldx<NT, T>(pEyx, ix) : T(0);
+      p.qzx = si >= 0 ? ldx<NT, T>(pEzx, ix) : T(0);
       return p;
     };
     const unsigned b0 = ((unsigned)i0 * pl + jk) * es;
@@ -598,7 +611,7 @@ struct H3Plane {
   T hx, hy, hz, ex, exk, exj, eyk, ezj, ey, ez, qxy, qzy, qxz, qyz, qyx, qzx;
 };
 
-template <typename T, int XC>
+template <typename T, int XC, bool NT>
 __global__ __launch_bounds__(kThreads) void k_3d_h(const Slot<T>* __restrict__ slots,
                                                    const int4* __restrict__ tiles,
                                                    const int* __restrict__ counts, int tstep) {
@@ -674,9 +687,9 @@ __global__ __launch_bounds__(kThreads) void k_3d_h(const Slot<T>* __restrict__ s
       H3Plane<T> p;
       const unsigned idx = (unsigned)i * pl + jk;
       const unsigned bo = idx * es;
-      p.hx = ldg<T>(Hx, bo);
-      p.hy = ldg<T>(Hy, bo);
-      p.hz = ldg<T>(Hz, bo);
+      p.hx = ldx<NT, T>(Hx, bo);
+      p.hy = ldx<NT, T>(Hy, bo);
+      p.hz = ldx<NT, T>(Hz, bo);
       p.ex = ldg<T>(Ex, bo);
       p.exk = ldg<T>(Ex, bo - offk);
       p.exj = ldg<T>(Ex, bo - offj);
@@ -685,15 +698,15 @@ __global__ __launch_bounds__(kThreads) void k_3d_h(const Slot<T>* __restrict__ s
       p.ey = ldg<T>(Ey, bo);
       p.ez = ldg<T>(Ez, bo);
       const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es, iz = (((unsigned)i * ny + j) * N + dk) * es;
-      p.qxy = pHxy ? ldg<T>(pHxy, iy) : T(0);
-      p.qzy = pHzy ? ldg<T>(pHzy, iy) : T(0);
-      p.qxz = sk >= 0 ? ldg<T>(pHxz, iz + zside) : T(0);
-      p.qyz = sk >= 0 ? ldg<T>(pHyz, iz + zside) : T(0);
+      p.qxy = pHxy ? ldx<NT, T>(pHxy, iy) : T(0);
+      p.qzy = pHzy ? ldx<NT, T>(pHzy, iy) : T(0);
+      p.qxz = sk >= 0 ? ldx<NT, T>(pHxz, iz + zside) : T(0);
+      p.qyz = sk >= 0 ? ldx<NT, T>(pHyz, iz + zside) : T(0);
       int di = 0, ci = 0;
       const int si = slab_h(i, ib, ie, N, di, ci);
       const unsigned ix = (((unsigned)di * ny + j) * nz + kc) * es + (si == 1 ? xhi : 0u);
-      p.qyx = si >= 0 ? ldg<T>(pHyx, ix) : T(0);
-      p.qzx = si >= 0 ? ldg<T>(pHzx, ix) : T(0);
+      p.qyx = si >= 0 ? ldx<NT, T>(pHyx, ix) : T(0);
+      p.qzx = si >= 0 ? ldx<NT, T>(pHzx, ix) : T(0);
       return p;
     };
     const unsigned bm = ((unsigned)max(i0 - 1, 0) * pl + jk) * es;
