@@ -28,7 +28,7 @@ constexpr int kBX = 64;
 constexpr int kBY = 4;
 constexpr int kThreads = kBX * kBY;
 // Frequencies per DFT work item.
-constexpr int kFB = 16;
+constexpr int kFB = 32;
 // Distinct cb values per E component (2-bit index per component in the
 // material byte).
 constexpr int kMatMax = 4;

@@ -207,7 +207,7 @@ def batch_vs_alone(descs, nt, launch=None):
     slots = [b.add(s, fg) for s, fg in runs]
     b.advance(-(-nt // 32))
     b.sync()
-    names = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz') if s0.scheme == '3d' else ('ez', 'hx', 'hy')
+    names = {'3d': ('ex', 'ey', 'ez', 'hx', 'hy', 'hz'), 'tm': ('ez', 'hx', 'hy'), 'te': ('ex', 'ey', 'hz')}[s0.scheme]
     try:
         for r, ((s, fg), slot) in enumerate(zip(runs, slots)):
             a = Batch(s.scheme, s.precision, max_slots=1)
@@ -231,6 +231,13 @@ def batch_vs_alone(descs, nt, launch=None):
 def test_config2_batch_members_identical_alone(launch):
     print('case config 2, 64 runs, launch', launch, flush=True)
     batch_vs_alone(configs.config2(), 160, launch)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('launch', [(4, 8, 0), (3, 16, 0)])
+def test_config2_tez_batch_members_identical_alone(launch):
+    print('case config 2 in TEz, 64 runs, launch', launch, flush=True)
+    batch_vs_alone(configs.config2(scheme='te'), 160, launch)
 
 
 @pytest.mark.gpu
