@@ -584,10 +584,23 @@ int rebuild_lists(oidt_batch* b) {
         for (int y0 = 0; y0 < d.ny; ++y0)
           for (int z0 = 0; z0 < d.nz; z0 += kHaloZ) tiles.push_back(make_int4(si, x0, y0, z0));
     } else if (b->variant == 4) {
-      // blocked step: one block per tile of its inner cells
+      // blocked step: one block per tile of its inner cells.  w = 1 marks an
+      // interior tile: its extended tile lies inside every update range and
+      // outside the CPML slabs and the source box, and no capture plane
+      // falls among its inner planes
       const int tx = kTbPlanes - 2 * xc, ty = 64 - 2 * xc;
+      const int xlo = d.offset + d.num_cpml, xhi = d.nx - d.offset - d.num_cpml;
+      const int ylo = d.offset + d.num_cpml, yhi = d.ny - d.offset - d.num_cpml;
       for (int x0 = 0; x0 < d.nx; x0 += tx)
-        for (int y0 = 0; y0 < d.ny; y0 += ty) tiles.push_back(make_int4(si, x0, y0, 0));
+        for (int y0 = 0; y0 < d.ny; y0 += ty) {
+          const int ex0 = x0 - xc, ex1 = ex0 + kTbPlanes, ey0 = y0 - xc, ey1 = ey0 + 64;
+          bool in = ex0 >= xlo && ex1 <= xhi && ey0 >= ylo && ey1 <= yhi;
+          if (d.src_field >= 0 && ex0 < d.src_box[1] && ex1 > d.src_box[0] && ey0 < d.src_box[3] && ey1 > d.src_box[2])
+            in = false;
+          for (const int pl : {d.port[0], d.port[0] + 1, d.port[1], d.port[1] + 1})
+            if (pl >= x0 && pl < x0 + tx) in = false;
+          tiles.push_back(make_int4(si, x0, y0, in ? 1 : 0));
+        }
     } else if (d.scheme == SCHEME_3D) {
       for (int x0 = 0; x0 < d.nx; x0 += xc)
         for (int y0 = 0; y0 < d.ny; y0 += kBY)

@@ -2266,7 +2266,37 @@ __global__ __launch_bounds__(64 * W) void k_tmz_tb(const Slot<T>* __restrict__ s
     T* const ring = v.ring;
     const int pcells = v.pcells;
     const size_t cw = (size_t)(j - v.wy0);
-    for (int t = 0; t < nsteps; ++t) {
+    // Interior tiles (tile flag set by the host): every cell of the extended
+    // tile is in every component's update range, outside the CPML slabs, the
+    // source box and the capture planes, so the update is the bulk curl
+    // alone (the same expressions as below, whose other terms are absent
+    // there).
+    for (int t = 0; tl.w != 0 && t < nsteps; ++t) {
+      xhy[w][lane] = hy[0];
+      lds_barrier();
+      const T hyup = w + 1 < W ? xhy[w + 1][lane] : hy[K - 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const T hxj = lane_down1(hx[q], hx[q]);
+        const T hyi = q + 1 < K ? hy[q + 1] : hyup;
+        const T dHy = hyi - hy[q];
+        const T dHx = hxj - hx[q];
+        ez[q] = ez[q] + cbz[q] * (dHy * lane_get(lk_ikx, q) - dHx * jcf.iky);
+      }
+      xez[w][lane] = ez[K - 1];
+      lds_barrier();
+      const T ezdn = w > 0 ? xez[w - 1][lane] : ez[0];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const T ezj = lane_up1(ez[q], ez[q]);
+        const T prevE = q > 0 ? ez[q - 1] : ezdn;
+        const T dEzy = ez[q] - ezj;
+        hx[q] = hx[q] + db * (-dEzy) * ikyh;
+        const T dEzx = ez[q] - prevE;
+        hy[q] = hy[q] + db * dEzx * lane_get(lk_ikh, q);
+      }
+    }
+    for (int t = 0; tl.w == 0 && t < nsteps; ++t) {
       const T jv = lane_get(lk_jv, t);
       const size_t rbase = (size_t)(tstep + t) * v.nplanes;
       // ---- E half-step (tmz_e per cell): Hy(i+1) of the last plane from the next wave
@@ -2484,7 +2514,33 @@ __global__ __launch_bounds__(64 * W) void k_tez_tb(const Slot<T>* __restrict__ s
     T* const ring = v.ring;
     const int pcells = v.pcells;
     const size_t cw = (size_t)(j - v.wy0);
-    for (int t = 0; t < nsteps; ++t) {
+    // Interior tiles (as in k_tmz_tb): the bulk curls alone.
+    for (int t = 0; tl.w != 0 && t < nsteps; ++t) {
+      xhz[w][lane] = hz[0];
+      lds_barrier();
+      const T hzup = w + 1 < W ? xhz[w + 1][lane] : hz[K - 1];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const T hzj = lane_down1(hz[q], hz[q]);
+        const T hzi = q + 1 < K ? hz[q + 1] : hzup;
+        const T dHzy = hzj - hz[q];
+        ex[q] = ex[q] + cb0[q] * dHzy * ikye;
+        const T dHzx = hzi - hz[q];
+        ey[q] = ey[q] + cb1[q] * (-dHzx) * lane_get(lk_ikx, q);
+      }
+      xey[w][lane] = ey[K - 1];
+      lds_barrier();
+      const T eydn = w > 0 ? xey[w - 1][lane] : ey[0];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const T exj = lane_up1(ex[q], ex[q]);
+        const T prevEy = q > 0 ? ey[q - 1] : eydn;
+        const T dEx = ex[q] - exj;
+        const T dEy = ey[q] - prevEy;
+        hz[q] = hz[q] + db * (dEx * ikyh - dEy * lane_get(lk_ikh, q));
+      }
+    }
+    for (int t = 0; tl.w == 0 && t < nsteps; ++t) {
       const T jv = lane_get(lk_jv, t);
       const size_t rbase = (size_t)(tstep + t) * v.nplanes;
       // ---- E half-step (tez_e per cell): Hz(i+1) of the last plane from the next wave

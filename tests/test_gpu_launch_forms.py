@@ -105,6 +105,34 @@ def test_blocked_step_fp64(scheme, tb, direction, extra):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('scheme,tb,direction', [('tm', 8, 'f'), ('tm', 4, 'b'), ('te', 8, 'b'), ('te', 2, 'f')])
+def test_blocked_step_interior_tiles_fp64(scheme, tb, direction):
+    """A grid wide enough (200 x 160) for tiles whose extended region lies
+    wholly inside the CPML-free bulk: those take the blocked step's interior
+    path; fields and port DFT still equal the oracle to 1e-10."""
+    p = configs.slab2d(200, ny=160, nt=301, nf=4, precision='float64', scheme=scheme, core=12, port_len=30,
+                       source_type=1, gauss_pulse_deg=-1.0)
+    p.update(rough_std=15e-9, rough_acl=40e-9, seed=9)
+    s, up, lo, fg = device_inputs(p)
+    o, _ = oracle_run(p, up, lo, direction)
+    print('case blocked interior', scheme, 'tb', tb, direction, flush=True)
+    b = Batch(scheme, np.float64, max_slots=1)
+    b.set_launch(tb, 0, 4)
+    slot = b.add(s, fg, direction)
+    b.advance(-(-s.nt // 32))
+    b.sync()
+    for name, ref in o.fields().items():
+        err = rel_l2(b.field(slot, name), ref)
+        assert err < 1e-10, (name, err)
+    w = window(scheme, o.c)
+    od = o.dft()
+    for key, z in b.dft(slot).items():
+        err = rel_l2(z[w], od[key][w])
+        assert err < 1e-10, (key, err)
+    b.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize('scheme', ['tm', 'te'])
 @pytest.mark.parametrize('precision', ['float32', 'float64'])
 @pytest.mark.parametrize('tb', [2, 4, 8])
