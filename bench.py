@@ -499,8 +499,8 @@ def roofline_of(scheme, es, variant, xchunk, cells, ms, key):
 
     ``achieved`` is the algorithmic bytes of the cell-updates one launch
     performs (each field component read twice and written once per Yee
-    step) over the launch's duration.  The temporally blocked TMz step
-    (variant 4) performs ``xchunk`` steps per launch while moving each field
+    step) over the launch's duration.  The temporally blocked steps
+    (variant 4) perform ``xchunk`` steps per launch while moving each field
     value through HBM once, so its figure can exceed the HBM peak; for it the
     line also gives the bytes it must move at least (every field value read
     once and written once per launch) and their rate, which cannot."""
@@ -510,7 +510,7 @@ def roofline_of(scheme, es, variant, xchunk, cells, ms, key):
     steps = 1
     if variant == 4:
         steps = int(xchunk)
-        kname, ms_k, bytes_k = ('k_tmz_tb', ms_e, (be + bh) * cells * steps)
+        kname, ms_k, bytes_k = (stem + '_tb', ms_e, (be + bh) * cells * steps)
     elif variant == 3:
         kname, ms_k, bytes_k = (stem + ('_halo' if scheme == '3d' else '_step'), ms_e, (be + bh) * cells)
     elif ms_e >= ms_h:

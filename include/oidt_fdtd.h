@@ -285,7 +285,8 @@ int oidt_batch_remove(oidt_batch* b, int32_t slot);
  * issues the next plane's loads before updating the current one; 2 = batched
  * loads per x-chunk, the 2D default; 3 = fused Yee step, one launch per step,
  * all schemes, xchunk also 12, 20 or 24 in 2D, 1, 2, 4, 8 or 16 in 3D; 4 = temporally
- * blocked step, TMz and TEz, xchunk = steps per launch, 2, 4 or 8).  Once a run
+ * blocked step, TMz, TEz and 3D, xchunk = steps per launch, 2, 4 or 8 in 2D,
+ * 2 in 3D).  Once a run
  * has started, its form cannot change between two-kernel, fused and blocked,
  * nor the blocked step's depth (OIDT_ESTATE).
  * No counterpart in the reference (fixed launch shapes, fdtd_main.py:263-273). */

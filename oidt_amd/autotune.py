@@ -8,7 +8,7 @@ that every timed launch does a whole step for every run.  Results are cached
 per workload class in a JSON file so a serving process tunes each class once.
 
 Kernel forms: 1 = per-plane march, 2 = batched loads (two launches per step),
-3 = fused Yee step (one launch per step), 4 = temporally blocked 2D step
+3 = fused Yee step (one launch per step), 4 = temporally blocked step
 (x-chunk = steps per launch).  Otherwise the x-chunk depth is the number of
 planes each thread or wave owns along x.
 """
@@ -28,7 +28,7 @@ CANDIDATES = {
            (3, 20, 0), (3, 24, 0), (3, 32, 0), (4, 2, 0), (4, 4, 0), (4, 8, 0)],
     'te': [(2, 2, 0), (2, 4, 0), (2, 4, 65536), (2, 8, 0), (1, 4, 0), (3, 1, 0), (3, 2, 0), (3, 8, 0), (3, 16, 0),
            (4, 2, 0), (4, 4, 0), (4, 8, 0)],
-    '3d': [(1, 2, 0), (1, 4, 0), (1, 4, 65536), (1, 8, 0), (2, 1, 0), (2, 1, 65536), (3, 8, 0)],
+    '3d': [(1, 2, 0), (1, 4, 0), (1, 4, 65536), (1, 8, 0), (2, 1, 0), (2, 1, 65536), (3, 8, 0), (4, 2, 0)],
 }
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'oidt_amd', 'autotune.json')

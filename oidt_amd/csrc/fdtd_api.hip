@@ -523,6 +523,10 @@ bool march_tiles(const oidt_batch* b) { return b->variant == 3; }
 // z-cells a wave of the fused 3D step owns (its lane 0 is the cell below).
 constexpr int kHaloZ = kBX - 1;
 
+// Waves (rows) of a blocked 3D step's block: 16 in FP32; 8 in FP64, whose
+// cells take twice the registers.
+int tb3_waves(int precision) { return precision == OIDT_F64 ? 8 : 16; }
+
 // Monitor table and the two work lists (after the E and after the H
 // half-step): one item per kMonCells consecutive box cells.
 template <typename T>
@@ -583,6 +587,13 @@ int rebuild_lists(oidt_batch* b) {
       for (int x0 = 0; x0 < d.nx; x0 += xc)
         for (int y0 = 0; y0 < d.ny; ++y0)
           for (int z0 = 0; z0 < d.nz; z0 += kHaloZ) tiles.push_back(make_int4(si, x0, y0, z0));
+    } else if (d.scheme == SCHEME_3D && b->variant == 4) {
+      // blocked 3D step: one block per segment of kTb3Seg planes by its inner
+      // rows and z-cells
+      const int ty = tb3_waves(b->precision) - 2 * xc, tz = kBX - 2 * xc;
+      for (int x0 = 0; x0 < d.nx; x0 += kTb3Seg)
+        for (int y0 = 0; y0 < d.ny; y0 += ty)
+          for (int z0 = 0; z0 < d.nz; z0 += tz) tiles.push_back(make_int4(si, x0, y0, z0));
     } else if (b->variant == 4) {
       // blocked step: one block per tile of its inner cells.  w = 1 marks an
       // interior tile: its extended tile lies inside every update range and
@@ -738,9 +749,19 @@ void launch_blocked_tb(oidt_batch* b, int gs, int tstep) {
     default: launch_blocked_w<T, 8, W>(b, gs, tstep); break;
   }
 }
+// The temporally blocked 3D step: 2 steps per launch, blocks of
+// tb3_waves() rows.
+template <typename T>
+void launch_blocked3d(oidt_batch* b, int gs, int tstep) {
+  const Slot<T>* sl = static_cast<const Slot<T>*>(b->d_slots);
+  constexpr int W = sizeof(T) == 8 ? 8 : 16;
+  hipLaunchKernelGGL((k_3d_tb<T, 2, W>), dim3(gs), dim3(64, W), 0, b->stream, sl, b->d_tiles, b->d_counts, tstep);
+}
+
 template <typename T>
 void launch_blocked(oidt_batch* b, int gs, int tstep) {
-  if (sizeof(T) == 8) launch_blocked_tb<T, 16>(b, gs, tstep);
+  if (b->scheme == SCHEME_3D) launch_blocked3d<T>(b, gs, tstep);
+  else if (sizeof(T) == 8) launch_blocked_tb<T, 16>(b, gs, tstep);
   else launch_blocked_tb<T, 8>(b, gs, tstep);
 }
 
@@ -1062,9 +1083,13 @@ int oidt_batch_set_launch(oidt_batch* b, int32_t xchunk, int32_t grid_blocks, in
   if (variant == 0) variant = b->variant;
   const int xc = xchunk ? xchunk : b->xchunk;   // the depth this launch form will use
   if (variant == 4) {
-    if (b->scheme != SCHEME_TMZ && b->scheme != SCHEME_TEZ)
-      return fail(OIDT_EINVAL, "the blocked step (variant 4) is for TMz and TEz runs");
-    if (xc != 2 && xc != 4 && xc != 8) return fail(OIDT_EINVAL, "the blocked step takes 2, 4 or 8 steps per launch");
+    if (b->scheme == SCHEME_3D) {
+      if (xc != 2) return fail(OIDT_EINVAL, "the blocked 3D step takes 2 steps per launch");
+    } else if (b->scheme != SCHEME_TMZ && b->scheme != SCHEME_TEZ) {
+      return fail(OIDT_EINVAL, "the blocked step (variant 4) is for TMz, TEz and 3D runs");
+    } else if (xc != 2 && xc != 4 && xc != 8) {
+      return fail(OIDT_EINVAL, "the blocked step takes 2, 4 or 8 steps per launch");
+    }
   }
   if (variant == 3 && b->scheme == SCHEME_3D && xc != 1 && xc != 2 && xc != 4 && xc != 8 && xc != 16)
     return fail(OIDT_EINVAL, "the fused 3D step takes 1, 2, 4, 8 or 16 planes per wave");

@@ -2998,6 +2998,418 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
   }
 }
 
+// ------------------------------------------------- temporally blocked 3D ---
+// TB Yee steps per launch, streamed along x.  A block of W waves takes W rows
+// (y, one per wave) by 64 z-cells (one per lane) and marches them over a
+// segment of planes; its own cells are the inner (W - 2 TB) rows by
+// (64 - 2 TB) z-cells of the kTb3Seg planes from x0.  Step s of the launch
+// runs s planes behind step 0: at position p, step 0 updates plane p - 1
+// (its E reads H of plane p, loaded one position ahead) and step s plane
+// p - 1 - s (its E reads H of plane p - s, which step s - 1 has just
+// produced; its H reads E of plane p - 2 - s, which it produced itself one
+// position earlier).  Every field and psi value crosses HBM once per launch
+// (with the TB ghost planes at both ends of the segment and the TB-cell
+// ghost frame of rows and z-cells), instead of twice per step.  Neighbours:
+// z by DPP lane shifts, y through LDS (old Hx and Hz of the row above for
+// the E update, new Ex and Ez of the row below for the H update), x in
+// registers.  Each step moves the cells that depend on values outside the
+// tile one cell inwards from every side (E reads H at +1 and H reads E at -1
+// along each axis), so after TB steps the inner cells are exact.  The
+// per-cell arithmetic and update ranges are k_3d_e's and k_3d_h's.  The
+// launch reads field/psi set ((n0 / TB) & 1) and writes the inner cells'
+// state after its steps to the other set.
+constexpr int kTb3Seg = 32;   // planes of a 3D blocked tile
+
+// The run descriptor through a pointer the compiler cannot see through, in
+// the constant address space: the pointers and constants read through it are
+// loaded again (scalar loads, cache hits) where they are used, instead of
+// being kept in scalar registers over the whole march, which does not have
+// enough of them.  The descriptors do not change during a stencil launch.
+template <typename T>
+using ConstSlot = const __attribute__((address_space(4))) Slot<T>;
+template <typename T>
+__device__ __forceinline__ ConstSlot<T>* reload_slot(const Slot<T>* p) {
+  asm volatile("" : "+s"(p));
+  return (ConstSlot<T>*)p;
+}
+
+// One cell's state: fields, psi and material byte.
+template <typename T>
+struct Cell3 {
+  T ex, ey, ez, hx, hy, hz;
+  T qxy, qzy, qxz, qyz, qyx, qzx;   // E psi: Ex_y, Ez_y, Ex_z, Ey_z, Ey_x, Ez_x
+  T rxy, rzy, rxz, ryz, ryx, rzx;   // H psi: Hx_y, Hz_y, Hx_z, Hy_z, Hy_x, Hz_x
+  unsigned m;
+};
+
+// Per-lane conditions of the blocked 3D step, one bit each.
+enum : unsigned {
+  L3_KIN = 1u, L3_KIN1 = 2u, L3_SRCE = 4u, L3_SRCH = 8u, L3_OWN = 16u, L3_SK = 32u, L3_SKH = 64u,
+  L3_CAPE0 = 128u,    // Ex, Ey at port 0 (capture planes 0, 1)
+  L3_CAPE1 = 256u,    // Ex, Ey at port 1 (planes 6, 7), or at port 0 when the ports coincide
+  L3_CAPA = 512u, L3_CAPB = 1024u, L3_CAPC = 2048u, L3_CAPD = 4096u,   // Hx, Hy at port0, +1, port1, +1
+  L3_CAP = L3_CAPE0 | L3_CAPE1 | L3_CAPA | L3_CAPB | L3_CAPC | L3_CAPD
+};
+
+template <typename T>
+__device__ __forceinline__ T opaque_vgpr(T v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <typename T, int TB, int W>
+__global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ slots,
+                                                  const int4* __restrict__ tiles,
+                                                  const int* __restrict__ counts, int tstep) {
+  constexpr int TY = W - 2 * TB, TZ = 64 - 2 * TB;   // inner rows and z-cells of a tile
+  static_assert(TB == 2 && TY > 0 && kTb3Seg + 2 * TB <= 64, "two steps per launch");
+  constexpr unsigned es = sizeof(T);
+  __shared__ T xh[TB][2][W][64];   // old Hx, Hz of each row at each step's plane
+  __shared__ T xe[2][2][W][64];    // new Ex, Ez of each row (alternate steps alternate halves)
+  const int ntiles = counts[0];
+  const int lane = threadIdx.x;
+  const int w = wave_id();
+  for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
+    const int4 tl = tiles[tt];
+    const Slot<T>* const sp = slots + tl.x;
+    const Slot<T>& s = *sp;
+    const int n0 = s.base + tstep;
+    if (!s.active || n0 >= s.nt) continue;      // uniform over the block
+    const int nsteps = min(TB, s.nt - n0);
+    const bool odd = ((n0 / TB) & 1) != 0;
+    const int nx = s.nx, ny = s.ny, nz = s.nz, N = s.N;
+    const int ib = s.ib, ie = s.ie, jb = s.jb, je = s.je, kb = s.kb, ke = s.ke;
+    const int x0 = tl.y, j0 = tl.z, k0 = tl.w;
+    const int j = j0 - TB + w, k = k0 - TB + lane;
+    const int jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
+    const bool own = j >= j0 && j < j0 + TY && j < ny && k >= k0 && k < k0 + TZ && k < nz;
+    const int xo1 = min(x0 + kTb3Seg, nx);                       // inner planes [x0, xo1)
+    const int xf = max(x0 - TB, 0), xl = min(xo1 - 1 + TB, nx - 1);   // loaded planes [xf, xl]
+    const unsigned pl = (unsigned)ny * nz;
+    const unsigned jk = (unsigned)jc * nz + kc;
+    // ---- row (wave-uniform) constants
+    const bool jin = j >= jb && j < je, jin1 = j > jb && j < je;
+    int dj = 0, cj = 0, djh = 0, cjh = 0;
+    const int sj = jin ? slab_e(j, jb, je, N, dj, cj) : -1;
+    const int sjh = jin1 ? slab_h(j, jb, je, N, djh, cjh) : -1;
+    const T iky = jin ? s.ike[1][j] : T(0), ikyh = jin1 ? s.ikh[1][j] : T(0);
+    const T bej = sj >= 0 ? s.be[cj] : T(0), cej = sj >= 0 ? s.ceo[cj] : T(0);
+    const T bhj = sjh >= 0 ? s.bh[cjh] : T(0), chj = sjh >= 0 ? s.cho[cjh] : T(0);
+    // ---- lane (z) constants and conditions; zs: some z-cell of the tile
+    // lies in a z slab
+    const bool kin = k >= kb && k < ke, kin1 = k > kb && k < ke;
+    int dk = 0, ck = 0, dkh = 0, ckh = 0;
+    const int sk = kin ? slab_e(k, kb, ke, N, dk, ck) : -1;
+    const int skh = kin1 ? slab_h(k, kb, ke, N, dkh, ckh) : -1;
+    const T ikz = kin ? s.ike[2][kc] : T(0), ikzh = kin1 ? s.ikh[2][kc] : T(0);
+    const T bek = sk >= 0 ? s.be[ck] : T(0), cek = sk >= 0 ? s.ceo[ck] : T(0);
+    const T bhk = skh >= 0 ? s.bh[ckh] : T(0), chk = skh >= 0 ? s.cho[ckh] : T(0);
+    const bool zs = k0 - TB < kb + N || k0 - TB + 64 > ke - N;
+    const unsigned xhi = s.psi_hi[0];
+    // z-slab psi offsets within a plane row (the row part is added per plane)
+    const unsigned izk = (unsigned)dk * es + (sk == 1 ? s.psi_hi[2] : 0u);
+    const unsigned izkh = (unsigned)dkh * es + (skh == 1 ? s.psi_hi[2] : 0u);
+    // sources: TE on Ey (E half-step), TM on Hy (start of the H half-step);
+    // lane t holds the waveform of step n0 + t
+    const bool ksrc = k >= s.sz0 && k < s.sz1 && j >= s.sy0 && j < s.sy1;
+    const T lk_jv = s.src_field >= 0 && lane < nsteps ? s.jsrc[n0 + lane] : T(0);
+    // capture: E at k = port (planes cap*6 + 0/1; 6/7 when the ports coincide),
+    // Hx/Hy at k = port (2/4, 8/10) and port + 1 (3/5, 9/11)
+    const bool jcap = own && j >= s.wy0 && j <= s.wy1;
+    const int port0 = s.port[0], port1 = s.port[1];
+    const unsigned lflags =
+        (kin ? L3_KIN : 0u) | (kin1 ? L3_KIN1 : 0u) | (s.src_field == F_EY && ksrc ? L3_SRCE : 0u) |
+        (s.src_field == F_HY && ksrc ? L3_SRCH : 0u) | (own ? L3_OWN : 0u) | (sk >= 0 ? L3_SK : 0u) |
+        (skh >= 0 ? L3_SKH : 0u) | (jcap && k == port0 ? L3_CAPE0 | L3_CAPA : 0u) |
+        (jcap && k == port1 ? L3_CAPE1 | L3_CAPC : 0u) | (jcap && k == port0 + 1 ? L3_CAPB : 0u) |
+        (jcap && k == port1 + 1 ? L3_CAPD : 0u);
+    // some lane of the tile captures (wave-uniform)
+    const bool tcap = __builtin_amdgcn_readfirstlane(__any((lflags & L3_CAP) != 0u)) != 0;
+    const bool tsrc = s.src_field >= 0;
+    // ---- x-only coefficients: lane u holds plane xf + u
+    T lx_ik = T(0), lx_be = T(0), lx_ce = T(0), lx_ikh = T(0), lx_bh = T(0), lx_ch = T(0);
+    {
+      const int ip = xf + lane;
+      if (ip <= xl) {
+        int d = 0, ci = 0;
+        lx_ik = s.ike[0][ip];
+        if (slab_e(ip, ib, ie, N, d, ci) >= 0) { lx_be = s.be[ci]; lx_ce = s.ceo[ci]; }
+        lx_ikh = s.ikh[0][ip];
+        if (slab_h(ip, ib, ie, N, d, ci) >= 0) { lx_bh = s.bh[ci]; lx_ch = s.cho[ci]; }
+      }
+    }
+
+    // Plane i's fields and material byte, and its psi values, from set A
+    // (rows and z-cells outside the grid load a clamped neighbour: no cell of
+    // the grid that a step updates reads them).
+    auto load_f = [&](int i, Cell3<T>& c) {
+      const auto* const q = reload_slot(sp);
+      const auto* const fa = odd ? q->f2 : q->f;
+      const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
+      c.ex = ldg<T>(fa[F_EX], bo);
+      c.ey = ldg<T>(fa[F_EY], bo);
+      c.ez = ldg<T>(fa[F_EZ], bo);
+      c.hx = ldg<T>(fa[F_HX], bo);
+      c.hy = ldg<T>(fa[F_HY], bo);
+      c.hz = ldg<T>(fa[F_HZ], bo);
+      c.m = ldg<uint8_t>(q->mat, idx);
+    };
+    auto load_h = [&](int i, T& hy, T& hz) {
+      const auto* const q = reload_slot(sp);
+      const auto* const fa = odd ? q->f2 : q->f;
+      const unsigned bo = ((unsigned)i * pl + jk) * es;
+      hy = ldg<T>(fa[F_HY], bo);
+      hz = ldg<T>(fa[F_HZ], bo);
+    };
+    auto load_q = [&](int i, unsigned lf, Cell3<T>& c) {
+      const auto* const q = reload_slot(sp);
+      const auto* const psiA = odd ? q->psi2 : q->psi;
+      int di = 0, ci = 0, dih = 0, cih = 0;
+      const int si = slab_e(i, ib, ie, N, di, ci);
+      const int sih = slab_h(i, ib, ie, N, dih, cih);
+      // every lane loads; outside a slab all lanes read element 0 (one line)
+      const unsigned ix = si >= 0 ? ((unsigned)di * pl + jk) * es + (si == 1 ? xhi : 0u) : 0u;
+      const unsigned ixh = sih >= 0 ? ((unsigned)dih * pl + jk) * es + (sih == 1 ? xhi : 0u) : 0u;
+      const unsigned iy = sj >= 0 ? (((unsigned)i * N + dj) * nz + kc) * es : 0u;
+      const unsigned iyh = sjh >= 0 ? (((unsigned)i * N + djh) * nz + kc) * es : 0u;
+      const unsigned zrow = ((unsigned)i * ny + jc) * N * es;
+      const unsigned iz = (lf & L3_SK) ? zrow + izk : 0u;
+      const unsigned izh = (lf & L3_SKH) ? zrow + izkh : 0u;
+      const int ys = max(sj, 0), ysh = max(sjh, 0);
+      c.qyx = ldg<T>(psiA[psi_id(F_EY, 0, 0)], ix);
+      c.qzx = ldg<T>(psiA[psi_id(F_EZ, 0, 0)], ix);
+      c.ryx = ldg<T>(psiA[psi_id(F_HY, 0, 0)], ixh);
+      c.rzx = ldg<T>(psiA[psi_id(F_HZ, 0, 0)], ixh);
+      c.qxy = ldg<T>(psiA[psi_id(F_EX, 1, ys)], iy);
+      c.qzy = ldg<T>(psiA[psi_id(F_EZ, 1, ys)], iy);
+      c.rxy = ldg<T>(psiA[psi_id(F_HX, 1, ysh)], iyh);
+      c.rzy = ldg<T>(psiA[psi_id(F_HZ, 1, ysh)], iyh);
+      c.qxz = ldg<T>(psiA[psi_id(F_EX, 2, 0)], iz);
+      c.qyz = ldg<T>(psiA[psi_id(F_EY, 2, 0)], iz);
+      c.rxz = ldg<T>(psiA[psi_id(F_HX, 2, 0)], izh);
+      c.ryz = ldg<T>(psiA[psi_id(F_HY, 2, 0)], izh);
+    };
+
+    // One Yee step of cell c (plane i, step n0 + st): E, then H.  hyn, hzn:
+    // old Hy, Hz of plane i + 1; hxu, hzu: old Hx, Hz of row j + 1; pey,
+    // pez: new Ey, Ez of plane i - 1 (replaced by this plane's).
+    auto step = [&](Cell3<T>& c, int i, int st, unsigned lf, T hyn, T hzn, T hxu, T hzu, T& pey, T& pez) {
+      const auto* const q = reload_slot(sp);
+      const T db = q->db;
+      const int u = i - xf;
+      const T ikx = lane_get(lx_ik, u), bex = lane_get(lx_be, u), cex = lane_get(lx_ce, u);
+      const T ikxh = lane_get(lx_ikh, u), bhx = lane_get(lx_bh, u), chx = lane_get(lx_ch, u);
+      const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
+      const bool si = i >= ib && (i < ib + N || i >= ie - N) && i < ie;          // in an E x slab
+      const bool sih = i > ib && (i < ib + N || i > ie - N) && i < ie;           // in an H x slab
+      const bool lkin = (lf & L3_KIN) != 0u, lkin1 = (lf & L3_KIN1) != 0u;
+      // ---- E (k_3d_e per cell)
+      const T hxk = lane_down1(c.hx, c.hx);
+      const T hyk = lane_down1(c.hy, c.hy);
+      T vx = c.ex, vy = c.ey, vz = c.ez;
+      if (iin1 && jin) {
+        const T cb = cb_of(q->cbv[0][0], q->cbv[0][1], q->cbv[0][2], q->cbv[0][3], c.m, 0);
+        const T dHz = hzu - c.hz;
+        const T dHy = hyk - c.hy;
+        T v = vx + cb * (dHz * iky - dHy * ikz);
+        if (sj >= 0) { const T qv = bej * c.qxy + cej * dHz; c.qxy = qv; v += cb * qv; }
+        if (zs) { const T qv = bek * c.qxz + cek * dHy; c.qxz = qv; v -= cb * qv; }
+        vx = lkin ? v : vx;
+      }
+      const bool srcp = tsrc && i >= q->sx0 && i < q->sx1 && (lf & L3_SRCE) != 0u;
+      const T jv = tsrc ? lane_get(lk_jv, st) : T(0);
+      if (iin && jin1) {
+        const T cb = cb_of(q->cbv[1][0], q->cbv[1][1], q->cbv[1][2], q->cbv[1][3], c.m, 1);
+        const T dHx = hxk - c.hx;
+        const T dHz = hzn - c.hz;
+        T v = vy + cb * (dHx * ikz - dHz * ikx);
+        v = srcp ? v - jv : v;
+        if (si) { const T qv = bex * c.qyx + cex * dHz; c.qyx = qv; v -= cb * qv; }
+        if (zs) { const T qv = bek * c.qyz + cek * dHx; c.qyz = qv; v += cb * qv; }
+        vy = lkin ? v : (srcp ? vy - jv : vy);
+      } else {
+        vy = srcp ? vy - jv : vy;
+      }
+      if (iin && jin) {
+        const T cb = cb_of(q->cbv[2][0], q->cbv[2][1], q->cbv[2][2], q->cbv[2][3], c.m, 2);
+        const T dHy = hyn - c.hy;
+        const T dHx = hxu - c.hx;
+        T v = vz + cb * (dHy * ikx - dHx * iky);
+        if (si) { const T qv = bex * c.qzx + cex * dHy; c.qzx = qv; v += cb * qv; }
+        if (sj >= 0) { const T qv = bej * c.qzy + cej * dHx; c.qzy = qv; v -= cb * qv; }
+        vz = lkin1 ? v : vz;
+      }
+      c.ex = vx;
+      c.ey = vy;
+      c.ez = vz;
+      const bool capx = tcap && i >= x0 && i < xo1 && i >= q->wx0 && i <= q->wx1;
+      if (capx) {
+        T* __restrict__ ring = q->ring;
+        const size_t pc = q->pcells;
+        const size_t rbase = (size_t)(tstep + st) * q->nplanes;
+        const size_t cw = (size_t)((i - q->wx0) * q->prow + (j - q->wy0));
+        if (lf & L3_CAPE0) {
+          ring[(rbase + 0) * pc + cw] = vx;
+          ring[(rbase + 1) * pc + cw] = vy;
+        }
+        if (lf & L3_CAPE1) {
+          ring[(rbase + 6) * pc + cw] = vx;
+          ring[(rbase + 7) * pc + cw] = vy;
+        }
+      }
+      // ---- new Ex, Ez of row j - 1 from the wave below
+      T* const xa = &xe[st & 1][0][0][0];
+      T* const xb = &xe[st & 1][1][0][0];
+      xa[w * 64 + lane] = vx;
+      xb[w * 64 + lane] = vz;
+      lds_barrier();
+      const T exm = w > 0 ? xa[(w - 1) * 64 + lane] : vx;
+      const T ezm = w > 0 ? xb[(w - 1) * 64 + lane] : vz;
+      // ---- H (k_3d_h per cell)
+      const T exk = lane_up1(vx, vx), eyk = lane_up1(vy, vy);
+      T hx = c.hx;
+      if (iin && jin1) {
+        const T dEy = vy - eyk;
+        const T dEz = vz - ezm;
+        T v = hx + db * (dEy * ikzh - dEz * ikyh);
+        if (sjh >= 0) { const T qv = bhj * c.rxy + chj * dEz; c.rxy = qv; v -= db * qv; }
+        if (zs) { const T qv = bhk * c.rxz + chk * dEy; c.rxz = qv; v += db * qv; }
+        hx = lkin1 ? v : hx;
+      }
+      const bool srch = tsrc && i >= q->sx0 && i < q->sx1 && (lf & L3_SRCH) != 0u;
+      T hy = srch ? c.hy - jv : c.hy;
+      if (iin1 && jin) {
+        const T dEz = vz - pez;
+        const T dEx = vx - exk;
+        T v = hy + db * (dEz * ikxh - dEx * ikzh);
+        if (sih) { const T qv = bhx * c.ryx + chx * dEz; c.ryx = qv; v += db * qv; }
+        if (zs) { const T qv = bhk * c.ryz + chk * dEx; c.ryz = qv; v -= db * qv; }
+        hy = lkin1 ? v : hy;
+      }
+      T hz = c.hz;
+      if (iin1 && jin1) {
+        const T dEx = vx - exm;
+        const T dEy = vy - pey;
+        T v = hz + db * (dEx * ikyh - dEy * ikxh);
+        if (sih) { const T qv = bhx * c.rzx + chx * dEy; c.rzx = qv; v -= db * qv; }
+        if (sjh >= 0) { const T qv = bhj * c.rzy + chj * dEx; c.rzy = qv; v += db * qv; }
+        hz = lkin ? v : hz;
+      }
+      c.hx = hx;
+      c.hy = hy;
+      c.hz = hz;
+      pey = vy;
+      pez = vz;
+      if (capx) {
+        T* __restrict__ ring = q->ring;
+        const size_t pc = q->pcells;
+        const size_t rbase = (size_t)(tstep + st) * q->nplanes;
+        const size_t cw = (size_t)((i - q->wx0) * q->prow + (j - q->wy0));
+        if (lf & L3_CAPA) { ring[(rbase + 2) * pc + cw] = hx; ring[(rbase + 4) * pc + cw] = hy; }
+        if (lf & L3_CAPB) { ring[(rbase + 3) * pc + cw] = hx; ring[(rbase + 5) * pc + cw] = hy; }
+        if (lf & L3_CAPC) { ring[(rbase + 8) * pc + cw] = hx; ring[(rbase + 10) * pc + cw] = hy; }
+        if (lf & L3_CAPD) { ring[(rbase + 9) * pc + cw] = hx; ring[(rbase + 11) * pc + cw] = hy; }
+      }
+    };
+
+    // plane i's inner cells to set B (fields unconditionally: cells no step
+    // updates hold the same value in both sets; psi where the steps update it)
+    auto store = [&](const Cell3<T>& c, int i, unsigned lf) {
+      const auto* const q = reload_slot(sp);
+      const auto* const fb = odd ? q->f : q->f2;
+      const auto* const psiB = odd ? q->psi : q->psi2;
+      const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
+      stg_nt<T>(fb[F_EX], bo, c.ex);
+      stg_nt<T>(fb[F_EY], bo, c.ey);
+      stg_nt<T>(fb[F_EZ], bo, c.ez);
+      stg_nt<T>(fb[F_HX], bo, c.hx);
+      stg_nt<T>(fb[F_HY], bo, c.hy);
+      stg_nt<T>(fb[F_HZ], bo, c.hz);
+      const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
+      const bool lkin = (lf & L3_KIN) != 0u, lkin1 = (lf & L3_KIN1) != 0u;
+      int di = 0, ci = 0, dih = 0, cih = 0;
+      const int si = slab_e(i, ib, ie, N, di, ci);
+      const int sih = slab_h(i, ib, ie, N, dih, cih);
+      if (si >= 0) {
+        const unsigned ix = ((unsigned)di * pl + jk) * es + (si == 1 ? xhi : 0u);
+        if (iin && jin1 && lkin) stg<T>(psiB[psi_id(F_EY, 0, 0)], ix, c.qyx);
+        if (iin && jin && lkin1) stg<T>(psiB[psi_id(F_EZ, 0, 0)], ix, c.qzx);
+      }
+      if (sih >= 0) {
+        const unsigned ixh = ((unsigned)dih * pl + jk) * es + (sih == 1 ? xhi : 0u);
+        if (iin1 && jin && lkin1) stg<T>(psiB[psi_id(F_HY, 0, 0)], ixh, c.ryx);
+        if (iin1 && jin1 && lkin) stg<T>(psiB[psi_id(F_HZ, 0, 0)], ixh, c.rzx);
+      }
+      if (sj >= 0) {
+        const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es;
+        if (iin1 && jin && lkin) stg<T>(psiB[psi_id(F_EX, 1, sj)], iy, c.qxy);
+        if (iin && jin && lkin1) stg<T>(psiB[psi_id(F_EZ, 1, sj)], iy, c.qzy);
+      }
+      if (sjh >= 0) {
+        const unsigned iyh = (((unsigned)i * N + djh) * nz + kc) * es;
+        if (iin && jin1 && lkin1) stg<T>(psiB[psi_id(F_HX, 1, sjh)], iyh, c.rxy);
+        if (iin1 && jin1 && lkin) stg<T>(psiB[psi_id(F_HZ, 1, sjh)], iyh, c.rzy);
+      }
+      if (zs) {
+        const unsigned zrow = ((unsigned)i * ny + jc) * N * es;
+        if (lf & L3_SK) {
+          if (iin1 && jin) stg<T>(psiB[psi_id(F_EX, 2, 0)], zrow + izk, c.qxz);
+          if (iin && jin1) stg<T>(psiB[psi_id(F_EY, 2, 0)], zrow + izk, c.qyz);
+        }
+        if (lf & L3_SKH) {
+          if (iin && jin1) stg<T>(psiB[psi_id(F_HX, 2, 0)], zrow + izkh, c.rxz);
+          if (iin1 && jin) stg<T>(psiB[psi_id(F_HY, 2, 0)], zrow + izkh, c.ryz);
+        }
+      }
+    };
+
+    // ---- the march.  At position p step 0 updates plane p - 1 and step 1
+    // plane p - 2, while plane p (fields and psi) and the old Hy, Hz of
+    // plane p + 1 load.  Three cell buffers take the roles in turn (the loop
+    // is unrolled by three, so no buffer is copied, and no load in flight
+    // waits).
+    T pey[TB], pez[TB];
+#pragma unroll
+    for (int st = 0; st < TB; ++st) pey[st] = pez[st] = T(0);
+    auto position = [&](int p, Cell3<T>& ld, Cell3<T>& c0, Cell3<T>& c1, T& hyl, T& hzl, T hyp, T hzp) {
+      const unsigned lf = opaque_vgpr(lflags);
+      if (p <= xl) {
+        load_f(p, ld);
+        load_q(p, lf, ld);
+      }
+      if (p + 1 <= xl) load_h(p + 1, hyl, hzl);
+      // old Hx, Hz of both steps' planes for the wave below
+      xh[0][0][w][lane] = c0.hx;
+      xh[0][1][w][lane] = c0.hz;
+      xh[1][0][w][lane] = c1.hx;
+      xh[1][1][w][lane] = c1.hz;
+      lds_barrier();
+      // uniform over the block
+      if (p - 1 >= xf && p - 1 <= xl) {
+        const T hxu = w + 1 < W ? xh[0][0][w + 1][lane] : c0.hx;
+        const T hzu = w + 1 < W ? xh[0][1][w + 1][lane] : c0.hz;
+        step(c0, p - 1, 0, lf, hyp, hzp, hxu, hzu, pey[0], pez[0]);
+      }
+      if (nsteps > 1 && p - 2 >= xf && p - 2 <= xl) {
+        const T hxu = w + 1 < W ? xh[1][0][w + 1][lane] : c1.hx;
+        const T hzu = w + 1 < W ? xh[1][1][w + 1][lane] : c1.hz;
+        step(c1, p - 2, 1, lf, c0.hy, c0.hz, hxu, hzu, pey[1], pez[1]);
+      }
+      // plane p - 2 is final
+      if (p - 2 >= x0 && p - 2 < xo1 && (lf & L3_OWN)) store(c1, p - 2, lf);
+    };
+    Cell3<T> b0{}, b1{}, b2{};
+    T hy0 = T(0), hz0 = T(0), hy1 = T(0), hz1 = T(0), hy2 = T(0), hz2 = T(0);
+    const int pend = xo1 - 1 + TB;
+    for (int p = xf; p <= pend; p += 3) {
+      position(p, b0, b2, b1, hy0, hz0, hy2, hz2);
+      if (p + 1 <= pend) position(p + 1, b1, b0, b2, hy1, hz1, hy0, hz0);
+      if (p + 2 <= pend) position(p + 2, b2, b1, b0, hy2, hz2, hy1, hz1);
+    }
+    lds_barrier();   // the next tile reuses the exchange planes
+  }
+}
+
 // ------------------------------------------------------------------- DFT ---
 // Work item (slot, q, cell block, first frequency | frequencies << 16, at most
 // kFB).  Folds the kChunk samples

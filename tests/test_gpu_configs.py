@@ -133,28 +133,33 @@ def c_oracle(p, direction='f', nsteps=None):
 @pytest.mark.gpu
 def test_config1_full_grid_against_c_oracle():
     """128 x 96 x 320 cells, FP32 device against the FP64 C oracle over the
-    first 1200 steps."""
+    first 1200 steps, in the default form and the blocked step."""
     p = dict(configs.config1(), nt=1200)
     print('case config 1, 1200 steps', flush=True)
     o, FG = c_oracle(p)
     s, up, lo = configs.realise(p)
     fg = materials.fg_mask(s, up, lo)
     assert np.array_equal(fg, FG)
-    dev = Solver(s, fg).run()
-    try:
-        for name, ref in o.fields().items():
-            assert np.abs(ref).max() > 0, name
-            err = rel_l2(dev.field(name), ref)
-            print(name, 'rel L2 %.3e' % err, flush=True)
-            assert err < 1e-4, (name, err)
-        w = window('3d', o.c)
-        od = o.dft()
-        for key, z in dev.dft().items():
-            err = rel_l2(z[w], od[key][w])
-            print('dft', key, 'rel L2 %.3e' % err, flush=True)
-            assert err < 1e-4, (key, err)
-    finally:
-        dev.close()
+    w = window('3d', o.c)
+    od = o.dft()
+    for launch in (None, (4, 2, 0)):
+        print('launch', launch, flush=True)
+        dev = Solver(s, fg)
+        if launch:
+            dev.set_launch(launch[1], launch[2], launch[0])
+        dev.run()
+        try:
+            for name, ref in o.fields().items():
+                assert np.abs(ref).max() > 0, name
+                err = rel_l2(dev.field(name), ref)
+                print(name, 'rel L2 %.3e' % err, flush=True)
+                assert err < 1e-4, (launch, name, err)
+            for key, z in dev.dft().items():
+                err = rel_l2(z[w], od[key][w])
+                print('dft', key, 'rel L2 %.3e' % err, flush=True)
+                assert err < 1e-4, (launch, key, err)
+        finally:
+            dev.close()
 
 
 @pytest.mark.gpu
@@ -241,7 +246,7 @@ def test_config2_tez_batch_members_identical_alone(launch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('launch', [None, (2, 1, 0), (3, 16, 0)])
+@pytest.mark.parametrize('launch', [None, (2, 1, 0), (3, 16, 0), (4, 2, 0)])
 def test_config3_batch_members_identical_alone(launch):
     print('case config 3, 16 runs, launch', launch, flush=True)
     batch_vs_alone(configs.config3(), 96, launch)

@@ -10,8 +10,9 @@ from helpers import device_inputs, oracle_run, rel_l2, window
 
 FORMS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (1, 32), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16)]
 FUSED = [(3, 1), (3, 2), (3, 4), (3, 8)]          # fused Yee step
-FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32), (4, 2), (4, 4), (4, 8)], '3d': [(3, 16)],
-              'te': [(3, 16), (3, 32), (4, 2), (4, 4), (4, 8)]}   # 2D also the temporally blocked step (variant 4)
+FUSED_LONG = {'tm': [(3, 12), (3, 16), (3, 20), (3, 24), (3, 32), (4, 2), (4, 4), (4, 8)],
+              '3d': [(3, 16), (4, 2)],
+              'te': [(3, 16), (3, 32), (4, 2), (4, 4), (4, 8)]}   # also the temporally blocked step (variant 4)
 
 
 @pytest.mark.gpu
@@ -77,9 +78,11 @@ def test_fused_step_fp64(scheme, xc, direction, extra):
 @pytest.mark.parametrize('scheme,tb,direction,extra', [('tm', 2, 'b', {'nt': 101}), ('tm', 4, 'f', {'nt': 101}),
                                                        ('tm', 8, 'f', {}), ('tm', 8, 'b', {'nt': 99}),
                                                        ('te', 2, 'f', {'nt': 101}), ('te', 4, 'b', {}),
-                                                       ('te', 8, 'b', {'nt': 99})])
+                                                       ('te', 8, 'b', {'nt': 99}),
+                                                       ('3d', 2, 'f', {'nt': 101}), ('3d', 2, 'b', {}),
+                                                       ('3d', 2, 'f', {'source_condition': 'TM', 'nt': 99})])
 def test_blocked_step_fp64(scheme, tb, direction, extra):
-    """The temporally blocked 2D step (tb steps per launch) in FP64 equals
+    """The temporally blocked step (tb steps per launch) in FP64 equals
     the oracle to 1e-10, also when the step count is not a multiple of tb (the
     last launch takes fewer steps and the state ends in the other field set)."""
     p = configs.small(scheme, 'float64', **extra)
@@ -168,6 +171,40 @@ def test_blocked_step_matches_fused(scheme, precision, tb):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('precision,tb', [('float32', 2), ('float64', 2)])
+def test_blocked_3d_matches_march(precision, tb):
+    """The blocked 3D step applies the march's per-cell arithmetic: on an
+    uneven pair of runs in one batch (plane-segment, row and z-tile edges
+    fall at different cells; 70 planes make three segments) its fields and
+    port DFT equal the two-kernel march's to rounding: relative L2 1e-12 in
+    FP64, 2e-6 in FP32."""
+    tol = 1e-12 if precision == 'float64' else 2e-6
+    runs = []
+    for nz, nx, ny, nt, seed in ((56, 36, 34, 101, 5), (90, 70, 40, 96, 6)):
+        p = configs.strip3d(nz, nx=nx, ny=ny, nt=nt, nf=6, precision=precision, width=12, height=8, port_len=10,
+                            buffer_z=6, source_type=1, gauss_pulse_deg=-1.0)
+        p.update(rough_std=15e-9, rough_acl=40e-9, seed=seed)
+        s, up, lo, fg = device_inputs(p)
+        runs.append((s, fg))
+    out = {}
+    for form in ((1, 4), (4, tb)):
+        print('case blocked 3d vs march', precision, 'form', form, flush=True)
+        b = Batch('3d', np.float32 if precision == 'float32' else np.float64, max_slots=2)
+        b.set_launch(form[1], 0, form[0])
+        slots = [b.add(s, fg) for s, fg in runs]
+        b.advance(-(-max(s.nt for s, _ in runs) // 32))
+        b.sync()
+        names = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
+        out[form] = [({n: b.field(sl, n) for n in names}, b.dft(sl)) for sl in slots]
+        b.close()
+    for (fa, da), (fb, dbb) in zip(out[(1, 4)], out[(4, tb)]):
+        errs = {n: rel_l2(fb[n], fa[n]) for n in fa}
+        errs.update({k: rel_l2(dbb[k], da[k]) for k in da})
+        print({k: '%.2e' % e for k, e in errs.items()}, flush=True)
+        assert max(errs.values()) < tol, errs
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize('precision,variant,xc,direction,extra', [
     ('float64', 2, 1, 'f', {}),                                # runs [64,128) and [128,192) outside the slabs
     ('float64', 2, 2, 'b', {'source_condition': 'TM'}),
@@ -182,14 +219,19 @@ def test_blocked_step_matches_fused(scheme, precision, tb):
     ('float32', 3, 2, 'f', {'p1': 62, 'p2': 125}),             # H samples at the run boundaries 63 and 126
     ('float32', 3, 16, 'b', {'nz': 300, 'p2': 270}),
     ('float64', 3, 1, 'f', {'nz': 300, 'p2': 270}),
+    ('float64', 4, 2, 'f', {'sz': 70}),                        # the blocked step: 60-cell inner z runs
+    ('float64', 4, 2, 'b', {'source_condition': 'TM', 'p1': 59, 'p2': 119}),   # H samples at 60 and 120
+    ('float32', 4, 2, 'f', {'nz': 300, 'p2': 270}),
+    ('float32', 4, 2, 'f', {'p1': 59, 'p2': 119}),
 ])
 def test_3d_long_z_runs(precision, variant, xc, direction, extra):
     """A guide long enough in z (nz = 250) for four 64-cell z runs per row:
     runs wholly outside the z-CPML slabs, runs that hold a source slab or a
     port plane (E samples plane p, H samples p and p+1, here on both sides of
     a run boundary), and the partial last run all give the oracle's fields and
-    port DFT in the two-kernel 3D forms and the fused step (whose waves own
-    63 z-cells each and compute the cell below their run)."""
+    port DFT in the two-kernel 3D forms, the fused step (whose waves own
+    63 z-cells each and compute the cell below their run) and the blocked
+    step (60 inner z-cells per tile)."""
     p = configs.strip3d(250, nx=36, ny=34, nt=200, nf=6, precision=precision, width=12, height=8, port_len=10,
                         buffer_z=6, source_type=1, gauss_pulse_deg=-1.0)
     p.update(extra)
@@ -247,7 +289,12 @@ def test_launch_form_validation():
                 d.set_launch(xc, 0, 3)
         assert d.launch()[:2] == (3, 16)
         with pytest.raises(OidtError):
-            d.set_launch(0, 0, 4)       # the blocked step is 2D only
+            d.set_launch(0, 0, 4)       # the blocked 3D step takes 2 steps per launch
+        d.set_launch(2, 0, 4)
+        assert d.launch()[:2] == (4, 2)
+        for xc in (1, 4, 8):
+            with pytest.raises(OidtError):
+                d.set_launch(xc, 0, 4)
     finally:
         d.close()
     # the blocked TMz step takes 2, 4 or 8 steps per launch
