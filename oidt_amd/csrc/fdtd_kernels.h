@@ -2999,15 +2999,15 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
 }
 
 // ------------------------------------------------- temporally blocked 3D ---
-// TB Yee steps per launch, streamed along x.  A block of W waves takes W rows
-// (y, one per wave) by 64 z-cells (one per lane) and marches them over a
+// TB = 2 Yee steps per launch, streamed along x.  A block of W waves takes W
+// rows (y, one per wave) by 64 z-cells (one per lane) and marches them over a
 // segment of planes; its own cells are the inner (W - 2 TB) rows by
-// (64 - 2 TB) z-cells of the kTb3Seg planes from x0.  Step s of the launch
-// runs s planes behind step 0: at position p, step 0 updates plane p - 1
-// (its E reads H of plane p, loaded one position ahead) and step s plane
-// p - 1 - s (its E reads H of plane p - s, which step s - 1 has just
-// produced; its H reads E of plane p - 2 - s, which it produced itself one
-// position earlier).  Every field and psi value crosses HBM once per launch
+// (64 - 2 TB) z-cells of the kTb3Seg planes from x0.  Step 1 runs one plane
+// behind step 0: at position p, step 0 updates plane p - 2 (its E reads H of
+// plane p - 1, loaded one position earlier) and step 1 plane p - 3 (its E
+// reads H of plane p - 2, which step 0 has just produced; its H reads E of
+// plane p - 4, which it produced itself one position earlier), while plane
+// p loads.  Every field and psi value crosses HBM once per launch
 // (with the TB ghost planes at both ends of the segment and the TB-cell
 // ghost frame of rows and z-cells), instead of twice per step.  Neighbours:
 // z by DPP lane shifts, y through LDS (old Hx and Hz of the row above for
@@ -3146,20 +3146,13 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       const auto* const q = reload_slot(sp);
       const auto* const fa = odd ? q->f2 : q->f;
       const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
+      c.hy = ldg<T>(fa[F_HY], bo);   // first: the step one plane behind needs them a position earlier
+      c.hz = ldg<T>(fa[F_HZ], bo);
       c.ex = ldg<T>(fa[F_EX], bo);
       c.ey = ldg<T>(fa[F_EY], bo);
       c.ez = ldg<T>(fa[F_EZ], bo);
       c.hx = ldg<T>(fa[F_HX], bo);
-      c.hy = ldg<T>(fa[F_HY], bo);
-      c.hz = ldg<T>(fa[F_HZ], bo);
       c.m = ldg<uint8_t>(q->mat, idx);
-    };
-    auto load_h = [&](int i, T& hy, T& hz) {
-      const auto* const q = reload_slot(sp);
-      const auto* const fa = odd ? q->f2 : q->f;
-      const unsigned bo = ((unsigned)i * pl + jk) * es;
-      hy = ldg<T>(fa[F_HY], bo);
-      hz = ldg<T>(fa[F_HZ], bo);
     };
     auto load_q = [&](int i, unsigned lf, Cell3<T>& c) {
       const auto* const q = reload_slot(sp);
@@ -3363,21 +3356,21 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       }
     };
 
-    // ---- the march.  At position p step 0 updates plane p - 1 and step 1
-    // plane p - 2, while plane p (fields and psi) and the old Hy, Hz of
-    // plane p + 1 load.  Three cell buffers take the roles in turn (the loop
-    // is unrolled by three, so no buffer is copied, and no load in flight
-    // waits).
+    // ---- the march.  At position p step 0 updates plane p - 2 and step 1
+    // plane p - 3, while plane p loads: each plane's cells load two positions
+    // before step 0 updates them (its Hy, Hz one position before, for step
+    // 0's update of the plane below).  Four cell buffers take the roles in
+    // turn (the loop is unrolled by four, so no buffer is copied, and no load
+    // in flight waits).
     T pey[TB], pez[TB];
 #pragma unroll
     for (int st = 0; st < TB; ++st) pey[st] = pez[st] = T(0);
-    auto position = [&](int p, Cell3<T>& ld, Cell3<T>& c0, Cell3<T>& c1, T& hyl, T& hzl, T hyp, T hzp) {
+    auto position = [&](int p, Cell3<T>& ld, const Cell3<T>& n1, Cell3<T>& c0, Cell3<T>& c1) {
       const unsigned lf = opaque_vgpr(lflags);
       if (p <= xl) {
         load_f(p, ld);
         load_q(p, lf, ld);
       }
-      if (p + 1 <= xl) load_h(p + 1, hyl, hzl);
       // old Hx, Hz of both steps' planes for the wave below
       xh[0][0][w][lane] = c0.hx;
       xh[0][1][w][lane] = c0.hz;
@@ -3385,26 +3378,26 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       xh[1][1][w][lane] = c1.hz;
       lds_barrier();
       // uniform over the block
-      if (p - 1 >= xf && p - 1 <= xl) {
+      if (p - 2 >= xf && p - 2 <= xl) {
         const T hxu = w + 1 < W ? xh[0][0][w + 1][lane] : c0.hx;
         const T hzu = w + 1 < W ? xh[0][1][w + 1][lane] : c0.hz;
-        step(c0, p - 1, 0, lf, hyp, hzp, hxu, hzu, pey[0], pez[0]);
+        step(c0, p - 2, 0, lf, n1.hy, n1.hz, hxu, hzu, pey[0], pez[0]);
       }
-      if (nsteps > 1 && p - 2 >= xf && p - 2 <= xl) {
+      if (nsteps > 1 && p - 3 >= xf && p - 3 <= xl) {
         const T hxu = w + 1 < W ? xh[1][0][w + 1][lane] : c1.hx;
         const T hzu = w + 1 < W ? xh[1][1][w + 1][lane] : c1.hz;
-        step(c1, p - 2, 1, lf, c0.hy, c0.hz, hxu, hzu, pey[1], pez[1]);
+        step(c1, p - 3, 1, lf, c0.hy, c0.hz, hxu, hzu, pey[1], pez[1]);
       }
-      // plane p - 2 is final
-      if (p - 2 >= x0 && p - 2 < xo1 && (lf & L3_OWN)) store(c1, p - 2, lf);
+      // plane p - 3 is final
+      if (p - 3 >= x0 && p - 3 < xo1 && (lf & L3_OWN)) store(c1, p - 3, lf);
     };
-    Cell3<T> b0{}, b1{}, b2{};
-    T hy0 = T(0), hz0 = T(0), hy1 = T(0), hz1 = T(0), hy2 = T(0), hz2 = T(0);
-    const int pend = xo1 - 1 + TB;
-    for (int p = xf; p <= pend; p += 3) {
-      position(p, b0, b2, b1, hy0, hz0, hy2, hz2);
-      if (p + 1 <= pend) position(p + 1, b1, b0, b2, hy1, hz1, hy0, hz0);
-      if (p + 2 <= pend) position(p + 2, b2, b1, b0, hy2, hz2, hy1, hz1);
+    Cell3<T> b0{}, b1{}, b2{}, b3{};
+    const int pend = xo1 + TB;
+    for (int p = xf; p <= pend; p += 4) {
+      position(p, b0, b3, b2, b1);
+      if (p + 1 <= pend) position(p + 1, b1, b0, b3, b2);
+      if (p + 2 <= pend) position(p + 2, b2, b1, b0, b3);
+      if (p + 3 <= pend) position(p + 3, b3, b2, b1, b0);
     }
     lds_barrier();   // the next tile reuses the exchange planes
   }
