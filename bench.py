@@ -648,7 +648,7 @@ def main():
                          'wi_te, wi_3d')
     ap.add_argument('--xchunk', type=int, default=0, help='0: library default for the scheme')
     ap.add_argument('--variant', type=int, default=0,
-                    help='kernel form: 1 march, 2 batched loads, 3 fused step, 4 blocked TMz step (xchunk steps per launch); '
+                    help='kernel form: 1 march, 2 batched loads, 3 fused step, 4 blocked step (xchunk steps per launch); '
                          '0: autotuner picks form and xchunk')
     ap.add_argument('--grid', type=int, default=0, help='persistent grid size cap of the stencil launches (0: 8192)')
     ap.add_argument('--profile-chunks', type=int, default=4)
