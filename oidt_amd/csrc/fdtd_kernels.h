@@ -3142,21 +3142,67 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
     // Plane i's fields and material byte, and its psi values, from set A
     // (rows and z-cells outside the grid load a clamped neighbour: no cell of
     // the grid that a step updates reads them).
+    // The run's arrays as 32-bit byte offsets from its first array (field Ex
+    // of set A; the allocation is below 4 GiB), one per lane of otab: 0-5
+    // fields read (set A), 6-11 fields written (set B), 12 material, 13
+    // capture ring, 14-25 psi read, 26-37 psi written (x slabs Ey Ez Hy Hz,
+    // y slabs of this row's side Ex Ez Hx Hz, z slabs Ex Ey Hx Hy).  A
+    // readlane turns one into a scalar operand where it is used: no array
+    // pointer is held in (or reloaded into) scalar registers over the march.
+    const char* const rb = reinterpret_cast<const char*>(s.f[F_EX]);
+    unsigned otab = 0u;
+    {
+      OIDT_GLOBAL T* const* const fa = odd ? s.f2 : s.f;
+      OIDT_GLOBAL T* const* const fb = odd ? s.f : s.f2;
+      const void* ptr = nullptr;
+      if (lane < 6) {
+        ptr = fa[lane];
+      } else if (lane < 12) {
+        ptr = fb[lane - 6];
+      } else if (lane == 12) {
+        ptr = s.mat;
+      } else if (lane == 13) {
+        ptr = s.ring;
+      } else if (lane < 38) {
+        const int t = (lane - 14) % 12;
+        OIDT_GLOBAL T* const* const pp = (lane < 26) == odd ? s.psi2 : s.psi;
+        const int ys = max(sj, 0), ysh = max(sjh, 0);
+        const int id = t == 0 ? psi_id(F_EY, 0, 0) : t == 1 ? psi_id(F_EZ, 0, 0) : t == 2 ? psi_id(F_HY, 0, 0)
+                     : t == 3 ? psi_id(F_HZ, 0, 0) : t == 4 ? psi_id(F_EX, 1, ys) : t == 5 ? psi_id(F_EZ, 1, ys)
+                     : t == 6 ? psi_id(F_HX, 1, ysh) : t == 7 ? psi_id(F_HZ, 1, ysh) : t == 8 ? psi_id(F_EX, 2, 0)
+                     : t == 9 ? psi_id(F_EY, 2, 0) : t == 10 ? psi_id(F_HX, 2, 0) : psi_id(F_HY, 2, 0);
+        ptr = pp[id];
+      }
+      if (ptr) otab = (unsigned)(reinterpret_cast<const char*>(ptr) - rb);
+      // 40-51: the cb values (component c, index q at 40 + 4c + q), FP32
+      // bits (FP64 runs keep them in scalar registers); 52-59: source and
+      // capture constants
+      if (sizeof(T) == 4 && lane >= 40 && lane < 52) otab = __float_as_uint((float)s.cbv[(lane - 40) >> 2][(lane - 40) & 3]);
+      if (lane == 52) otab = (unsigned)s.sx0;
+      if (lane == 53) otab = (unsigned)s.sx1;
+      if (lane == 54) otab = (unsigned)s.wx0;
+      if (lane == 55) otab = (unsigned)s.wx1;
+      if (lane == 56) otab = (unsigned)s.wy0;
+      if (lane == 57) otab = (unsigned)s.pcells;
+      if (lane == 58) otab = (unsigned)s.prow;
+      if (lane == 59) otab = (unsigned)s.nplanes;
+    }
+    unsigned ot = otab;   // the copy the accesses read (made opaque once per position)
+    auto O = [&](int c) { return (unsigned)__builtin_amdgcn_readlane((int)ot, c); };
+    const T* const rbT = reinterpret_cast<const T*>(rb);
+    T* const rbW = const_cast<T*>(rbT);
+
     auto load_f = [&](int i, Cell3<T>& c) {
-      const auto* const q = reload_slot(sp);
-      const auto* const fa = odd ? q->f2 : q->f;
       const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
-      c.hy = ldg<T>(fa[F_HY], bo);   // first: the step one plane behind needs them a position earlier
-      c.hz = ldg<T>(fa[F_HZ], bo);
-      c.ex = ldg<T>(fa[F_EX], bo);
-      c.ey = ldg<T>(fa[F_EY], bo);
-      c.ez = ldg<T>(fa[F_EZ], bo);
-      c.hx = ldg<T>(fa[F_HX], bo);
-      c.m = ldg<uint8_t>(q->mat, idx);
+      c.hy = ldg<T>(rbT, O(F_HY) + bo);   // first: the step one plane behind needs them a position earlier
+      c.hz = ldg<T>(rbT, O(F_HZ) + bo);
+      c.ex = ldg<T>(rbT, O(F_EX) + bo);
+      c.ey = ldg<T>(rbT, O(F_EY) + bo);
+      c.ez = ldg<T>(rbT, O(F_EZ) + bo);
+      c.hx = ldg<T>(rbT, O(F_HX) + bo);
+      c.m = ldg<uint8_t>(reinterpret_cast<const uint8_t*>(rb), O(12) + idx);
     };
     auto load_q = [&](int i, unsigned lf, Cell3<T>& c) {
-      const auto* const q = reload_slot(sp);
-      const auto* const psiA = odd ? q->psi2 : q->psi;
       int di = 0, ci = 0, dih = 0, cih = 0;
       const int si = slab_e(i, ib, ie, N, di, ci);
       const int sih = slab_h(i, ib, ie, N, dih, cih);
@@ -3168,27 +3214,38 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       const unsigned zrow = ((unsigned)i * ny + jc) * N * es;
       const unsigned iz = (lf & L3_SK) ? zrow + izk : 0u;
       const unsigned izh = (lf & L3_SKH) ? zrow + izkh : 0u;
-      const int ys = max(sj, 0), ysh = max(sjh, 0);
-      c.qyx = ldg<T>(psiA[psi_id(F_EY, 0, 0)], ix);
-      c.qzx = ldg<T>(psiA[psi_id(F_EZ, 0, 0)], ix);
-      c.ryx = ldg<T>(psiA[psi_id(F_HY, 0, 0)], ixh);
-      c.rzx = ldg<T>(psiA[psi_id(F_HZ, 0, 0)], ixh);
-      c.qxy = ldg<T>(psiA[psi_id(F_EX, 1, ys)], iy);
-      c.qzy = ldg<T>(psiA[psi_id(F_EZ, 1, ys)], iy);
-      c.rxy = ldg<T>(psiA[psi_id(F_HX, 1, ysh)], iyh);
-      c.rzy = ldg<T>(psiA[psi_id(F_HZ, 1, ysh)], iyh);
-      c.qxz = ldg<T>(psiA[psi_id(F_EX, 2, 0)], iz);
-      c.qyz = ldg<T>(psiA[psi_id(F_EY, 2, 0)], iz);
-      c.rxz = ldg<T>(psiA[psi_id(F_HX, 2, 0)], izh);
-      c.ryz = ldg<T>(psiA[psi_id(F_HY, 2, 0)], izh);
+      c.qyx = ldg<T>(rbT, O(14) + ix);
+      c.qzx = ldg<T>(rbT, O(15) + ix);
+      c.ryx = ldg<T>(rbT, O(16) + ixh);
+      c.rzx = ldg<T>(rbT, O(17) + ixh);
+      c.qxy = ldg<T>(rbT, O(18) + iy);
+      c.qzy = ldg<T>(rbT, O(19) + iy);
+      c.rxy = ldg<T>(rbT, O(20) + iyh);
+      c.rzy = ldg<T>(rbT, O(21) + iyh);
+      c.qxz = ldg<T>(rbT, O(22) + iz);
+      c.qyz = ldg<T>(rbT, O(23) + iz);
+      c.rxz = ldg<T>(rbT, O(24) + izh);
+      c.ryz = ldg<T>(rbT, O(25) + izh);
     };
 
     // One Yee step of cell c (plane i, step n0 + st): E, then H.  hyn, hzn:
     // old Hy, Hz of plane i + 1; hxu, hzu: old Hx, Hz of row j + 1; pey,
     // pez: new Ey, Ez of plane i - 1 (replaced by this plane's).
+    const T db = s.db;
+    // cb of component c for cell material byte m: FP32 from the lane table
+    // (a cross-lane read, no scalar registers), FP64 from scalar registers
+    const T cbd[3][4] = {{s.cbv[0][0], s.cbv[0][1], s.cbv[0][2], s.cbv[0][3]},
+                         {s.cbv[1][0], s.cbv[1][1], s.cbv[1][2], s.cbv[1][3]},
+                         {s.cbv[2][0], s.cbv[2][1], s.cbv[2][2], s.cbv[2][3]}};
+    auto cbc = [&](unsigned m, int c) -> T {
+      if constexpr (sizeof(T) == 4) {
+        const int q = (m >> (2 * c)) & 3;
+        return __int_as_float(__builtin_amdgcn_ds_bpermute((40 + 4 * c + q) * 4, (int)ot));
+      } else {
+        return cb_of(cbd[c][0], cbd[c][1], cbd[c][2], cbd[c][3], m, c);
+      }
+    };
     auto step = [&](Cell3<T>& c, int i, int st, unsigned lf, T hyn, T hzn, T hxu, T hzu, T& pey, T& pez) {
-      const auto* const q = reload_slot(sp);
-      const T db = q->db;
       const int u = i - xf;
       const T ikx = lane_get(lx_ik, u), bex = lane_get(lx_be, u), cex = lane_get(lx_ce, u);
       const T ikxh = lane_get(lx_ikh, u), bhx = lane_get(lx_bh, u), chx = lane_get(lx_ch, u);
@@ -3201,7 +3258,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       const T hyk = lane_down1(c.hy, c.hy);
       T vx = c.ex, vy = c.ey, vz = c.ez;
       if (iin1 && jin) {
-        const T cb = cb_of(q->cbv[0][0], q->cbv[0][1], q->cbv[0][2], q->cbv[0][3], c.m, 0);
+        const T cb = cbc(c.m, 0);
         const T dHz = hzu - c.hz;
         const T dHy = hyk - c.hy;
         T v = vx + cb * (dHz * iky - dHy * ikz);
@@ -3209,10 +3266,11 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
         if (zs) { const T qv = bek * c.qxz + cek * dHy; c.qxz = qv; v -= cb * qv; }
         vx = lkin ? v : vx;
       }
-      const bool srcp = tsrc && i >= q->sx0 && i < q->sx1 && (lf & L3_SRCE) != 0u;
+      const bool srcx = tsrc && i >= (int)O(52) && i < (int)O(53);
+      const bool srcp = srcx && (lf & L3_SRCE) != 0u;
       const T jv = tsrc ? lane_get(lk_jv, st) : T(0);
       if (iin && jin1) {
-        const T cb = cb_of(q->cbv[1][0], q->cbv[1][1], q->cbv[1][2], q->cbv[1][3], c.m, 1);
+        const T cb = cbc(c.m, 1);
         const T dHx = hxk - c.hx;
         const T dHz = hzn - c.hz;
         T v = vy + cb * (dHx * ikz - dHz * ikx);
@@ -3224,7 +3282,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
         vy = srcp ? vy - jv : vy;
       }
       if (iin && jin) {
-        const T cb = cb_of(q->cbv[2][0], q->cbv[2][1], q->cbv[2][2], q->cbv[2][3], c.m, 2);
+        const T cb = cbc(c.m, 2);
         const T dHy = hyn - c.hy;
         const T dHx = hxu - c.hx;
         T v = vz + cb * (dHy * ikx - dHx * iky);
@@ -3235,20 +3293,15 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       c.ex = vx;
       c.ey = vy;
       c.ez = vz;
-      const bool capx = tcap && i >= x0 && i < xo1 && i >= q->wx0 && i <= q->wx1;
+      const bool capx = tcap && i >= x0 && i < xo1 && i >= (int)O(54) && i <= (int)O(55);
+      auto put = [&](T v, unsigned plane) {
+        const unsigned rbase = (unsigned)(tstep + st) * O(59);
+        const unsigned cw = (unsigned)(i - (int)O(54)) * O(58) + (unsigned)(j - (int)O(56));
+        stg<T>(rbW, O(13) + ((rbase + plane) * O(57) + cw) * es, v);
+      };
       if (capx) {
-        T* __restrict__ ring = q->ring;
-        const size_t pc = q->pcells;
-        const size_t rbase = (size_t)(tstep + st) * q->nplanes;
-        const size_t cw = (size_t)((i - q->wx0) * q->prow + (j - q->wy0));
-        if (lf & L3_CAPE0) {
-          ring[(rbase + 0) * pc + cw] = vx;
-          ring[(rbase + 1) * pc + cw] = vy;
-        }
-        if (lf & L3_CAPE1) {
-          ring[(rbase + 6) * pc + cw] = vx;
-          ring[(rbase + 7) * pc + cw] = vy;
-        }
+        if (lf & L3_CAPE0) { put(vx, 0); put(vy, 1); }
+        if (lf & L3_CAPE1) { put(vx, 6); put(vy, 7); }
       }
       // ---- new Ex, Ez of row j - 1 from the wave below
       T* const xa = &xe[st & 1][0][0][0];
@@ -3269,7 +3322,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
         if (zs) { const T qv = bhk * c.rxz + chk * dEy; c.rxz = qv; v += db * qv; }
         hx = lkin1 ? v : hx;
       }
-      const bool srch = tsrc && i >= q->sx0 && i < q->sx1 && (lf & L3_SRCH) != 0u;
+      const bool srch = srcx && (lf & L3_SRCH) != 0u;
       T hy = srch ? c.hy - jv : c.hy;
       if (iin1 && jin) {
         const T dEz = vz - pez;
@@ -3294,30 +3347,23 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       pey = vy;
       pez = vz;
       if (capx) {
-        T* __restrict__ ring = q->ring;
-        const size_t pc = q->pcells;
-        const size_t rbase = (size_t)(tstep + st) * q->nplanes;
-        const size_t cw = (size_t)((i - q->wx0) * q->prow + (j - q->wy0));
-        if (lf & L3_CAPA) { ring[(rbase + 2) * pc + cw] = hx; ring[(rbase + 4) * pc + cw] = hy; }
-        if (lf & L3_CAPB) { ring[(rbase + 3) * pc + cw] = hx; ring[(rbase + 5) * pc + cw] = hy; }
-        if (lf & L3_CAPC) { ring[(rbase + 8) * pc + cw] = hx; ring[(rbase + 10) * pc + cw] = hy; }
-        if (lf & L3_CAPD) { ring[(rbase + 9) * pc + cw] = hx; ring[(rbase + 11) * pc + cw] = hy; }
+        if (lf & L3_CAPA) { put(hx, 2); put(hy, 4); }
+        if (lf & L3_CAPB) { put(hx, 3); put(hy, 5); }
+        if (lf & L3_CAPC) { put(hx, 8); put(hy, 10); }
+        if (lf & L3_CAPD) { put(hx, 9); put(hy, 11); }
       }
     };
 
     // plane i's inner cells to set B (fields unconditionally: cells no step
     // updates hold the same value in both sets; psi where the steps update it)
     auto store = [&](const Cell3<T>& c, int i, unsigned lf) {
-      const auto* const q = reload_slot(sp);
-      const auto* const fb = odd ? q->f : q->f2;
-      const auto* const psiB = odd ? q->psi : q->psi2;
       const unsigned idx = (unsigned)i * pl + jk, bo = idx * es;
-      stg_nt<T>(fb[F_EX], bo, c.ex);
-      stg_nt<T>(fb[F_EY], bo, c.ey);
-      stg_nt<T>(fb[F_EZ], bo, c.ez);
-      stg_nt<T>(fb[F_HX], bo, c.hx);
-      stg_nt<T>(fb[F_HY], bo, c.hy);
-      stg_nt<T>(fb[F_HZ], bo, c.hz);
+      stg_nt<T>(rbW, O(6 + F_EX) + bo, c.ex);
+      stg_nt<T>(rbW, O(6 + F_EY) + bo, c.ey);
+      stg_nt<T>(rbW, O(6 + F_EZ) + bo, c.ez);
+      stg_nt<T>(rbW, O(6 + F_HX) + bo, c.hx);
+      stg_nt<T>(rbW, O(6 + F_HY) + bo, c.hy);
+      stg_nt<T>(rbW, O(6 + F_HZ) + bo, c.hz);
       const bool iin = i >= ib && i < ie, iin1 = i > ib && i < ie;
       const bool lkin = (lf & L3_KIN) != 0u, lkin1 = (lf & L3_KIN1) != 0u;
       int di = 0, ci = 0, dih = 0, cih = 0;
@@ -3325,33 +3371,33 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       const int sih = slab_h(i, ib, ie, N, dih, cih);
       if (si >= 0) {
         const unsigned ix = ((unsigned)di * pl + jk) * es + (si == 1 ? xhi : 0u);
-        if (iin && jin1 && lkin) stg<T>(psiB[psi_id(F_EY, 0, 0)], ix, c.qyx);
-        if (iin && jin && lkin1) stg<T>(psiB[psi_id(F_EZ, 0, 0)], ix, c.qzx);
+        if (iin && jin1 && lkin) stg<T>(rbW, O(26) + ix, c.qyx);
+        if (iin && jin && lkin1) stg<T>(rbW, O(27) + ix, c.qzx);
       }
       if (sih >= 0) {
         const unsigned ixh = ((unsigned)dih * pl + jk) * es + (sih == 1 ? xhi : 0u);
-        if (iin1 && jin && lkin1) stg<T>(psiB[psi_id(F_HY, 0, 0)], ixh, c.ryx);
-        if (iin1 && jin1 && lkin) stg<T>(psiB[psi_id(F_HZ, 0, 0)], ixh, c.rzx);
+        if (iin1 && jin && lkin1) stg<T>(rbW, O(28) + ixh, c.ryx);
+        if (iin1 && jin1 && lkin) stg<T>(rbW, O(29) + ixh, c.rzx);
       }
       if (sj >= 0) {
         const unsigned iy = (((unsigned)i * N + dj) * nz + kc) * es;
-        if (iin1 && jin && lkin) stg<T>(psiB[psi_id(F_EX, 1, sj)], iy, c.qxy);
-        if (iin && jin && lkin1) stg<T>(psiB[psi_id(F_EZ, 1, sj)], iy, c.qzy);
+        if (iin1 && jin && lkin) stg<T>(rbW, O(30) + iy, c.qxy);
+        if (iin && jin && lkin1) stg<T>(rbW, O(31) + iy, c.qzy);
       }
       if (sjh >= 0) {
         const unsigned iyh = (((unsigned)i * N + djh) * nz + kc) * es;
-        if (iin && jin1 && lkin1) stg<T>(psiB[psi_id(F_HX, 1, sjh)], iyh, c.rxy);
-        if (iin1 && jin1 && lkin) stg<T>(psiB[psi_id(F_HZ, 1, sjh)], iyh, c.rzy);
+        if (iin && jin1 && lkin1) stg<T>(rbW, O(32) + iyh, c.rxy);
+        if (iin1 && jin1 && lkin) stg<T>(rbW, O(33) + iyh, c.rzy);
       }
       if (zs) {
         const unsigned zrow = ((unsigned)i * ny + jc) * N * es;
         if (lf & L3_SK) {
-          if (iin1 && jin) stg<T>(psiB[psi_id(F_EX, 2, 0)], zrow + izk, c.qxz);
-          if (iin && jin1) stg<T>(psiB[psi_id(F_EY, 2, 0)], zrow + izk, c.qyz);
+          if (iin1 && jin) stg<T>(rbW, O(34) + zrow + izk, c.qxz);
+          if (iin && jin1) stg<T>(rbW, O(35) + zrow + izk, c.qyz);
         }
         if (lf & L3_SKH) {
-          if (iin && jin1) stg<T>(psiB[psi_id(F_HX, 2, 0)], zrow + izkh, c.rxz);
-          if (iin1 && jin) stg<T>(psiB[psi_id(F_HY, 2, 0)], zrow + izkh, c.ryz);
+          if (iin && jin1) stg<T>(rbW, O(36) + zrow + izkh, c.rxz);
+          if (iin1 && jin) stg<T>(rbW, O(37) + zrow + izkh, c.ryz);
         }
       }
     };
@@ -3367,6 +3413,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
     for (int st = 0; st < TB; ++st) pey[st] = pez[st] = T(0);
     auto position = [&](int p, Cell3<T>& ld, const Cell3<T>& n1, Cell3<T>& c0, Cell3<T>& c1) {
       const unsigned lf = opaque_vgpr(lflags);
+      ot = opaque_vgpr(otab);
       if (p <= xl) {
         load_f(p, ld);
         load_q(p, lf, ld);
