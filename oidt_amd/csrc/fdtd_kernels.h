@@ -3020,19 +3020,6 @@ __global__ __launch_bounds__(kThreads, 4) void k_3d_halo(const Slot<T>* __restri
 // state after its steps to the other set.
 constexpr int kTb3Seg = 32;   // planes of a 3D blocked tile
 
-// The run descriptor through a pointer the compiler cannot see through, in
-// the constant address space: the pointers and constants read through it are
-// loaded again (scalar loads, cache hits) where they are used, instead of
-// being kept in scalar registers over the whole march, which does not have
-// enough of them.  The descriptors do not change during a stencil launch.
-template <typename T>
-using ConstSlot = const __attribute__((address_space(4))) Slot<T>;
-template <typename T>
-__device__ __forceinline__ ConstSlot<T>* reload_slot(const Slot<T>* p) {
-  asm volatile("" : "+s"(p));
-  return (ConstSlot<T>*)p;
-}
-
 // One cell's state: fields, psi and material byte.
 template <typename T>
 struct Cell3 {
@@ -3071,8 +3058,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
   const int w = wave_id();
   for (int tt = xcd_block(); tt < ntiles; tt += gridDim.x) {
     const int4 tl = tiles[tt];
-    const Slot<T>* const sp = slots + tl.x;
-    const Slot<T>& s = *sp;
+    const Slot<T>& s = slots[tl.x];
     const int n0 = s.base + tstep;
     if (!s.active || n0 >= s.nt) continue;      // uniform over the block
     const int nsteps = min(TB, s.nt - n0);
