@@ -3069,6 +3069,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
     const int j = j0 - TB + w, k = k0 - TB + lane;
     const int jc = min(max(j, 0), ny - 1), kc = min(max(k, 0), nz - 1);
     const bool own = j >= j0 && j < j0 + TY && j < ny && k >= k0 && k < k0 + TZ && k < nz;
+    const bool jgrid = j >= 0 && j < ny;
     const int xo1 = min(x0 + kTb3Seg, nx);                       // inner planes [x0, xo1)
     const int xf = max(x0 - TB, 0), xl = min(xo1 - 1 + TB, nx - 1);   // loaded planes [xf, xl]
     const unsigned pl = (unsigned)ny * nz;
@@ -3400,7 +3401,10 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
     auto position = [&](int p, Cell3<T>& ld, const Cell3<T>& n1, Cell3<T>& c0, Cell3<T>& c1) {
       const unsigned lf = opaque_vgpr(lflags);
       ot = opaque_vgpr(otab);
-      if (p <= xl) {
+      // rows outside the grid (ghost rows of the edge tiles) load nothing:
+      // no updated cell reads them (E rows end below ny - 1, H rows start
+      // above row 0), and their stale values only reach such cells
+      if (p <= xl && jgrid) {
         load_f(p, ld);
         load_q(p, lf, ld);
       }
@@ -3471,12 +3475,17 @@ __device__ __forceinline__ T sample_at(const Slot<T>& s, const T* __restrict__ r
   return T(0.25) * (b[P + c] + b[2 * P + c] + b[P + c + 1] + b[2 * P + c + 1]);
 }
 
+// The fold also ends the chunk: counts[2] counts the blocks that have folded
+// their items, and the last block to arrive advances every active run's first
+// step of the chunk in flight by kChunk (what the next chunk's stencil
+// launches read; no block reads it after it has been counted) and re-arms the
+// count.  No block waits for another.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ slots,
-                                                  const int4* __restrict__ work,
-                                                  const int* __restrict__ counts) {
+__global__ __launch_bounds__(kThreads) void k_dft(Slot<T>* slots, const int4* __restrict__ work, int* counts,
+                                                  int nslots) {
   __shared__ double cs[kFB][kChunk];
   __shared__ double sn[kFB][kChunk];
+  __shared__ int last;
   const int nwork = counts[1];
   for (int ww = xcd_block(); ww < nwork; ww += gridDim.x) {
     const int4 wk = work[ww];
@@ -3521,6 +3530,17 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
       a[0] += re;
       a[1] -= im;
     }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(&counts[2], 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last) {
+    for (int i = threadIdx.x; i < nslots; i += kThreads)
+      if (slots[i].active) slots[i].base += kChunk;
+    if (threadIdx.x == 0) atomicExch(&counts[2], 0);
   }
 }
 
@@ -3602,13 +3622,6 @@ __global__ __launch_bounds__(kThreads) void k_port_power(const double* __restric
     out[2 * f] = r;
     out[2 * f + 1] = i;
   }
-}
-
-// End-of-chunk: every resident run moves to its next chunk.
-template <typename T>
-__global__ void k_advance(Slot<T>* __restrict__ slots, int nslots) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nslots && slots[i].active) slots[i].base += kChunk;
 }
 
 // ------------------------------------------------ 2D TMz, lossy study form ---
