@@ -269,7 +269,8 @@ def bench_study_batch(args, scheme, study, warm, K):
 
 def study_cpu_baseline(scheme, study, seconds):
     """The FP64 study oracle (oracle/studies.py, one thread) on the study's own
-    grid, whole steps (recording included) for about ``seconds``."""
+    grid, whole steps (recording included) for about ``seconds``, at most the
+    study's own step count."""
     from oidt_amd import variants
     from oracle import studies as ost
     c = variants.config(scheme, study)
@@ -277,7 +278,8 @@ def study_cpu_baseline(scheme, study, seconds):
     o.step(0)
     o.record(0)
     n, t0 = 1, time.perf_counter()
-    while True:
+    dt = 0.0
+    while n < c.nt:
         o.step(n)
         o.record(n)
         n += 1
