@@ -3220,7 +3220,8 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
     // pez: new Ey, Ez of plane i - 1 (replaced by this plane's).
     const T db = s.db;
     // cb of component c for cell material byte m: FP32 from the lane table
-    // (a cross-lane read, no scalar registers), FP64 from scalar registers
+    // (a cross-lane read, no scalar registers; called only under wave-uniform
+    // branches, so the table's lanes are active), FP64 from scalar registers
     const T cbd[3][4] = {{s.cbv[0][0], s.cbv[0][1], s.cbv[0][2], s.cbv[0][3]},
                          {s.cbv[1][0], s.cbv[1][1], s.cbv[1][2], s.cbv[1][3]},
                          {s.cbv[2][0], s.cbv[2][1], s.cbv[2][2], s.cbv[2][3]}};
