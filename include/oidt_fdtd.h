@@ -304,8 +304,7 @@ int oidt_batch_destroy(oidt_batch* b);
 /* Enqueue nchunks chunks as individual launches with HIP event pairs on the
  * batch stream, wait, and return the mean duration (ms) per launch of
  * out_ms4[0] E half-step (the whole step in the fused and blocked forms),
- * [1] H half-step, [2] DFT fold (which also moves the runs to the next
- * chunk), [3] 0 (no separate advance launch).  Two-kernel forms get an
+ * [1] H half-step, [2] DFT fold, [3] chunk advance.  Two-kernel forms get an
  * event pair around every kernel; the fused and blocked forms one pair around
  * each chunk's back-to-back stencil launches. */
 int oidt_batch_profile(oidt_batch* b, int32_t nchunks, double* out_ms4);

@@ -292,7 +292,7 @@ struct oidt_batch {
   int device = 0;              // device current at creation; the batch's stream and memory live there
   hipStream_t stream = nullptr;
   void* d_slots = nullptr;
-  int* d_counts = nullptr;  // [0] stencil tiles, [1] DFT work items, [2] DFT blocks done (0 between launches)
+  int* d_counts = nullptr;  // [0] stencil tiles, [1] DFT work items
   int4* d_tiles = nullptr;
   size_t tiles_cap = 0;
   int4* d_work = nullptr;
@@ -658,7 +658,7 @@ int rebuild_lists(oidt_batch* b) {
   if ((rc = grow(b, &b->d_work, &b->work_cap, std::max<size_t>(work.size(), 1))) != OIDT_OK) return rc;
   if (!tiles.empty() && (rc = upload_bytes(b, b->d_tiles, tiles.data(), tiles.size() * sizeof(int4))) != OIDT_OK) return rc;
   if (!work.empty() && (rc = upload_bytes(b, b->d_work, work.data(), work.size() * sizeof(int4))) != OIDT_OK) return rc;
-  int counts[3] = {(int)tiles.size(), (int)work.size(), 0};
+  int counts[2] = {(int)tiles.size(), (int)work.size()};
   if ((rc = upload_bytes(b, b->d_counts, counts, sizeof(counts))) != OIDT_OK) return rc;
   b->ntiles = counts[0];
   b->nwork = counts[1];
@@ -799,12 +799,16 @@ void launch_stencil(oidt_batch* b, int gs, int tstep, bool e_half) {
   }
 }
 
-// The DFT fold of the chunk in flight; its last block also moves every run
-// to its next chunk.
 template <typename T>
 void launch_dft(oidt_batch* b, int gd) {
-  hipLaunchKernelGGL((k_dft<T>), dim3(gd), dim3(kThreads), 0, b->stream, static_cast<Slot<T>*>(b->d_slots), b->d_work,
-                     b->d_counts, b->max_slots);
+  hipLaunchKernelGGL((k_dft<T>), dim3(gd), dim3(kThreads), 0, b->stream,
+                     static_cast<const Slot<T>*>(b->d_slots), b->d_work, b->d_counts);
+}
+
+template <typename T>
+void launch_advance(oidt_batch* b) {
+  hipLaunchKernelGGL((k_advance<T>), dim3((b->max_slots + 255) / 256), dim3(256), 0, b->stream,
+                     static_cast<Slot<T>*>(b->d_slots), b->max_slots);
 }
 
 // Grid of the stencil launches.  The marching step gives every wave exactly
@@ -836,6 +840,7 @@ int enqueue_chunk(oidt_batch* b) {
     launch_monitors<T>(b, t, 1);
   }
   launch_dft<T>(b, gd);
+  launch_advance<T>(b);
   return hipGetLastError() == hipSuccess ? OIDT_OK : fail(OIDT_EHIP, "kernel launch failed");
 }
 
@@ -843,9 +848,7 @@ int enqueue_chunk(oidt_batch* b) {
 template <typename T>
 int profile_chunks(oidt_batch* b, int nchunks, double* out) {
   const int gs = stencil_grid(b), gd = dft_grid(b);
-  // kinds: 0 E half-step, 1 H half-step, 2 DFT fold (which also advances the
-  // chunk; kind 3 has no launch of its own and reads 0), 4 / 5 monitors
-  // after E / H.
+  // kinds: 0 E half-step, 1 H half-step, 2 DFT fold, 3 advance, 4 / 5 monitors after E / H.
   // The one-launch forms (fused, blocked) have a single stencil kernel per
   // step: the chunk's stencil launches are timed back to back between one
   // pair of events (per-launch events would add their own gaps to launches
@@ -878,6 +881,7 @@ int profile_chunks(oidt_batch* b, int nchunks, double* out) {
         }
     }
     if ((rc = timed(2, 1, [&] { launch_dft<T>(b, gd); })) != OIDT_OK) return rc;
+    if ((rc = timed(3, 1, [&] { launch_advance<T>(b); })) != OIDT_OK) return rc;
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -1038,7 +1042,7 @@ int oidt_batch_create(int32_t scheme, int32_t precision, int32_t max_slots, oidt
   if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "hipStreamCreate");
   if ((e = hipMalloc(&b->d_slots, (size_t)max_slots * b->slot_bytes())) != hipSuccess) return bad(e, "hipMalloc slots");
   if ((e = hipMemsetAsync(b->d_slots, 0, (size_t)max_slots * b->slot_bytes(), b->stream)) != hipSuccess) return bad(e, "hipMemset slots");
-  if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_counts), 3 * sizeof(int))) != hipSuccess) return bad(e, "hipMalloc counts");
+  if ((e = hipMalloc(reinterpret_cast<void**>(&b->d_counts), 2 * sizeof(int))) != hipSuccess) return bad(e, "hipMalloc counts");
   if ((e = hipMemsetAsync(b->d_counts, 0, 2 * sizeof(int), b->stream)) != hipSuccess) return bad(e, "hipMemset counts");
   *out = b;
   return OIDT_OK;

@@ -3476,17 +3476,12 @@ __device__ __forceinline__ T sample_at(const Slot<T>& s, const T* __restrict__ r
   return T(0.25) * (b[P + c] + b[2 * P + c] + b[P + c + 1] + b[2 * P + c + 1]);
 }
 
-// The fold also ends the chunk: counts[2] counts the blocks that have folded
-// their items, and the last block to arrive advances every active run's first
-// step of the chunk in flight by kChunk (what the next chunk's stencil
-// launches read; no block reads it after it has been counted) and re-arms the
-// count.  No block waits for another.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_dft(Slot<T>* slots, const int4* __restrict__ work, int* counts,
-                                                  int nslots) {
+__global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ slots,
+                                                  const int4* __restrict__ work,
+                                                  const int* __restrict__ counts) {
   __shared__ double cs[kFB][kChunk];
   __shared__ double sn[kFB][kChunk];
-  __shared__ int last;
   const int nwork = counts[1];
   for (int ww = xcd_block(); ww < nwork; ww += gridDim.x) {
     const int4 wk = work[ww];
@@ -3531,17 +3526,6 @@ __global__ __launch_bounds__(kThreads) void k_dft(Slot<T>* slots, const int4* __
       a[0] += re;
       a[1] -= im;
     }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(&counts[2], 1) == (int)gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last) {
-    for (int i = threadIdx.x; i < nslots; i += kThreads)
-      if (slots[i].active) slots[i].base += kChunk;
-    if (threadIdx.x == 0) atomicExch(&counts[2], 0);
   }
 }
 
@@ -3623,6 +3607,15 @@ __global__ __launch_bounds__(kThreads) void k_port_power(const double* __restric
     out[2 * f] = r;
     out[2 * f + 1] = i;
   }
+}
+
+// End-of-chunk: every resident run moves to its next chunk.  (A separate
+// launch: counting the DFT launch's blocks with one atomic each, so that its
+// last block could do this, measured slower on every batch but the smallest.)
+template <typename T>
+__global__ void k_advance(Slot<T>* __restrict__ slots, int nslots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nslots && slots[i].active) slots[i].base += kChunk;
 }
 
 // ------------------------------------------------ 2D TMz, lossy study form ---
