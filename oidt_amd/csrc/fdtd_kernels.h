@@ -3193,7 +3193,7 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       int di = 0, ci = 0, dih = 0, cih = 0;
       const int si = slab_e(i, ib, ie, N, di, ci);
       const int sih = slab_h(i, ib, ie, N, dih, cih);
-      // every lane loads; outside a slab all lanes read element 0 (one line)
+      // outside its slab a z lane reads element 0 (one line for the wave)
       const unsigned ix = si >= 0 ? ((unsigned)di * pl + jk) * es + (si == 1 ? xhi : 0u) : 0u;
       const unsigned ixh = sih >= 0 ? ((unsigned)dih * pl + jk) * es + (sih == 1 ? xhi : 0u) : 0u;
       const unsigned iy = sj >= 0 ? (((unsigned)i * N + dj) * nz + kc) * es : 0u;
@@ -3201,18 +3201,31 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
       const unsigned zrow = ((unsigned)i * ny + jc) * N * es;
       const unsigned iz = (lf & L3_SK) ? zrow + izk : 0u;
       const unsigned izh = (lf & L3_SKH) ? zrow + izkh : 0u;
-      c.qyx = ldg<T>(rbT, O(14) + ix);
-      c.qzx = ldg<T>(rbT, O(15) + ix);
-      c.ryx = ldg<T>(rbT, O(16) + ixh);
-      c.rzx = ldg<T>(rbT, O(17) + ixh);
-      c.qxy = ldg<T>(rbT, O(18) + iy);
-      c.qzy = ldg<T>(rbT, O(19) + iy);
-      c.rxy = ldg<T>(rbT, O(20) + iyh);
-      c.rzy = ldg<T>(rbT, O(21) + iyh);
-      c.qxz = ldg<T>(rbT, O(22) + iz);
-      c.qyz = ldg<T>(rbT, O(23) + iz);
-      c.rxz = ldg<T>(rbT, O(24) + izh);
-      c.ryz = ldg<T>(rbT, O(25) + izh);
+      // psi loads only where a slab can hold the cell (the plane, the row or
+      // the tile is uniform over the wave); elsewhere the values are zero
+      c.qyx = c.qzx = c.ryx = c.rzx = c.qxy = c.qzy = c.rxy = c.rzy = c.qxz = c.qyz = c.rxz = c.ryz = T(0);
+      if (si >= 0) {
+        c.qyx = ldg<T>(rbT, O(14) + ix);
+        c.qzx = ldg<T>(rbT, O(15) + ix);
+      }
+      if (sih >= 0) {
+        c.ryx = ldg<T>(rbT, O(16) + ixh);
+        c.rzx = ldg<T>(rbT, O(17) + ixh);
+      }
+      if (sj >= 0) {
+        c.qxy = ldg<T>(rbT, O(18) + iy);
+        c.qzy = ldg<T>(rbT, O(19) + iy);
+      }
+      if (sjh >= 0) {
+        c.rxy = ldg<T>(rbT, O(20) + iyh);
+        c.rzy = ldg<T>(rbT, O(21) + iyh);
+      }
+      if (zs) {
+        c.qxz = ldg<T>(rbT, O(22) + iz);
+        c.qyz = ldg<T>(rbT, O(23) + iz);
+        c.rxz = ldg<T>(rbT, O(24) + izh);
+        c.ryz = ldg<T>(rbT, O(25) + izh);
+      }
     };
 
     // One Yee step of cell c (plane i, step n0 + st): E, then H.  hyn, hzn:
