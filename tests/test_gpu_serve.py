@@ -25,15 +25,17 @@ def _requests():
 
 FUSED = {'tm': (3, 16, 0), '3d': (3, 4, 0)}
 BLOCKED = {'tm': (4, 8, 0), '3d': (1, 4, 0)}
+BLOCKED_3D = {'tm': (4, 4, 0), '3d': (4, 2, 0)}
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('budget_runs,launch', [(100, None), (2, None), (100, FUSED), (2, BLOCKED)])
+@pytest.mark.parametrize('budget_runs,launch', [(100, None), (2, None), (100, FUSED), (2, BLOCKED),
+                                                (100, BLOCKED_3D)])
 def test_results_identical_alone_and_in_mix(budget_runs, launch):
     """Also with the fused step forms, whose runs alternate between two field
-    sets and share per-wave tile lists, and the blocked TMz step (8 steps per
-    launch): a run admitted while others are half way through still gives
-    the phasors it gives alone."""
+    sets and share per-wave tile lists, and the blocked steps (TMz 8 or 4,
+    3D 2 steps per launch): a run admitted while others are half way through
+    still gives the phasors it gives alone."""
     descs = _requests()
     reqs = serve.make_requests(descs, [0.0, 0.002, 0.004, 0.006, 0.02, 0.03])
     be = serve.DeviceBackend(max_slots=8, launch=launch)
