@@ -3038,6 +3038,11 @@ enum : unsigned {
   L3_CAP = L3_CAPE0 | L3_CAPE1 | L3_CAPA | L3_CAPB | L3_CAPC | L3_CAPD
 };
 
+// A copy of a per-lane value that the compiler cannot see through.  Taken
+// once per position of the march, it keeps what the loop derives from the
+// value (per-lane flag masks, readlanes of a lane table) where it is used,
+// instead of hoisted out of the loop into scalar registers for the whole
+// march, which does not have enough of them.
 template <typename T>
 __device__ __forceinline__ T opaque_vgpr(T v) {
   asm volatile("" : "+v"(v));
