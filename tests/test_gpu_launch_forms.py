@@ -108,6 +108,35 @@ def test_blocked_step_fp64(scheme, tb, direction, extra):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('scheme,tb', [('3d', 2), ('tm', 8), ('te', 4)])
+def test_blocked_step_incident_run_fp64(scheme, tb):
+    """The protocol's incident run (smooth guide cut to the two port lengths,
+    both port planes on the same cell, fdtd_auto_setup.py:82-91): the blocked
+    step writes every sample plane of both ports from the one plane, and its
+    fields and port DFT equal the oracle's to 1e-10."""
+    inc = configs.incident(configs.small(scheme, 'float64', nt=101))
+    assert inc['p1'] == inc['p2']
+    s, up, lo, fg = device_inputs(inc)
+    o, _ = oracle_run(inc, up, lo)
+    print('case blocked incident', scheme, 'tb', tb, flush=True)
+    b = Batch(scheme, np.float64, max_slots=1)
+    b.set_launch(tb, 0, 4)
+    slot = b.add(s, fg)
+    b.advance(-(-s.nt // 32))
+    b.sync()
+    for name, ref in o.fields().items():
+        err = rel_l2(b.field(slot, name), ref)
+        assert err < 1e-10, (name, err)
+    w = window(scheme, o.c)
+    od = o.dft()
+    for key, z in b.dft(slot).items():
+        assert np.abs(od[key][w]).max() > 0, key
+        err = rel_l2(z[w], od[key][w])
+        assert err < 1e-10, (key, err)
+    b.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize('scheme,tb,direction', [('tm', 8, 'f'), ('tm', 4, 'b'), ('te', 8, 'b'), ('te', 2, 'f')])
 def test_blocked_step_interior_tiles_fp64(scheme, tb, direction):
     """A grid wide enough (200 x 160) for tiles whose extended region lies
