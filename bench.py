@@ -543,14 +543,15 @@ def roofline_of(scheme, es, variant, xchunk, cells, ms, key):
     return out
 
 
-def s21_accuracy(descs):
+def s21_accuracy(descs, launch=None):
     """Per-realisation S21 / S11 of the full-length ensemble in FP32 against
     the same ensemble in FP64 (one shared incident run each, device
-    extraction): max |dB| and max |degrees| differences over the frequencies."""
+    extraction), both in the launch form the timed leg uses: max |dB| and max
+    |degrees| differences over the frequencies."""
     from oidt_amd import ensemble
     t0 = time.perf_counter()
-    e32 = ensemble.Ensemble(descs, precision='float32').run()
-    e64 = ensemble.Ensemble(descs, precision='float64').run()
+    e32 = ensemble.Ensemble(descs, precision='float32', launch=launch).run()
+    e64 = ensemble.Ensemble(descs, precision='float64', launch=launch).run()
     rows = []
     for r in range(len(descs)):
         a, b = e32.s_params(r), e64.s_params(r)
@@ -563,7 +564,7 @@ def s21_accuracy(descs):
     e32.close()
     e64.close()
     return {'reference': 'the same ensemble in FP64 on the device (FP64 kernels pinned to the FP64 oracle at 1e-10)',
-            'steps': int(e32.nt), 'incident_runs': len(e32.incident),
+            'launch': list(launch) if launch else None, 'steps': int(e32.nt), 'incident_runs': len(e32.incident),
             'cell_updates_saved_by_shared_incident': int(saved),
             'max_s21_err_db': max(r['s21_err_db'] for r in rows),
             'max_s21_err_deg': max(r['s21_err_deg'] for r in rows),
@@ -617,7 +618,7 @@ def bench_ensemble(args, descs, label):
                                    profile_kernels(runs, launch, args.profile_chunks), args.workload)
     if args.workload in ('c3', 'c2', 'c2te') and not args.no_accuracy:
         log('%s: full-length S21 accuracy, FP32 against FP64' % args.workload)
-        line['s21_vs_fp64'] = s21_accuracy(descs)
+        line['s21_vs_fp64'] = s21_accuracy(descs, launch)
     if not args.no_cpu:
         p0 = dict(descs[0])
         lbl = 'the %s grid of the first run' % 'x'.join(str(p0[k]) for k in ('nx', 'ny', 'nz') if k in p0)
