@@ -672,7 +672,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
-        print('bench.py: this build measures one GPU only; --gpus %d is not supported '
+        print('bench.py: measures one GPU only; --gpus %d is not supported '
               '(runs are independent and shard across GPUs without communication)' % args.gpus,
               file=sys.stderr)
         sys.exit(2)

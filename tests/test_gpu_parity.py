@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) against the FP64 oracle.
 
-Tolerances (north_star): FP32 fields and DFT phasors within relative L2 1e-4
+Tolerances: FP32 fields and DFT phasors within relative L2 1e-4
 of the oracle; FP64 runs agree to 1e-10 (same algorithm, same precision; only
 the order of floating-point operations differs)."""
 import numpy as np

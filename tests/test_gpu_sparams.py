@@ -4,7 +4,7 @@ The reference protocol for one line: forward/backward incident runs on the
 shortened guide and forward/backward runs on the full rough guide
 (simulation_launcher.py:26-35), phasors extracted to S = sqrt(Pref/Pinc)
 (extract_sparams.py:52-104).  The device runs all four as one uneven batch.
-Bar (north_star): |S| within 0.01 dB and angle within 0.1 degree of the
+Bar: |S| within 0.01 dB and angle within 0.1 degree of the
 oracle for S11, S21, S12, S22 (FP32 fields, FP64 DFT and normalisation)."""
 import numpy as np
 import pytest

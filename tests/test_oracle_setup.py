@@ -15,7 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 USR = json.load(open(os.path.join(HERE, 'golden', 'reference_user_configs.json')))
 
 # Grid integers of the reference defaults (user_config.py of each body), derived
-# by hand from fdtd_auto_setup.py with numpy-1.20 promotion.
+# by hand from fdtd_auto_setup.py with numpy-1.20 promotion: there a Python float
+# divided by an np.float32 scalar is a float64 division, so int(length / dx) uses
+# the float32-rounded dx in float64 (numpy 2 would divide in float32).
 GOLDEN = {
     ('tm', 'r'): dict(nx=1480, ny=430, nt=9239, p1=506, p2=974),
     ('tm', 'i'): dict(nx=1013, ny=430, nt=9239, p1=506, p2=507),
