@@ -3467,31 +3467,53 @@ __global__ __launch_bounds__(64 * W) void k_3d_tb(const Slot<T>* __restrict__ sl
 // sample is formed from the raw planes in the run precision with the
 // reference's averaging expression.
 
+// The sample of component q at window cell c is K values of one step's
+// captured planes at fixed offsets o[] (from that step's plane 0), summed in
+// the order ((o0 + o1) + o2) + o3 and, for K > 1, scaled by 1/K; returns K.
 template <typename T>
-__device__ __forceinline__ T sample_at(const Slot<T>& s, const T* __restrict__ r, int q, int c) {
-  // r points at plane 0 of this step
+__device__ __forceinline__ int sample_taps(const Slot<T>& s, int q, int c, size_t (&o)[4]) {
   const size_t P = s.pcells;
   if (s.scheme == SCHEME_3D) {
     const int port = q >> 2, comp = q & 3;
     const int wyn = s.wy1 - s.wy0;
     const int ii = c / wyn, jj = c - ii * wyn;
-    const int rc = ii * s.prow + jj;
-    const T* b = r + (size_t)port * 6 * P;
+    const size_t b = (size_t)port * 6 * P + (size_t)ii * s.prow + jj;
     switch (comp) {
-      case 0: return T(0.5) * (b[rc] + b[rc + s.prow]);                              // Ex, over i
-      case 1: return T(0.5) * (b[P + rc] + b[P + rc + 1]);                           // Ey, over j
-      case 2: return T(0.25) * (b[2 * P + rc] + b[2 * P + rc + 1] + b[3 * P + rc] + b[3 * P + rc + 1]);
-      default: return T(0.25) * (b[4 * P + rc] + b[4 * P + rc + s.prow] + b[5 * P + rc] + b[5 * P + rc + s.prow]);
+      case 0: o[0] = b; o[1] = b + s.prow; return 2;                                 // Ex, over i
+      case 1: o[0] = P + b; o[1] = P + b + 1; return 2;                              // Ey, over j
+      case 2: o[0] = 2 * P + b; o[1] = 2 * P + b + 1; o[2] = 3 * P + b; o[3] = 3 * P + b + 1; return 4;
+      default:
+        o[0] = 4 * P + b; o[1] = 4 * P + b + s.prow; o[2] = 5 * P + b; o[3] = 5 * P + b + s.prow;
+        return 4;
     }
   }
   const int port = q >> 1, comp = q & 1;
-  const T* b = r + (size_t)port * 3 * P;
+  const size_t b = (size_t)port * 3 * P + c;
   if (s.scheme == SCHEME_TMZ) {
-    if (comp == 0) return b[c];                         // Ez[ic, j]
-    return T(0.5) * (b[P + c] + b[2 * P + c]);          // 0.5 (Hy[ic,j] + Hy[ic+1,j])
+    if (comp == 0) { o[0] = b; return 1; }                   // Ez[ic, j]
+    o[0] = P + b; o[1] = 2 * P + b; return 2;                // 0.5 (Hy[ic,j] + Hy[ic+1,j])
   }
-  if (comp == 0) return T(0.5) * (b[c] + b[c + 1]);     // 0.5 (Ey[ic,j] + Ey[ic,j+1])
-  return T(0.25) * (b[P + c] + b[2 * P + c] + b[P + c + 1] + b[2 * P + c + 1]);
+  if (comp == 0) { o[0] = b; o[1] = b + 1; return 2; }      // 0.5 (Ey[ic,j] + Ey[ic,j+1])
+  o[0] = P + b; o[1] = 2 * P + b; o[2] = P + b + 1; o[3] = 2 * P + b + 1;
+  return 4;
+}
+
+// The chunk's samples smp[t], t < tv, of one cell (zero from tv on).  Every
+// step's values load without a branch (steps past tv read step tv - 1 and are
+// discarded), so all of them are in flight together.
+template <typename T, int K>
+__device__ __forceinline__ void gather_samples(const T* __restrict__ ring, size_t stride, const size_t (&o)[4],
+                                               int tv, T (&smp)[kChunk]) {
+#pragma unroll
+  for (int t = 0; t < kChunk; ++t) {
+    const T* r = ring + (size_t)min(t, tv - 1) * stride;
+    T v = r[o[0]];
+    if constexpr (K > 1) v = v + r[o[1]];
+    if constexpr (K > 2) v = (v + r[o[2]]) + r[o[3]];
+    if constexpr (K == 2) v = T(0.5) * v;
+    if constexpr (K == 4) v = T(0.25) * v;
+    smp[t] = t < tv ? v : T(0);
+  }
 }
 
 template <typename T>
@@ -3512,27 +3534,61 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
     const bool hq = (s.scheme == SCHEME_3D) ? ((q & 3) >= 2) : ((q & 1) == 1);
     const double phi = hq ? 0.5 : 0.0;
     const int f0 = wk.w & 0xffff, fbn = wk.w >> 16;
+    // the samples load first (cells past the window read its last cell and
+    // fold nothing), in flight while the phase table is formed
+    const int c = wk.z * kThreads + threadIdx.x;
+    size_t o[4];
+    const int k = sample_taps(s, q, min(c, s.wcells - 1), o);
+    const size_t stride = (size_t)s.nplanes * s.pcells;
+    T smp[kChunk];
+    if (k == 1)
+      gather_samples<T, 1>(s.ring, stride, o, tv, smp);
+    else if (k == 2)
+      gather_samples<T, 2>(s.ring, stride, o, tv, smp);
+    else
+      gather_samples<T, 4>(s.ring, stride, o, tv, smp);
     __syncthreads();
     for (int e = threadIdx.x; e < fbn * kChunk; e += kThreads) {
       const int fl = e / kChunk, t = e - fl * kChunk;
       const int f = f0 + fl;
-      double c = 0.0, sv = 0.0;
+      double cv = 0.0, sv = 0.0;
       if (f < s.nf && t < tv) {
         const double ph = (s.w[f] * ((double)(n0 + t) + phi)) * s.dt;
-        sincos(ph, &sv, &c);
+        sincos(ph, &sv, &cv);
       }
-      cs[fl][t] = c;
+      cs[fl][t] = cv;
       sn[fl][t] = sv;
     }
     __syncthreads();
-    const int c = wk.z * kThreads + threadIdx.x;
     if (c >= s.wcells) continue;
-    T smp[kChunk];
-#pragma unroll
-    for (int t = 0; t < kChunk; ++t)
-      smp[t] = (t < tv) ? sample_at<T>(s, s.ring + (size_t)t * s.nplanes * s.pcells, q, c) : T(0);
     const int fend = min(fbn, s.nf - f0);
-    for (int fl = 0; fl < fend; ++fl) {
+    double* a = s.acc + (((size_t)q * s.nf + f0) * s.wcells + c) * 2;
+    const size_t fs = (size_t)s.wcells * 2;   // between frequencies
+    // four frequencies at a time: their accumulators load before the sums
+    int fl = 0;
+    for (; fl + 4 <= fend; fl += 4, a += 4 * fs) {
+      double ar[4], ai[4], re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        ar[u] = a[u * fs];
+        ai[u] = a[u * fs + 1];
+      }
+#pragma unroll
+      for (int t = 0; t < kChunk; ++t) {
+        const double v = (double)smp[t];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          re[u] += v * cs[fl + u][t];
+          im[u] += v * sn[fl + u][t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u * fs] = ar[u] + re[u];
+        a[u * fs + 1] = ai[u] - im[u];
+      }
+    }
+    for (; fl < fend; ++fl, a += fs) {
       double re = 0.0, im = 0.0;
 #pragma unroll
       for (int t = 0; t < kChunk; ++t) {
@@ -3540,7 +3596,6 @@ __global__ __launch_bounds__(kThreads) void k_dft(const Slot<T>* __restrict__ sl
         re += v * cs[fl][t];
         im += v * sn[fl][t];
       }
-      double* a = s.acc + (((size_t)q * s.nf + f0 + fl) * s.wcells + c) * 2;
       a[0] += re;
       a[1] -= im;
     }
